@@ -1,0 +1,228 @@
+"""ctypes bindings of libcof (include/cof.h) and libcofsynth for tests and benchmarks.
+
+The search itself runs in libcof.so (HIP kernels for gfx950 behind a C ABI);
+this module only marshals arguments.  Loading fails loudly when the built
+libraries are missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+COF_OK, COF_E_INVALID, COF_E_HIP, COF_E_OVERFLOW = 0, 1, 2, 3
+COF_E_IO, COF_E_NOGENOME, COF_E_NOMEM, COF_E_UNSUPPORTED = 4, 5, 6, 7
+
+# Every symbol include/cof.h declares (checked by the CPU test suite).
+COF_SYMBOLS = ["cof_open", "cof_load_genome_file", "cof_load_genome", "cof_search",
+               "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error"]
+
+
+class CofQuery(C.Structure):
+    _fields_ = [("n_guides", C.c_uint32),
+                ("guides", C.POINTER(C.c_char_p)),
+                ("filters", C.POINTER(C.c_char_p)),
+                ("max_mismatches", C.POINTER(C.c_uint32)),
+                ("max_dna_bulges", C.POINTER(C.c_uint32)),
+                ("max_rna_bulges", C.POINTER(C.c_uint32)),
+                ("batch_nucl", C.c_uint64)]
+
+
+class CofChunk(C.Structure):
+    _fields_ = [("chrom", C.c_uint32), ("_pad", C.c_uint32), ("start", C.c_uint64),
+                ("end", C.c_uint64), ("byte_offset", C.c_uint64)]
+
+
+class CofHit(C.Structure):
+    _fields_ = [("chrom_pos", C.c_uint64), ("chrom", C.c_uint32), ("pattern", C.c_uint32),
+                ("mismatches", C.c_uint16), ("dna_bulge", C.c_uint16), ("rna_bulge", C.c_uint16),
+                ("n_ops", C.c_uint16), ("ops", C.c_uint64 * 3)]
+
+
+class CofStats(C.Structure):
+    _fields_ = [("search_ms", C.c_double), ("kernel_ms_plain", C.c_double),
+                ("kernel_ms_filter", C.c_double), ("kernel_ms_verify", C.c_double),
+                ("n_plain_launches", C.c_uint64), ("n_candidates", C.c_uint64),
+                ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64)]
+
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """libcof.so with argument types set; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(BUILD, "libcof.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libcof.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.cof_open.argtypes = [C.c_int, C.POINTER(CofQuery), C.POINTER(C.c_void_p)]
+        L.cof_load_genome_file.argtypes = [C.c_void_p, C.c_char_p]
+        L.cof_load_genome.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(CofChunk), C.c_uint64,
+                                      C.POINTER(C.c_char_p), C.c_uint32]
+        L.cof_search.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
+        L.cof_write_tsv.argtypes = [C.c_void_p, C.c_char_p]
+        L.cof_get_stats.argtypes = [C.c_void_p, C.POINTER(CofStats)]
+        L.cof_close.argtypes = [C.c_void_p]
+        L.cof_strerror.argtypes = [C.c_int]
+        L.cof_strerror.restype = C.c_char_p
+        L.cof_last_error.argtypes = [C.c_void_p]
+        L.cof_last_error.restype = C.c_char_p
+        for f in ("cof_open", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
+                  "cof_get_stats"):
+            getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        path = os.path.join(BUILD, "libcofsynth.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libcofsynth.so is not built (run __graft_entry__.build())")
+        S = C.CDLL(path)
+        S.cofsynth_new.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64), C.c_uint32]
+        S.cofsynth_new.restype = C.c_void_p
+        S.cofsynth_set_n.argtypes = [C.c_void_p, C.c_double, C.c_uint64, C.c_uint64]
+        S.cofsynth_set_repeats.argtypes = [C.c_void_p, C.c_double, C.c_uint32, C.c_uint32, C.c_double]
+        S.cofsynth_set_iupac.argtypes = [C.c_void_p, C.c_double]
+        S.cofsynth_plant.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        S.cofsynth_build.argtypes = [C.c_void_p]
+        S.cofsynth_packed_bytes.argtypes = [C.c_void_p]
+        S.cofsynth_packed_bytes.restype = C.c_uint64
+        S.cofsynth_packed.argtypes = [C.c_void_p]
+        S.cofsynth_packed.restype = C.c_void_p
+        S.cofsynth_n_chunks.argtypes = [C.c_void_p]
+        S.cofsynth_n_chunks.restype = C.c_uint64
+        S.cofsynth_chunks.argtypes = [C.c_void_p]
+        S.cofsynth_chunks.restype = C.POINTER(CofChunk)
+        S.cofsynth_n_chroms.argtypes = [C.c_void_p]
+        S.cofsynth_n_chroms.restype = C.c_uint32
+        S.cofsynth_chrom_name.argtypes = [C.c_void_p, C.c_uint32]
+        S.cofsynth_chrom_name.restype = C.c_char_p
+        S.cofsynth_write_fasta.argtypes = [C.c_void_p, C.c_char_p]
+        S.cofsynth_free.argtypes = [C.c_void_p]
+        _synth = S
+    return _synth
+
+
+class SynthGenome:
+    """Seeded synthetic genome (see csrc/synth.cpp for the model)."""
+
+    def __init__(self, seed, chrom_lens, line_width=60, n_frac=0.0, gap=(100, 100), repeats=None,
+                 iupac_frac=0.0, plants=()):
+        S = synth_lib()
+        arr = (C.c_uint64 * len(chrom_lens))(*chrom_lens)
+        self.h = S.cofsynth_new(seed, len(chrom_lens), arr, line_width)
+        if not self.h:
+            raise ValueError("bad synthetic genome spec")
+        if n_frac:
+            S.cofsynth_set_n(self.h, n_frac, gap[0], gap[1])
+        if repeats:
+            S.cofsynth_set_repeats(self.h, *repeats)
+        if iupac_frac:
+            S.cofsynth_set_iupac(self.h, iupac_frac)
+        for site, goff, glen, subs, count in plants:
+            if S.cofsynth_plant(self.h, site.encode(), goff, glen, subs, count) != 0:
+                raise ValueError("bad plant spec")
+        S.cofsynth_build(self.h)
+
+    def write_fasta(self, path):
+        if synth_lib().cofsynth_write_fasta(self.h, str(path).encode()) != 0:
+            raise IOError(path)
+
+    def layout(self):
+        S = synth_lib()
+        n = S.cofsynth_n_chunks(self.h)
+        names = [S.cofsynth_chrom_name(self.h, i) for i in range(S.cofsynth_n_chroms(self.h))]
+        return S.cofsynth_packed(self.h), S.cofsynth_packed_bytes(self.h), S.cofsynth_chunks(self.h), n, names
+
+    def packed_bytes(self):
+        ptr, nb, _, _, _ = self.layout()
+        return C.string_at(ptr, nb)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _synth is not None:
+            _synth.cofsynth_free(self.h)
+            self.h = None
+
+
+class CofError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{lib().cof_strerror(status).decode()}: {msg}")
+        self.status = status
+
+
+class Search:
+    """One libcof context: query -> resident genome -> hits."""
+
+    def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0):
+        L = lib()
+        n = len(guides)
+        self._keep = [g.encode() for g in guides], [f.encode() for f in filters]
+        q = CofQuery()
+        q.n_guides = n
+        q.guides = (C.c_char_p * n)(*self._keep[0])
+        q.filters = (C.c_char_p * n)(*self._keep[1])
+        q.max_mismatches = (C.c_uint32 * n)(*mms)
+        q.max_dna_bulges = (C.c_uint32 * n)(*(dnas or [0] * n))
+        q.max_rna_bulges = (C.c_uint32 * n)(*(rnas or [0] * n))
+        q.batch_nucl = batch_nucl
+        self._q = q
+        h = C.c_void_p()
+        st = L.cof_open(device, C.byref(q), C.byref(h))
+        if st != COF_OK:
+            raise CofError(st, "cof_open")
+        self.h = h
+
+    def _check(self, st, what):
+        if st != COF_OK:
+            raise CofError(st, f"{what}: {lib().cof_last_error(self.h).decode()}")
+
+    def load_file(self, path):
+        self._check(lib().cof_load_genome_file(self.h, str(path).encode()), "load_genome_file")
+
+    def load_synth(self, sg):
+        ptr, nb, chunks, n, names = sg.layout()
+        arr = (C.c_char_p * len(names))(*names)
+        self._check(lib().cof_load_genome(self.h, ptr, nb, chunks, n, arr, len(names)), "load_genome")
+
+    def search(self):
+        n = C.c_uint64()
+        st = lib().cof_search(self.h, None, 0, C.byref(n))
+        if st not in (COF_OK, COF_E_OVERFLOW):
+            self._check(st, "search")
+        return n.value
+
+    def hits(self):
+        n = self.search()
+        buf = (CofHit * max(n, 1))()
+        self._check(lib().cof_search(self.h, buf, n, None), "search")
+        return buf[:n]
+
+    def write_tsv(self, path):
+        self._check(lib().cof_write_tsv(self.h, str(path).encode()), "write_tsv")
+
+    def stats(self):
+        s = CofStats()
+        lib().cof_get_stats(self.h, C.byref(s))
+        return s
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cof_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def read_rows(path):
+    """Sorted data rows of a reference-format TSV (header dropped)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    assert lines and lines[0].startswith("#Bulge type"), "missing header"
+    return sorted(lines[1:])

@@ -1,0 +1,204 @@
+// C ABI of libcof (include/cof.h): context, genome residency, search driver.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <memory>
+
+#include "cof_internal.hpp"
+#include "engine.hpp"
+#include "hip_check.hpp"
+
+struct cof_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cof::Query q;
+    cof::Genome g;
+    std::vector<cof::Batch> batches;
+    uint32_t* d_genome = nullptr;
+    uint64_t d_genome_words = 0;
+    cof::PlainPlan* plain = nullptr;
+    cof::BulgePlan* bulge = nullptr;
+    bool have_genome = false;
+    bool cached = false;
+    std::vector<cof_hit> hits;
+    std::vector<cof::HitAux> aux;
+    cof_stats stats{};
+    std::string err;
+};
+
+namespace {
+
+cof_status fail(cof_ctx* ctx, const cof::Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+}
+
+void upload_genome(cof_ctx* ctx) {
+    if (ctx->d_genome) { (void)hipFree(ctx->d_genome); ctx->d_genome = nullptr; }
+    uint64_t words = (ctx->g.n_bytes + 3) / 4;
+    uint64_t tile = cof::plain_tile_nucl();
+    uint64_t n_tiles = (ctx->g.n_nibbles() + tile - 1) / tile + 1;
+    uint64_t alloc_words = std::max<uint64_t>(words, n_tiles * tile / 8) + cof::plain_pad_words() + cof::bulge_pad_words();
+    HIPCHECK(hipMalloc(&ctx->d_genome, alloc_words * 4));
+    HIPCHECK(hipMemsetAsync(ctx->d_genome, 0, alloc_words * 4, ctx->stream));
+    if (ctx->g.n_bytes)
+        HIPCHECK(hipMemcpyAsync(ctx->d_genome, ctx->g.packed.data(), ctx->g.n_bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->d_genome_words = alloc_words;
+    ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
+    ctx->have_genome = true;
+    ctx->cached = false;
+    ctx->stats.genome_nucl = 0;
+    for (auto& c : ctx->g.chunks) ctx->stats.genome_nucl += c.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out) {
+    if (!out) return COF_E_INVALID;
+    *out = nullptr;
+    auto ctx = std::make_unique<cof_ctx>();
+    try {
+        ctx->device = hip_device;
+        HIPCHECK(hipSetDevice(hip_device));
+        HIPCHECK(hipStreamCreate(&ctx->stream));
+        ctx->q = cof::prepare_query(q);
+        ctx->plain = cof::plain_plan_create(ctx->q);
+        ctx->bulge = cof::bulge_plan_create(ctx->q);
+    } catch (const cof::Error& e) {
+        cof_status s = e.code;
+        cof_close(ctx.release());
+        return s;
+    }
+    *out = ctx.release();
+    return COF_OK;
+}
+
+cof_status cof_load_genome_file(cof_ctx* ctx, const char* path) {
+    if (!ctx || !path) return COF_E_INVALID;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        cof::read_genome_file(path, ctx->g);
+        upload_genome(ctx);
+    } catch (const cof::Error& e) {
+        ctx->have_genome = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
+                           uint64_t n_chunks, const char* const* names, uint32_t n_chroms) {
+    if (!ctx || (!packed && n_bytes) || (!chunks && n_chunks) || (!names && n_chroms)) return COF_E_INVALID;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        cof::Genome g;
+        for (uint32_t i = 0; i < n_chroms; i++) g.chrom_names.push_back(names[i] ? names[i] : "");
+        uint64_t expect = 0;
+        for (uint64_t i = 0; i < n_chunks; i++) {
+            const cof_chunk& c = chunks[i];
+            if (c.chrom >= n_chroms || c.end < c.start || c.end - c.start > cof::kChunkNucl || c.byte_offset != expect)
+                throw cof::Error(COF_E_INVALID, "chunk table is not a tight packing of <=64 Knt chunks");
+            if (i > 0 && !(chunks[i - 1].end == c.start || c.start == 0))
+                throw cof::Error(COF_E_INVALID, "chromosome chunks out of order");
+            g.chunks.push_back({c.chrom, c.start, c.end, c.byte_offset});
+            expect += (c.end - c.start + 1) / 2;
+        }
+        if (expect != n_bytes) throw cof::Error(COF_E_INVALID, "packed size does not match the chunk table");
+        g.n_bytes = n_bytes;
+        g.packed.assign(packed, packed + n_bytes);
+        g.packed.resize(n_bytes + 256, 0);
+        ctx->g = std::move(g);
+        upload_genome(ctx);
+    } catch (const cof::Error& e) {
+        ctx->have_genome = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits) {
+    if (!ctx) return COF_E_INVALID;
+    if (!ctx->have_genome) return COF_E_NOGENOME;
+    try {
+        if (!ctx->cached) {
+            HIPCHECK(hipSetDevice(ctx->device));
+            auto t0 = std::chrono::steady_clock::now();
+            cof_stats& st = ctx->stats;
+            uint64_t genome_nucl = st.genome_nucl;
+            st = cof_stats{};
+            st.genome_nucl = genome_nucl;
+            ctx->hits.clear();
+            ctx->aux.clear();
+            std::vector<cof::PlainRaw> praw;
+            cof::plain_run(ctx->plain, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw, &st.kernel_ms_plain,
+                           &st.n_plain_launches);
+            std::vector<cof::BulgeRaw> braw;
+            cof::bulge_run(ctx->bulge, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
+            st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            st.n_raw_hits = praw.size() + braw.size();
+            cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
+            cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
+            st.n_hits = ctx->hits.size();
+            ctx->cached = true;
+        }
+        if (n_hits) *n_hits = ctx->hits.size();
+        if (cap < ctx->hits.size()) {
+            if (out && cap) memcpy(out, ctx->hits.data(), cap * sizeof(cof_hit));
+            return COF_E_OVERFLOW;
+        }
+        if (out && !ctx->hits.empty()) memcpy(out, ctx->hits.data(), ctx->hits.size() * sizeof(cof_hit));
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_write_tsv(cof_ctx* ctx, const char* path) {
+    if (!ctx || !path) return COF_E_INVALID;
+    if (!ctx->cached) return COF_E_INVALID;
+    FILE* f = strcmp(path, "-") == 0 ? stdout : fopen(path, "wb");
+    if (!f) { ctx->err = std::string("cannot open ") + path; return COF_E_IO; }
+    std::vector<char> buf(1 << 22);
+    setvbuf(f, buf.data(), _IOFBF, buf.size());
+    cof::write_tsv(ctx->g, ctx->q, ctx->hits, ctx->aux, f);
+    if (f != stdout) fclose(f); else { fflush(f); setvbuf(f, nullptr, _IOLBF, 0); }
+    return COF_OK;
+}
+
+cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out) {
+    if (!ctx || !out) return COF_E_INVALID;
+    *out = ctx->stats;
+    return COF_OK;
+}
+
+void cof_close(cof_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    cof::plain_plan_destroy(ctx->plain);
+    cof::bulge_plan_destroy(ctx->bulge);
+    if (ctx->d_genome) (void)hipFree(ctx->d_genome);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* cof_strerror(cof_status s) {
+    switch (s) {
+        case COF_OK: return "ok";
+        case COF_E_INVALID: return "invalid argument or query";
+        case COF_E_HIP: return "HIP runtime error";
+        case COF_E_OVERFLOW: return "output buffer too small";
+        case COF_E_IO: return "genome file unreadable or malformed";
+        case COF_E_NOGENOME: return "no genome loaded";
+        case COF_E_NOMEM: return "out of memory";
+        case COF_E_UNSUPPORTED: return "query not supported by the kernels";
+        default: return "unknown status";
+    }
+}
+
+const char* cof_last_error(const cof_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+}  // extern "C"

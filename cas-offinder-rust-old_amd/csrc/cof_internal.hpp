@@ -1,0 +1,132 @@
+// Internal declarations shared by the host side of libcof (C++17) and the HIP
+// translation units.  Nothing here is part of the C ABI (include/cof.h).
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cof.h"
+
+namespace cof {
+
+// ---------------------------------------------------------------------------
+// bit4 nucleotide code (same table values as cas-offinder-lib/src/bit4ops.rs
+// :1-121, because the packed genome layout is part of the boundary):
+// T=1 C=2 A=4 G=8, IUPAC = OR, N = 0xF, lowercase == uppercase, other = 0.
+// ---------------------------------------------------------------------------
+struct Codec {
+    uint8_t enc_mixed[256];   // any IUPAC letter
+    uint8_t enc_plain[256];   // A/C/G/T only (cmp_chars' DNA side)
+    char dec[16];             // bit4 -> uppercase letter, 0 -> NUL
+    Codec();
+};
+extern const Codec kCodec;
+
+inline uint8_t nib(const uint8_t* packed, uint64_t i) {
+    uint8_t b = packed[i >> 1];
+    return (i & 1) ? (uint8_t)(b >> 4) : (uint8_t)(b & 0xF);
+}
+inline uint8_t comp4(uint8_t v) { return (uint8_t)(((v << 2) | (v >> 2)) & 0xF); }
+char comp_char(char c);                                   // keeps case, maps IUPAC
+std::string revcomp(const std::string& s);
+inline bool is_n_char(char c) { return c == 'N' || c == 'n'; }
+// cmp_chars(dna, rna) of bit4ops.rs:233-237 on characters
+inline bool cmp_chars(uint8_t dna, uint8_t rna) { return (kCodec.enc_plain[dna] & kCodec.enc_mixed[rna]) != 0; }
+// one-bit (A/C/G/T) nibble test
+inline bool single_base(uint8_t v) { return v != 0 && (v & (v - 1)) == 0; }
+
+struct Error : std::runtime_error {
+    cof_status code;
+    Error(cof_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---------------------------------------------------------------------------
+// Resident genome: tight-packed 64 Knt chunks (search.rs:1376-1427 layout).
+// ---------------------------------------------------------------------------
+static const uint64_t kChunkNucl = 1u << 16;
+
+struct Chunk {
+    uint32_t chrom;
+    uint64_t start, end;      // within chromosome
+    uint64_t byte_off;        // in packed buffer
+    uint64_t size() const { return end - start; }
+    uint64_t nib_begin() const { return byte_off * 2; }
+};
+
+struct Genome {
+    std::vector<std::string> chrom_names;
+    std::vector<Chunk> chunks;
+    std::vector<uint8_t> packed;   // host copy, + zero padding
+    uint64_t n_bytes = 0;          // valid bytes
+    uint64_t n_nibbles() const { return n_bytes * 2; }
+    // chunk containing global nibble i (upper_bound over byte offsets)
+    size_t chunk_of(uint64_t nibble) const;
+};
+
+void read_genome_file(const std::string& path, Genome& g);   // FASTA / 2bit / folder
+
+// Virtual batches of the reference batcher (search.rs:1525-1572): batch b
+// holds chunks [first, last), the first being a head carried over from the
+// previous batch when has_head.
+struct Batch {
+    size_t first, last;
+    bool has_head;
+};
+std::vector<Batch> make_batches(size_t n_chunks, uint64_t batch_nucl);
+
+// ---------------------------------------------------------------------------
+// Query preparation.
+// ---------------------------------------------------------------------------
+struct Pattern {
+    std::string ascii;          // as given (forward) or reverse complement
+    std::string eff_filter;     // filter (forward) or revcomp(filter) (search.rs:1213-1225)
+    std::vector<uint8_t> b4;    // one nibble per position
+    uint32_t L, M, D, R;
+    uint32_t guide;             // forward guide index
+    bool forward;
+    uint32_t pam_len_out;       // trailing Ns of the first guide of its run (main.rs:80-91)
+    bool bulge;                 // D>0 || R>0
+};
+
+struct Query {
+    std::vector<Pattern> pats;  // n fwd then n rev
+    uint32_t n_guides = 0;
+    uint64_t batch_nucl = 1ull << 30;
+    bool any_plain = false, any_bulge = false;
+};
+Query prepare_query(const cof_query* q);
+
+// Raw device hit of the plain scan: global start nibble + pattern.
+struct PlainRaw {
+    uint64_t pos;
+    uint32_t pattern;
+    uint32_t flags;
+};
+
+// Raw device hit of the bulge path: alignment start nibble, pattern, counts, ops.
+struct BulgeRaw {
+    uint64_t start;
+    uint32_t pattern;
+    uint8_t mm, db, rb, n_ops;
+    uint64_t ops[3];          // forward order, 2 bits per op
+};
+
+// Host-side conversion (chunk lookup, batch-seam and chromosome-end drop
+// rules, writer predicate) of raw device hits into final hits.
+// Per-hit data the row writer needs beyond cof_hit: the global start nibble
+// and the end of the batch view the hit was reported from.
+struct HitAux {
+    uint64_t nib;
+    uint64_t view_end;
+};
+void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& batches,
+                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux);
+void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches,
+                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux);
+void write_tsv(const Genome& g, const Query& q, const std::vector<cof_hit>& hits,
+               const std::vector<HitAux>& aux, FILE* f);
+
+}  // namespace cof

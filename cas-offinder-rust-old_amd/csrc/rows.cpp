@@ -1,0 +1,210 @@
+// Host-side conversion of raw device hits into reference output rows.
+//
+// Plain path: the device emits every (start, pattern) whose window passes the
+// writer predicate on the resident genome.  Here each hit is placed in the
+// virtual batch the reference would have reported it from (search.rs:1525-1572
+// batcher, :540-546 head-overlap start limit, :1449-1454 drop rule), the
+// predicate is re-evaluated on that batch's view (bytes past the batch end
+// read as 0, as the reference's zero-filled batch buffer does), and the row
+// is formatted as main.rs:158-242 does.
+//
+// Bulge path: the device already scanned per virtual batch (window floor and
+// PAM limit); here the chromosome-end drop rule of convert_matches_myers
+// (search.rs:1316-1374) is applied and rows are formatted (main.rs:145-157).
+#include <algorithm>
+#include <cstdio>
+
+#include "cof_internal.hpp"
+
+namespace cof {
+
+namespace {
+
+struct BatchIndex {
+    std::vector<int64_t> nonhead_batch;    // per chunk
+    std::vector<uint64_t> end_nib;         // per batch: end of its last chunk's bytes (nibbles)
+    std::vector<uint64_t> active_begin;    // per batch: first end position scanned (bulge path)
+    BatchIndex(const Genome& g, const std::vector<Batch>& bs) {
+        nonhead_batch.assign(g.chunks.size(), -1);
+        for (size_t b = 0; b < bs.size(); b++) {
+            for (size_t c = bs[b].first + (bs[b].has_head ? 1 : 0); c < bs[b].last; c++) nonhead_batch[c] = (int64_t)b;
+            uint64_t e = bs[b].last < g.chunks.size() ? g.chunks[bs[b].last].byte_off : g.n_bytes;
+            end_nib.push_back(e * 2);
+            active_begin.push_back(g.chunks[bs[b].first + (bs[b].has_head ? 1 : 0)].byte_off * 2);
+        }
+    }
+};
+
+// Writer predicate of main.rs:158-194 evaluated in pattern orientation on the
+// N-masked window (search.rs:512-531), with nibbles at or past view_end = 0.
+bool plain_predicate(const Genome& g, const Pattern& p, uint64_t pos, uint64_t view_end, uint32_t* mm_out) {
+    int32_t pop = 0;
+    uint32_t amm = 0;
+    for (uint32_t k = 0; k < p.L; k++) {
+        uint64_t i = pos + k;
+        uint8_t v = i < view_end ? nib(g.packed.data(), i) : 0;
+        if (v == 0xF) v = 0;
+        uint8_t pb = p.b4[k];
+        pop += __builtin_popcount(pb & v);
+        uint8_t fc = (uint8_t)p.eff_filter[k];
+        bool sb = single_base(v);
+        if (!is_n_char((char)fc) && !(sb && (v & kCodec.enc_mixed[fc]))) return false;
+        if (pb != 0xF && !(sb && (v & pb))) amm++;
+    }
+    int32_t kmm = (int32_t)p.L - pop;
+    if (kmm > (int32_t)p.M || amm > p.M) return false;
+    *mm_out = amm;
+    return true;
+}
+
+}  // namespace
+
+void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs,
+                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux) {
+    BatchIndex bi(g, bs);
+    for (const auto& r : raw) {
+        const Pattern& p = q.pats[r.pattern];
+        size_t c = g.chunk_of(r.pos);
+        const Chunk& ch = g.chunks[c];
+        uint64_t chrom_pos = ch.start + (r.pos - ch.nib_begin());
+        bool past_end = chrom_pos + p.L > ch.end;
+        int64_t b = bi.nonhead_batch[c];
+        if (b < 0) continue;
+        bool last_in_b = c + 1 == bs[(size_t)b].last;
+        bool next_new_chrom = c + 1 < g.chunks.size() && g.chunks[c + 1].start == 0;
+        uint32_t mm = 0;
+        uint64_t view_end = 0;
+        bool emit = false;
+        // reported by batch b, where c is not the head
+        if (!((last_in_b || next_new_chrom) && past_end)) {
+            view_end = bi.end_nib[(size_t)b];
+            emit = plain_predicate(g, p, r.pos, view_end, &mm);
+        }
+        // reported by batch b+1, where c is the head (search.rs:540-546); the
+        // two cases exclude each other (the second needs past_end)
+        if (!emit && last_in_b && (size_t)b + 1 < bs.size()) {
+            uint64_t c1 = g.chunks[c + 1].byte_off * 2 - ch.nib_begin();
+            uint64_t min_start = c1 >= p.L - 1 ? c1 - (p.L - 1) : 0;
+            uint64_t local = r.pos - ch.nib_begin();
+            if (local >= min_start && !(next_new_chrom && past_end)) {
+                view_end = bi.end_nib[(size_t)b + 1];
+                emit = plain_predicate(g, p, r.pos, view_end, &mm);
+            }
+        }
+        if (!emit) continue;
+        cof_hit h{};
+        h.chrom_pos = chrom_pos;
+        h.chrom = ch.chrom;
+        h.pattern = r.pattern;
+        h.mismatches = (uint16_t)mm;
+        out.push_back(h);
+        aux.push_back({r.pos, view_end});
+    }
+}
+
+void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs,
+                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux) {
+    BatchIndex bi(g, bs);
+    for (const auto& r : raw) {
+        const Pattern& p = q.pats[r.pattern];
+        uint64_t span = (uint64_t)p.L + r.db - r.rb;
+        uint64_t j = r.start + span - 1;
+        // batch whose scanned end range holds j
+        size_t b = (size_t)(std::upper_bound(bi.active_begin.begin(), bi.active_begin.end(), j) - bi.active_begin.begin());
+        if (b == 0) continue;
+        b -= 1;
+        size_t c = g.chunk_of(r.start);
+        if (c < bs[b].first) c = bs[b].first;
+        const Chunk& ch = g.chunks[c];
+        uint64_t chrom_pos = ch.start + (r.start - ch.nib_begin());
+        bool is_end_chrom = c + 1 == bs[b].last || g.chunks[c + 1].start == 0;
+        bool past_end = chrom_pos + span > ch.end;
+        if (is_end_chrom && past_end) continue;
+        cof_hit h{};
+        h.chrom_pos = chrom_pos;
+        h.chrom = ch.chrom;
+        h.pattern = r.pattern;
+        h.mismatches = r.mm;
+        h.dna_bulge = r.db;
+        h.rna_bulge = r.rb;
+        h.n_ops = r.n_ops;
+        h.ops[0] = r.ops[0]; h.ops[1] = r.ops[1]; h.ops[2] = r.ops[2];
+        out.push_back(h);
+        aux.push_back({r.start, bi.end_nib[b]});
+    }
+}
+
+// Rows of main.rs:130-247.
+void write_tsv(const Genome& g, const Query& q, const std::vector<cof_hit>& hits,
+               const std::vector<HitAux>& aux, FILE* f) {
+    fputs("#Bulge type\tcrRNA\tDNA\tChromosome\tPosition\tDirection\tMismatches\tBulge Size\n", f);
+    std::string line, rna, dna;
+    for (size_t hi = 0; hi < hits.size(); hi++) {
+        const cof_hit& h = hits[hi];
+        const Pattern& p = q.pats[h.pattern];
+        rna.clear();
+        dna.clear();
+        const char* type = "X";
+        uint32_t bsize = 0;
+        if (!p.bulge) {
+            uint64_t pos = aux[hi].nib, view_end = aux[hi].view_end;
+            for (uint32_t k = 0; k < p.L; k++) {
+                uint64_t i = pos + k;
+                uint8_t v = i < view_end ? nib(g.packed.data(), i) : 0;
+                if (v == 0xF) v = 0;
+                dna.push_back(kCodec.dec[v]);
+                rna.push_back(kCodec.dec[p.b4[k]]);
+            }
+            if (!p.forward) {
+                std::reverse(dna.begin(), dna.end());
+                for (auto& c : dna) c = comp_char(c);
+                std::reverse(rna.begin(), rna.end());
+                for (auto& c : rna) c = comp_char(c);
+            }
+            for (size_t k = 0; k < dna.size(); k++) {         // main.rs:198-217
+                char& d = dna[k];
+                bool rn = is_n_char(rna[k]);
+                if (d == 0) d = rn ? 'N' : 'n';
+                else if (rn) {}
+                else if (is_n_char(d)) d = 'n';
+                else if (!cmp_chars((uint8_t)d, (uint8_t)rna[k])) d = (char)(d | 0x20);
+            }
+        } else {
+            // aligned strings from the ops (search.rs:152-206 / traceback.rs:355-386)
+            uint64_t t = aux[hi].nib;
+            uint32_t pi = 0;
+            for (uint32_t k = 0; k < h.n_ops; k++) {
+                uint32_t op = (uint32_t)(h.ops[k / 32] >> (2 * (k % 32))) & 3;
+                if (op <= 1) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
+                else if (op == 3) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back('-'); }
+                else { rna.push_back('-'); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
+            }
+            if (!p.forward) {
+                std::reverse(dna.begin(), dna.end());
+                for (auto& c : dna) c = comp_char(c);
+                std::reverse(rna.begin(), rna.end());
+                for (auto& c : rna) c = comp_char(c);
+            }
+            uint32_t tot = h.dna_bulge + h.rna_bulge;
+            type = tot == 0 ? "X" : h.dna_bulge > 0 ? "DNA" : "RNA";
+            bsize = tot;
+        }
+        uint64_t shift = 0;
+        if (!p.forward) {
+            uint64_t a = p.pam_len_out > 0 ? p.pam_len_out - 1 : 0;
+            shift = a > h.dna_bulge ? a - h.dna_bulge : 0;
+        }
+        line.clear();
+        line += type; line += '\t';
+        line += rna; line += '\t';
+        line.append(dna.data(), dna.size()); line += '\t';
+        line += g.chrom_names[h.chrom]; line += '\t';
+        line += std::to_string(h.chrom_pos + shift); line += '\t';
+        line += p.forward ? '+' : '-'; line += '\t';
+        line += std::to_string(h.mismatches); line += '\t';
+        line += std::to_string(bsize); line += '\n';
+        fwrite(line.data(), 1, line.size(), f);
+    }
+}
+
+}  // namespace cof

@@ -1,0 +1,137 @@
+/*
+ * cof.h — C ABI of the MI355X (gfx950) off-target search library (libcof.so).
+ *
+ * The library replaces the GPU backend of cas-offinder-lib: the batched
+ * search behind the public `search()` entry point
+ * (cas-offinder-lib/src/search.rs:1490-1663), i.e. the backend seam
+ * `search_chunk_cuda` / `search_chunk_cuda_myers` (search.rs:322-366,
+ * 1193-1314) together with the host code whose output defines the hit set:
+ * `convert_matches` / `convert_matches_myers` (search.rs:1316-1374,
+ * 1428-1488) and the CLI writer (cas-offinder-cli/src/main.rs:130-247).
+ *
+ * Conventions: status codes, never aborts; the caller owns host buffers, the
+ * library owns device memory; one context per device per host thread (not
+ * thread-safe), as the reference runs one host thread per device
+ * (search.rs:344-361).  Output order is unspecified; compare sorted rows.
+ */
+#ifndef COF_H
+#define COF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int cof_status;
+#define COF_OK 0
+#define COF_E_INVALID 1     /* bad query / argument (reference: CliError::ArgumentError, assert!) */
+#define COF_E_HIP 2         /* HIP runtime error (reference: DriverError -> panic!) */
+#define COF_E_OVERFLOW 3    /* caller buffer too small; *n_hits holds the count needed */
+#define COF_E_IO 4          /* genome file unreadable / malformed (reference: CliError::BadFileFormat) */
+#define COF_E_NOGENOME 5    /* cof_search before a genome was loaded */
+#define COF_E_NOMEM 6       /* device or host allocation failed */
+#define COF_E_UNSUPPORTED 7 /* query outside what the kernels implement (e.g. Myers pattern > 64 nt) */
+
+typedef struct cof_ctx cof_ctx;
+
+/*
+ * A search query.  One entry per forward guide; the library appends the
+ * reverse complement of every guide exactly as main.rs:253-259 does, so
+ * pattern indices run 0..n_guides-1 (forward) then n_guides..2*n_guides-1.
+ *
+ * Reference mode (what one reference run can express, cli_utils.rs:37-125):
+ * every guide has the same filter, length and limits.  The library also
+ * accepts per-guide filters, lengths and limits; results then equal the
+ * union of one reference run per (filter, length, M, D, R) group.
+ *
+ * Guides whose D and R are both 0 use the plain mismatch semantics
+ * (search.rs:1513, the "legacy popcount" path plus main.rs:158-222);
+ * guides with D>0 or R>0 use the bulge semantics (PAM quick check, Myers
+ * filter, all-alignment DP traceback).
+ */
+typedef struct {
+    uint32_t n_guides;
+    const char* const* guides;       /* ASCII IUPAC, guide i has length len(filters[i]) */
+    const char* const* filters;      /* ASCII IUPAC filter (PAM mask) per guide */
+    const uint32_t* max_mismatches;  /* per guide */
+    const uint32_t* max_dna_bulges;  /* per guide, NULL = all 0 */
+    const uint32_t* max_rna_bulges;  /* per guide, NULL = all 0 */
+    uint64_t batch_nucl;             /* virtual batch size in nt, multiple of 65536;
+                                        0 = 1<<30 (reference GPU, search.rs:22,44-50);
+                                        1<<22 reproduces the reference CPU path */
+} cof_query;
+
+/* One chunk of the tight-packed genome (chrom_chunk.rs:1-12, search.rs:57-79). */
+typedef struct {
+    uint32_t chrom;        /* index into chrom_names */
+    uint32_t _pad;
+    uint64_t start, end;   /* nucleotide range within the chromosome */
+    uint64_t byte_offset;  /* first byte of the chunk in the packed buffer */
+} cof_chunk;
+
+/* One hit after chunk->chromosome conversion and the writer's filters
+ * (Match, chrom_chunk.rs:15-25, with the writer's recomputed mismatches). */
+typedef struct {
+    uint64_t chrom_pos;    /* Match.chrom_idx: leftmost + strand coordinate (before the
+                              reverse-strand shift of main.rs:229-236) */
+    uint32_t chrom;        /* chromosome index */
+    uint32_t pattern;      /* pattern index (see cof_query) */
+    uint16_t mismatches;
+    uint16_t dna_bulge;
+    uint16_t rna_bulge;
+    uint16_t n_ops;        /* bulge path: alignment length; plain path: 0 */
+    uint64_t ops[3];       /* bulge path: 2 bits per op in alignment order, op k at bits
+                              2*(k%32) of ops[k/32]; 0 match, 1 mismatch, 2 DNA bulge, 3 RNA bulge */
+} cof_hit;
+
+typedef struct {
+    double search_ms;          /* wall time of the last cof_search (device work + D2H of hits) */
+    double kernel_ms_plain;    /* summed HIP-event time of plain-scan launches */
+    double kernel_ms_filter;   /* summed HIP-event time of PAM+Myers filter launches */
+    double kernel_ms_verify;   /* summed HIP-event time of DP traceback launches */
+    uint64_t n_plain_launches;
+    uint64_t n_candidates;     /* Myers survivors sent to the DP traceback */
+    uint64_t n_raw_hits;       /* hits before the host-side seam/drop rules */
+    uint64_t n_hits;           /* final hits */
+    uint64_t genome_nucl;      /* nucleotides in the resident genome (incl. N) */
+} cof_stats;
+
+/* Replaces search_chunk_cuda{,_myers}'s set-up (search.rs:322-343,1193-1281):
+ * validates the query, builds per-pattern tables and uploads them. */
+cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out);
+
+/* Makes a genome resident in HBM from a FASTA file, a 2bit file or a folder of
+ * *.fa files, with the reference readers' chunking (read_fasta.rs:9-71,
+ * read_2bit.rs:33-118, read_fasta_folder.rs:8-23, main.rs:106-121). */
+cof_status cof_load_genome_file(cof_ctx* ctx, const char* path);
+
+/* Makes a caller-built genome resident: `packed` is the concatenation of the
+ * chunks in bit4 (search.rs:1376-1427 layout: ceil(size/2) bytes per chunk,
+ * even nucleotide in the low nibble). */
+cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes,
+                           const cof_chunk* chunks, uint64_t n_chunks,
+                           const char* const* chrom_names, uint32_t n_chroms);
+
+/* Runs the search over the resident genome.  Writes up to `cap` hits to `out`
+ * (may be NULL with cap 0) and the total to *n_hits.  Returns COF_E_OVERFLOW
+ * when cap < total (nothing partial is guaranteed; results stay cached, so a
+ * retry with a larger buffer does not search again).  Replaces the per-batch
+ * loop of search_device_cuda{,_myers} (search.rs:249-320,1054-1191) and its
+ * overflow bisection. */
+cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
+
+/* Writes the cached hits of the last search as reference TSV rows, header
+ * included (main.rs:130-247).  path "-" = stdout. */
+cof_status cof_write_tsv(cof_ctx* ctx, const char* path);
+
+cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out);
+void cof_close(cof_ctx* ctx);
+const char* cof_strerror(cof_status s);
+const char* cof_last_error(const cof_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COF_H */

@@ -1,0 +1,123 @@
+"""Plain mismatch scan on the GPU vs the CPU oracle: sorted rows must be identical.
+
+Cases follow the reference's own tests (integration_bulge.rs:156-202: the
+117-row query, CPU == GPU on sorted rows) and widen them with seeded synthetic
+genomes covering the PAM shapes of the benchmark grid, N gaps, IUPAC genome
+letters, odd FASTA line widths (odd chunks with padding nibbles) and small
+virtual batches (many batch seams).
+"""
+import os
+
+import pytest
+
+import cofpy
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def run_both(tmp_path, genome, filt, guides, mm, batch_nucl=1 << 30):
+    in_text = f"{genome}\n{filt}\n" + "".join(f"{g} {mm}\n" for g in guides)
+    ref_out = tmp_path / "ref.tsv"
+    oracle.run(in_text, ref_out, batch_nucl=batch_nucl)
+    want = cofpy.read_rows(ref_out)
+    f = filt.split()[0]
+    s = cofpy.Search(guides, [f] * len(guides), [mm] * len(guides), batch_nucl=batch_nucl)
+    s.load_file(genome)
+    s.search()
+    out = tmp_path / "gpu.tsv"
+    s.write_tsv(out)
+    got = cofpy.read_rows(out)
+    st = s.stats()
+    s.close()
+    return got, want, st
+
+
+def test_reference_117_rows(tmp_path):
+    # integration_bulge.rs:159-175: filter N x24, two guides at 11 mismatches -> 117 rows
+    for genome in ("upstream1000.fa", "upstream1000.2bit"):
+        got, want, _ = run_both(tmp_path, os.path.join(GOLDEN, genome), "N" * 24,
+                                ["CCGTGGTTCAACATTTGCTTAGCA", "GATGTTGGTAAGTGGGATATGGCA"], 11)
+        assert len(want) == 117
+        assert got == want
+
+
+def _guides(rng, n, length):
+    return ["".join(rng.choice("ACGT") for _ in range(length)) for _ in range(n)]
+
+
+PAMS = [("NRG", True), ("NNGRRT", True), ("TTTN", False)]
+
+
+@pytest.mark.parametrize("glen", [20, 25])
+@pytest.mark.parametrize("pam,three_prime", PAMS)
+@pytest.mark.parametrize("mm", [3, 6])
+def test_synthetic_pam_grid(tmp_path, glen, pam, three_prime, mm):
+    import random
+    rng = random.Random(glen * 100 + mm * 7 + len(pam))
+    guides = _guides(rng, 6, glen)
+    if three_prime:
+        filt = "N" * glen + pam
+        pats = [g + "N" * len(pam) for g in guides]
+        plants = [(g + pam, 0, glen, mm, 25) for g in guides]
+    else:
+        filt = pam + "N" * glen
+        pats = ["N" * len(pam) + g for g in guides]
+        plants = [(pam + g, len(pam), glen, mm, 25) for g in guides]
+    sg = cofpy.SynthGenome(1000 + glen + mm, [300_000, 170_000, 61_000], n_frac=0.02, gap=(50, 2000),
+                           plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    got, want, st = run_both(tmp_path, fa, filt, pats, mm)
+    assert len(want) > 0
+    assert got == want
+
+
+@pytest.mark.parametrize("batch_nucl", [1 << 17, 3 << 16])
+def test_batch_seams(tmp_path, batch_nucl):
+    # chunks of 65520 nt, chromosomes with tiny tails, batches of 2-3 chunks
+    guides = ["GGCCGACCTGTCGCTGACGC", "ACCTGTCGCTGACGCGGCCG"]
+    plants = [(g + "AGG", 0, 20, 5, 60) for g in guides]
+    sg = cofpy.SynthGenome(42, [65520 * 3 + 5, 65520 + 1, 131_041, 65520 * 2], n_frac=0.01, gap=(20, 500),
+                           plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    got, want, _ = run_both(tmp_path, fa, "NNNNNNNNNNNNNNNNNNNNNRG", [g + "NNN" for g in guides], 6,
+                            batch_nucl=batch_nucl)
+    assert got == want
+
+
+def test_odd_line_width_and_iupac(tmp_path):
+    # odd line width -> odd-sized chunks whose padding nibble sits inside the
+    # tight-packed layout; IUPAC genome letters exercise the exact path
+    guides = ["GATGTTGGTAAGTGGGATATGGCA"]
+    sg = cofpy.SynthGenome(9, [200_001, 77_777], line_width=61, n_frac=0.02, gap=(10, 300), iupac_frac=0.002,
+                           plants=[(guides[0], 0, 24, 8, 80)])
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    for batch in (1 << 17, 1 << 30):
+        got, want, _ = run_both(tmp_path, fa, "N" * 24, guides, 9, batch_nucl=batch)
+        assert len(want) > 0
+        assert got == want
+    # IUPAC in guides and filter
+    got, want, _ = run_both(tmp_path, fa, "NNNNNNNNNNNNNNNNNNNNNNRS", ["GATGTTGGTRAGTGGGATATGGNN"], 5)
+    assert got == want
+
+
+def test_overflow_retry_and_api(tmp_path):
+    guides = ["GGCCGACCTGTCGCTGACGC"]
+    sg = cofpy.SynthGenome(5, [250_000], plants=[(guides[0] + "TGG", 0, 20, 2, 30)])
+    s = cofpy.Search([g + "NNN" for g in guides], ["NNNNNNNNNNNNNNNNNNNNNRG"], [4])
+    s.load_synth(sg)
+    n = s.search()
+    assert n >= 30
+    hits = s.hits()
+    assert len(hits) == n
+    # a too-small buffer reports the needed count and COF_E_OVERFLOW
+    import ctypes as C
+    buf = (cofpy.CofHit * 2)()
+    need = C.c_uint64()
+    st = cofpy.lib().cof_search(s.h, buf, 2, C.byref(need))
+    assert st == cofpy.COF_E_OVERFLOW and need.value == n
+    s.close()
