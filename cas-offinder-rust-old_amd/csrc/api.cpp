@@ -124,7 +124,8 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
     if (!ctx) return COF_E_INVALID;
     if (!ctx->have_genome) return COF_E_NOGENOME;
     try {
-        if (!ctx->cached) {
+        // a count query (no buffer) always searches; a fetch reuses the cache
+        if (!ctx->cached || !out) {
             HIPCHECK(hipSetDevice(ctx->device));
             auto t0 = std::chrono::steady_clock::now();
             cof_stats& st = ctx->stats;
@@ -138,10 +139,10 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
                            &st.n_plain_launches);
             std::vector<cof::BulgeRaw> braw;
             cof::bulge_run(ctx->bulge, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
-            st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             st.n_raw_hits = praw.size() + braw.size();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
             cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
+            st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             st.n_hits = ctx->hits.size();
             ctx->cached = true;
         }

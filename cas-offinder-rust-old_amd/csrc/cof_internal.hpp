@@ -99,11 +99,12 @@ struct Query {
 };
 Query prepare_query(const cof_query* q);
 
-// Raw device hit of the plain scan: global start nibble + pattern.
+// Raw device hit of the plain scan: global start nibble, pattern and the
+// writer's recounted mismatches on the resident genome.
 struct PlainRaw {
     uint64_t pos;
     uint32_t pattern;
-    uint32_t flags;
+    uint32_t flags;   // recounted mismatches
 };
 
 // Raw device hit of the bulge path: alignment start nibble, pattern, counts, ops.

@@ -46,25 +46,64 @@ struct GroupDev {
     uint32_t pat_begin, pat_count;
     uint32_t L, k_lo, n_pam, W;
     uint32_t pam_pos[kMaxPam];
-    uint32_t pam_mask[kMaxPam];
+    uint32_t pam_mask[kMaxPam];   // filter code in every nibble
+};
+
+// One pattern of a group: its compare window (the pattern's nibbles from
+// k_lo, with filter-fixed N positions zeroed), the popcount threshold
+// L - M - (zeroed positions), M and the query pattern index.  64 bytes, so a
+// wave fetches a record with one scalar load.
+struct __attribute__((aligned(64))) PatRec {
+    uint32_t w[8];
+    int32_t thr;
+    uint32_t M;
+    uint32_t pid;
+    uint32_t pad[5];
 };
 
 // 4-bit nibble k (0..7) of word w
 __device__ __forceinline__ uint32_t nib_of(uint32_t w, uint32_t k) { return (w >> (4 * k)) & 0xFu; }
 
+// bit 3 of each nibble set iff the nibble is nonzero
+__device__ __forceinline__ uint32_t nz_nibbles(uint32_t y) { return (((y & 0x77777777u) + 0x77777777u) | y) & 0x88888888u; }
+// bit 3 of each nibble set iff the nibble has two or more bits set
+__device__ __forceinline__ uint32_t multi_nibbles(uint32_t w) {
+    uint32_t x = w & (w >> 1) & 0x55555555u;
+    uint32_t o = (w | (w >> 1)) & 0x55555555u;
+    return ((x | (x >> 2) | (o & (o >> 2))) & 0x11111111u) << 3;
+}
+// bit 3 of nibble j -> bit j
+__device__ __forceinline__ uint32_t gather_nibble_flags(uint32_t f) {
+    uint32_t m = f >> 3;
+    m = (m | (m >> 3)) & 0x03030303u;
+    m = (m | (m >> 6)) & 0x000F000Fu;
+    return (m | (m >> 12)) & 0xFFu;
+}
+
+// Exact writer recount (main.rs:184-187) of one window: pattern positions
+// that are not N and do not hold a plain base of the pattern's set.
+__device__ __forceinline__ uint32_t recount(const uint32_t* gw, uint32_t i, const uint32_t* __restrict__ fw, uint32_t L) {
+    uint32_t amm = 0;
+    for (uint32_t k = 0; k < L; k++) {
+        uint32_t q = i + k;
+        uint32_t g = nib_of(gw[q >> 3], q & 7u);
+        uint32_t pb = nib_of(fw[k >> 3], k & 7u);
+        bool single = g != 0 && (g & (g - 1)) == 0;
+        amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
+    }
+    return amm;
+}
+
 template <int W>
 __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
     const GroupDev* __restrict__ groups, uint32_t n_groups,
-    const uint32_t* __restrict__ cmp_words,   // [pattern][8]
-    const int32_t* __restrict__ thr,          // [pattern]
-    const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles (IUPAC path)
-    const uint32_t* __restrict__ max_mm,      // [pattern]
-    const uint32_t* __restrict__ pat_id,      // [pattern] -> query pattern index
+    const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
+    const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
     PlainRaw* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap) {
     __shared__ uint32_t gw[kTileWords];
     __shared__ uint16_t cand[kTile];
-    __shared__ uint32_t n_cand;
+    __shared__ uint32_t wave_sum[kThreads / 64];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -77,39 +116,67 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
     for (uint32_t i = tid; i < (uint32_t)kTileWords; i += kThreads) {
         uint32_t w = genome[w0 + i];
         uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
-        uint32_t x = w & (w >> 1) & 0x55555555u;
-        uint32_t o = (w | (w >> 1)) & 0x55555555u;
-        uint32_t multi = (x | (x >> 2) | (o & (o >> 2))) & 0x11111111u;  // >= 2 bits set
-        iupac |= (multi & ~f) != 0;
+        iupac |= (multi_nibbles(w) & ~(f << 3)) != 0;
         gw[i] = w & ~(f * 0xFu);
     }
     iupac = __syncthreads_or(iupac);
 
+    // valid-start mask of this thread's 64 positions (the last tile ends at scan_end)
+    const uint64_t my0 = s0 + 64u * tid;
+    const uint64_t n_valid = scan_end > my0 ? scan_end - my0 : 0;
+
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const GroupDev& G = groups[gi];
-        if (tid == 0) n_cand = 0;
-        __syncthreads();
-        // 2. PAM test + compaction
-        for (uint32_t base = 0; base < (uint32_t)kTile; base += kThreads) {
-            uint32_t i = base + tid;
-            bool ok = s0 + i < scan_end;
-            for (uint32_t k = 0; k < G.n_pam && ok; k++) {
-                uint32_t q = i + G.pam_pos[k];
-                uint32_t v = nib_of(gw[q >> 3], q & 7u);
-                ok = (G.pam_mask[k] >> v) & 1u;
+        // 2. PAM test, 8 positions per word: nibble j of X_k is the genome at
+        //    start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble is
+        //    in the filter's plain-base set iff it overlaps the filter code.
+        uint32_t m8[8];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            uint32_t t = tid * 8u + (uint32_t)w;
+            uint32_t ok = 0x88888888u;
+            for (uint32_t k = 0; k < G.n_pam; k++) {
+                uint32_t pk = G.pam_pos[k];
+                uint32_t q = t + (pk >> 3);
+                uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], (pk & 7u) * 4u);
+                uint32_t bad = iupac ? multi_nibbles(X) : 0u;
+                ok &= nz_nibbles(X & G.pam_mask[k]) & ~bad;
             }
-            uint64_t bal = __ballot(ok);
-            uint32_t cnt = (uint32_t)__popcll(bal);
-            uint32_t wbase = 0;
-            if (lane == 0 && cnt) wbase = atomicAdd(&n_cand, cnt);
-            wbase = __shfl(wbase, 0);
-            if (ok) {
-                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                cand[wbase + rank] = (uint16_t)i;
+            uint32_t m = gather_nibble_flags(ok);
+            uint64_t first = (uint64_t)w * 8u;
+            if (n_valid <= first) m = 0;
+            else if (n_valid < first + 8) m &= (1u << (uint32_t)(n_valid - first)) - 1u;
+            m8[w] = m;
+            cnt += (uint32_t)__popc(m);
+        }
+        // block-wide exclusive scan of the per-thread counts
+        uint32_t v = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t u = __shfl_up(v, d);
+            if (lane >= (uint32_t)d) v += u;
+        }
+        if (lane == 63) wave_sum[wave] = v;
+        __syncthreads();
+        uint32_t nc = 0, off = 0;
+#pragma unroll
+        for (int x = 0; x < kThreads / 64; x++) {
+            uint32_t ws = wave_sum[x];
+            off += (uint32_t)x < wave ? ws : 0u;
+            nc += ws;
+        }
+        uint32_t pos = off + v - cnt;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            uint32_t m = m8[w];
+            while (m) {
+                uint32_t j = (uint32_t)__builtin_ctz(m);
+                cand[pos++] = (uint16_t)(tid * 64u + (uint32_t)w * 8u + j);
+                m &= m - 1u;
             }
         }
         __syncthreads();
-        const uint32_t nc = n_cand;
         // 3. compare windows against the group's patterns
         for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
             uint32_t c = c0 + lane;
@@ -121,14 +188,39 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
                 uint32_t win[W];
 #pragma unroll
                 for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
-                    const uint32_t* cw = cmp_words + (size_t)p * 8;
+                const uint32_t pe = G.pat_begin + G.pat_count;
+                uint32_t p = G.pat_begin;
+                // four patterns per iteration: their records are fetched with
+                // one batch of scalar loads, and one branch covers the four hits
+                for (; p + 4 <= pe; p += 4) {
+                    uint32_t hit = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const PatRec& r = pats[p + u];
+                        uint32_t sum = 0;
+#pragma unroll
+                        for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
+                        hit |= ((int32_t)sum >= r.thr ? 1u : 0u) << u;
+                    }
+                    if (active && hit) {
+                        for (int u = 0; u < 4; u++) {
+                            if (!((hit >> u) & 1u)) continue;
+                            const PatRec& r = pats[p + u];
+                            uint32_t amm = recount(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                            uint32_t idx = atomicAdd(out_count, 1u);
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                        }
+                    }
+                }
+                for (; p < pe; p++) {
+                    const PatRec& r = pats[p];
                     uint32_t sum = 0;
 #pragma unroll
-                    for (int w = 0; w < W; w++) sum += __popc(win[w] & cw[w]);
-                    if (active && (int32_t)sum >= thr[p]) {
+                    for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
+                    if (active && (int32_t)sum >= r.thr) {
+                        uint32_t amm = recount(gw, i, full_words + (size_t)p * 8, G.L);
                         uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pat_id[p], 0u};
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                     }
                 }
             } else {
@@ -144,10 +236,10 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
                         bool single = g != 0 && (g & (g - 1)) == 0;
                         amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
                     }
-                    uint32_t M = max_mm[p];
+                    uint32_t M = pats[p].M;
                     if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
                         uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pat_id[p], 1u};
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
                     }
                 }
             }
@@ -166,8 +258,8 @@ struct PlainPlan {
     std::vector<GroupDev> groups;
     std::vector<Launch> launches;
     GroupDev* d_groups = nullptr;
-    uint32_t *d_cmp = nullptr, *d_full = nullptr, *d_mm = nullptr, *d_pid = nullptr;
-    int32_t* d_thr = nullptr;
+    PatRec* d_pats = nullptr;
+    uint32_t* d_full = nullptr;
     PlainRaw* d_out = nullptr;
     uint32_t* d_count = nullptr;
     uint32_t out_cap = 0;
@@ -187,8 +279,8 @@ PlainPlan* plain_plan_create(const Query& q) {
         for (auto& c : f) c = (char)toupper((unsigned char)c);
         by_key[{p.L, f}].push_back(i);
     }
-    std::vector<uint32_t> cmp, full, mm, pid;
-    std::vector<int32_t> thr;
+    std::vector<PatRec> recs;
+    std::vector<uint32_t> full;
     std::map<int, std::vector<GroupDev>> by_w;
     uint32_t pat_index = 0;
     for (auto& kv : by_key) {
@@ -201,12 +293,10 @@ PlainPlan* plain_plan_create(const Query& q) {
         for (uint32_t k = 0; k < L; k++) {
             if (is_n_char(f[k])) continue;
             if (G.n_pam >= (uint32_t)kMaxPam) throw Error(COF_E_UNSUPPORTED, "too many PAM positions");
-            uint32_t mask = 0;
-            uint8_t fv = kCodec.enc_mixed[(uint8_t)f[k]];
-            for (uint32_t v = 0; v < 16; v++)
-                if (single_base((uint8_t)v) && (v & fv)) mask |= 1u << v;
+            // filter code replicated over the 8 nibbles of a word
+            uint32_t fv = kCodec.enc_mixed[(uint8_t)f[k]];
             G.pam_pos[G.n_pam] = k;
-            G.pam_mask[G.n_pam] = mask;
+            G.pam_mask[G.n_pam] = fv * 0x11111111u;
             G.n_pam++;
             bool all_n = true;
             for (uint32_t pi : kv.second) all_n &= q.pats[pi].b4[k] == 0xF;
@@ -232,11 +322,13 @@ PlainPlan* plain_plan_create(const Query& q) {
                 cw[r / 8] |= (uint32_t)p.b4[k] << (4 * (r % 8));
             }
             for (uint32_t k = 0; k < L; k++) fw[k / 8] |= (uint32_t)p.b4[k] << (4 * (k % 8));
-            cmp.insert(cmp.end(), cw, cw + 8);
+            PatRec r{};
+            for (int w = 0; w < 8; w++) r.w[w] = cw[w];
+            r.thr = (int32_t)L - (int32_t)n_excl - (int32_t)p.M;
+            r.M = p.M;
+            r.pid = pi;
+            recs.push_back(r);
             full.insert(full.end(), fw, fw + 8);
-            thr.push_back((int32_t)L - (int32_t)n_excl - (int32_t)p.M);
-            mm.push_back(p.M);
-            pid.push_back(pi);
             pat_index++;
         }
         by_w[(int)G.W].push_back(G);
@@ -253,12 +345,9 @@ PlainPlan* plain_plan_create(const Query& q) {
             HIPCHECK(hipMalloc(&dptr, v.size() * sizeof(v[0])));
             HIPCHECK(hipMemcpy(dptr, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
         };
-        up(pp->d_cmp, cmp);
+        up(pp->d_pats, recs);
         up(pp->d_full, full);
-        up(pp->d_thr, thr);
-        up(pp->d_mm, mm);
-        up(pp->d_pid, pid);
-        pp->out_cap = 1u << 20;
+        pp->out_cap = 1u << 22;
         HIPCHECK(hipMalloc(&pp->d_out, (size_t)pp->out_cap * sizeof(PlainRaw)));
         HIPCHECK(hipMalloc(&pp->d_count, sizeof(uint32_t)));
         HIPCHECK(hipEventCreate(&pp->ev0));
@@ -270,11 +359,8 @@ PlainPlan* plain_plan_create(const Query& q) {
 void plain_plan_destroy(PlainPlan* pp) {
     if (!pp) return;
     (void)hipFree(pp->d_groups);
-    (void)hipFree(pp->d_cmp);
+    (void)hipFree(pp->d_pats);
     (void)hipFree(pp->d_full);
-    (void)hipFree(pp->d_thr);
-    (void)hipFree(pp->d_mm);
-    (void)hipFree(pp->d_pid);
     (void)hipFree(pp->d_out);
     (void)hipFree(pp->d_count);
     if (pp->ev0) (void)hipEventDestroy(pp->ev0);
@@ -306,7 +392,7 @@ void plain_run(PlainPlan* pp, const uint32_t* d_genome, uint64_t scan_end, hipSt
 #define COF_PLAIN_LAUNCH(WW)                                                                              \
     case WW:                                                                                              \
         hipLaunchKernelGGL(plain_scan_kernel<WW>, dim3(nb), dim3(kThreads), 0, st, d_genome, scan_end, first, g, \
-                           l.n_groups, pp->d_cmp, pp->d_thr, pp->d_full, pp->d_mm, pp->d_pid, pp->d_out,  \
+                           l.n_groups, pp->d_pats, pp->d_full, pp->d_out,                                 \
                            pp->d_count, pp->out_cap);                                                    \
         break;
                 switch (l.W) {

@@ -76,20 +76,22 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
         uint64_t view_end = 0;
         bool emit = false;
         // reported by batch b, where c is not the head
-        if (!((last_in_b || next_new_chrom) && past_end)) {
-            view_end = bi.end_nib[(size_t)b];
-            emit = plain_predicate(g, p, r.pos, view_end, &mm);
-        }
+        // The device evaluated the predicate on the resident genome, which
+        // equals the batch view whenever the window ends inside it; only
+        // windows running past the view end are evaluated again here.
+        auto check = [&](uint64_t ve) {
+            view_end = ve;
+            if (r.pos + p.L <= ve) { mm = r.flags; return true; }
+            return plain_predicate(g, p, r.pos, ve, &mm);
+        };
+        if (!((last_in_b || next_new_chrom) && past_end)) emit = check(bi.end_nib[(size_t)b]);
         // reported by batch b+1, where c is the head (search.rs:540-546); the
         // two cases exclude each other (the second needs past_end)
         if (!emit && last_in_b && (size_t)b + 1 < bs.size()) {
             uint64_t c1 = g.chunks[c + 1].byte_off * 2 - ch.nib_begin();
             uint64_t min_start = c1 >= p.L - 1 ? c1 - (p.L - 1) : 0;
             uint64_t local = r.pos - ch.nib_begin();
-            if (local >= min_start && !(next_new_chrom && past_end)) {
-                view_end = bi.end_nib[(size_t)b + 1];
-                emit = plain_predicate(g, p, r.pos, view_end, &mm);
-            }
+            if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)b + 1]);
         }
         if (!emit) continue;
         cof_hit h{};

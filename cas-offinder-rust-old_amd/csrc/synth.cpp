@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -88,7 +89,7 @@ struct cofsynth {
     std::vector<uint64_t> chrom_first_chunk;
     uint64_t chunk_nt = 0;
     std::vector<std::vector<std::pair<uint64_t, uint64_t>>> n_gaps;    // per chrom, sorted
-    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> occupied;  // planted intervals
+    std::vector<std::set<std::pair<uint64_t, uint64_t>>> occupied;     // planted intervals
 
     uint64_t gnib(uint32_t chrom, uint64_t pos) const {
         // every chunk but a chromosome's last holds exactly chunk_nt
@@ -275,7 +276,12 @@ int cofsynth_build(cofsynth* s) {
                 if (x + len > s->lens[c]) continue;
                 bool bad = false;
                 for (uint64_t i = 0; i < len && !bad; i++) bad = s->get(c, x + i) == 0xF;
-                for (auto& iv : s->occupied[c]) bad = bad || (x < iv.second && iv.first < x + len);
+                if (!bad) {
+                    // neighbours in the start-sorted interval set
+                    auto it = s->occupied[c].lower_bound({x, 0});
+                    if (it != s->occupied[c].end() && it->first < x + len) bad = true;
+                    if (it != s->occupied[c].begin() && std::prev(it)->second > x) bad = true;
+                }
                 if (bad) continue;
                 std::vector<uint8_t> v(len);
                 for (uint64_t i = 0; i < len; i++) {
@@ -296,7 +302,7 @@ int cofsynth_build(cofsynth* s) {
                     if (rev) b = (uint8_t)(((b << 2) | (b >> 2)) & 0xF);
                     s->put(c, x + i, b);
                 }
-                s->occupied[c].push_back({x, x + len});
+                s->occupied[c].insert({x, x + len});
                 break;
             }
         }

@@ -87,7 +87,7 @@ typedef struct {
 } cof_hit;
 
 typedef struct {
-    double search_ms;          /* wall time of the last cof_search (device work + D2H of hits) */
+    double search_ms;          /* wall time of the last search: device work, D2H of hits, conversion */
     double kernel_ms_plain;    /* summed HIP-event time of plain-scan launches */
     double kernel_ms_filter;   /* summed HIP-event time of PAM+Myers filter launches */
     double kernel_ms_verify;   /* summed HIP-event time of DP traceback launches */
@@ -114,12 +114,14 @@ cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes
                            const cof_chunk* chunks, uint64_t n_chunks,
                            const char* const* chrom_names, uint32_t n_chroms);
 
-/* Runs the search over the resident genome.  Writes up to `cap` hits to `out`
- * (may be NULL with cap 0) and the total to *n_hits.  Returns COF_E_OVERFLOW
- * when cap < total (nothing partial is guaranteed; results stay cached, so a
- * retry with a larger buffer does not search again).  Replaces the per-batch
- * loop of search_device_cuda{,_myers} (search.rs:249-320,1054-1191) and its
- * overflow bisection. */
+/* Runs the search over the resident genome and writes the hit count to
+ * *n_hits.  With out == NULL the search always runs (a count query); with a
+ * buffer, the hits of the last search are returned when they are cached,
+ * otherwise the search runs first.  Returns COF_E_OVERFLOW when cap < total
+ * (the first cap hits are written; the cache is kept, so a retry with a larger
+ * buffer does not search again).  Replaces the per-batch loop of
+ * search_device_cuda{,_myers} (search.rs:249-320,1054-1191) and its overflow
+ * bisection (search.rs:1140-1152). */
 cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
 
 /* Writes the cached hits of the last search as reference TSV rows, header
