@@ -1,0 +1,106 @@
+"""Bulge search (DNA/RNA bulges) on the GPU vs the CPU oracle: sorted rows must be identical.
+
+Starts from the reference's own bulge test (integration_bulge.rs:208-259:
+N x25 + GG with 1 DNA / 1 RNA bulge, CPU rows == GPU rows, non-empty, 8
+columns, type in {X, DNA, RNA}) and widens it with seeded synthetic genomes:
+3' and 5' PAMs (PAM-last and PAM-first quick checks), bulge limits 0-2 per
+type, N gaps and IUPAC letters, odd chunks and small virtual batches (window
+floors and PAM limits at batch seams), and a mixed plain + bulge query.
+"""
+import os
+import random
+
+import pytest
+
+import cofpy
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def oracle_rows(tmp_path, genome, filt_line, guides, mm, batch_nucl):
+    in_text = f"{genome}\n{filt_line}\n" + "".join(f"{g} {mm}\n" for g in guides)
+    oracle.run(in_text, tmp_path / "ref.tsv", batch_nucl=batch_nucl)
+    return cofpy.read_rows(tmp_path / "ref.tsv")
+
+
+def gpu_rows(tmp_path, genome, filt, guides, mms, ds, rs, batch_nucl, filters=None):
+    s = cofpy.Search(guides, filters or [filt] * len(guides), mms, ds, rs, batch_nucl=batch_nucl)
+    s.load_file(genome)
+    s.search()
+    s.write_tsv(tmp_path / "gpu.tsv")
+    st = s.stats()
+    s.close()
+    return cofpy.read_rows(tmp_path / "gpu.tsv"), st
+
+
+def test_reference_bulge_query(tmp_path):
+    genome = os.path.join(GOLDEN, "upstream1000.fa")
+    filt = "NNNNNNNNNNNNNNNNNNNNNNNNNGG"
+    guide = "CCGTGGTTCAACATTTGCTTAGCANNN"
+    want = oracle_rows(tmp_path, genome, filt + " 1 1", [guide], 5, 1 << 30)
+    got, st = gpu_rows(tmp_path, genome, filt, [guide], [5], [1], [1], 1 << 30)
+    assert got == want
+    assert len(got) > 0
+    for row in got:
+        cols = row.split("\t")
+        assert len(cols) == 8 and cols[0] in ("X", "DNA", "RNA")
+
+
+CASES = [
+    # pam, 3', glen, M, D, R
+    ("NRG", True, 20, 4, 2, 2),
+    ("NGG", True, 20, 3, 1, 0),
+    ("NGG", True, 20, 3, 0, 1),
+    ("TTTN", False, 20, 3, 1, 1),
+    ("NNGRRT", True, 21, 3, 1, 2),
+    ("NRG", True, 28, 5, 2, 1),     # W = 33: 64-bit window kernels
+]
+
+
+@pytest.mark.parametrize("pam,three,glen,M,D,R", CASES)
+def test_synthetic_bulge(tmp_path, pam, three, glen, M, D, R):
+    rng = random.Random(glen * 31 + M + D * 7 + R * 13)
+    guides = ["".join(rng.choice("ACGT") for _ in range(glen)) for _ in range(2)]
+    if three:
+        filt = "N" * glen + pam
+        pats = [g + "N" * len(pam) for g in guides]
+        plants = [(g + pam, 0, glen, M, 15) for g in guides]
+    else:
+        filt = pam + "N" * glen
+        pats = ["N" * len(pam) + g for g in guides]
+        plants = [(pam + g, len(pam), glen, M, 15) for g in guides]
+    sg = cofpy.SynthGenome(77 + glen + M, [60_000, 41_000], n_frac=0.02, gap=(20, 400), iupac_frac=0.001,
+                           repeats=(0.1, 4, 300, 0.1), plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    want = oracle_rows(tmp_path, fa, f"{filt} {D} {R}", pats, M, 1 << 30)
+    got, st = gpu_rows(tmp_path, fa, filt, pats, [M] * 2, [D] * 2, [R] * 2, 1 << 30)
+    assert len(want) > 0
+    assert got == want
+
+
+@pytest.mark.parametrize("batch_nucl", [1 << 17, 3 << 16])
+def test_bulge_batch_seams_odd_chunks(tmp_path, batch_nucl):
+    guides = ["GGCCGACCTGTCGCTGACGC"]
+    sg = cofpy.SynthGenome(5, [65520 * 2 + 40, 65521, 90_000], line_width=61, n_frac=0.01, gap=(10, 200),
+                           plants=[(guides[0] + "TGG", 0, 20, 3, 40)])
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    want = oracle_rows(tmp_path, fa, "NNNNNNNNNNNNNNNNNNNNNRG 1 1", [g + "NNN" for g in guides], 3, batch_nucl)
+    got, _ = gpu_rows(tmp_path, fa, "NNNNNNNNNNNNNNNNNNNNNRG", [g + "NNN" for g in guides], [3], [1], [1], batch_nucl)
+    assert got == want and len(want) > 0
+
+
+def test_mixed_plain_and_bulge_query(tmp_path):
+    # one context holding a plain run and a bulge run == union of two reference runs
+    g1, g2 = "GGCCGACCTGTCGCTGACGC", "ACCTGTCGCTGACGCGGCCG"
+    sg = cofpy.SynthGenome(9, [120_000], plants=[(g1 + "AGG", 0, 20, 4, 20), (g2 + "CGG", 0, 20, 2, 20)])
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    f = "NNNNNNNNNNNNNNNNNNNNNRG"
+    want = sorted(oracle_rows(tmp_path, fa, f, [g1 + "NNN"], 4, 1 << 30) +
+                  oracle_rows(tmp_path, fa, f + " 1 1", [g2 + "NNN"], 2, 1 << 30))
+    got, _ = gpu_rows(tmp_path, fa, f, [g1 + "NNN", g2 + "NNN"], [4, 2], [0, 1], [0, 1], 1 << 30)
+    assert got == want
