@@ -12,6 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 
 COF_OK, COF_E_INVALID, COF_E_HIP, COF_E_OVERFLOW = 0, 1, 2, 3
 COF_E_IO, COF_E_NOGENOME, COF_E_NOMEM, COF_E_UNSUPPORTED = 4, 5, 6, 7
+COF_FLAG_COUNT_ONLY = 1
 
 # Every symbol include/cof.h declares (checked by the CPU test suite).
 COF_SYMBOLS = ["cof_open", "cof_load_genome_file", "cof_load_genome", "cof_search",
@@ -25,7 +26,8 @@ class CofQuery(C.Structure):
                 ("max_mismatches", C.POINTER(C.c_uint32)),
                 ("max_dna_bulges", C.POINTER(C.c_uint32)),
                 ("max_rna_bulges", C.POINTER(C.c_uint32)),
-                ("batch_nucl", C.c_uint64)]
+                ("batch_nucl", C.c_uint64),
+                ("flags", C.c_uint32)]
 
 
 class CofChunk(C.Structure):
@@ -43,7 +45,9 @@ class CofStats(C.Structure):
     _fields_ = [("search_ms", C.c_double), ("kernel_ms_plain", C.c_double),
                 ("kernel_ms_filter", C.c_double), ("kernel_ms_verify", C.c_double),
                 ("n_plain_launches", C.c_uint64), ("n_candidates", C.c_uint64),
-                ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64)]
+                ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64),
+                ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
+                ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64)]
 
 
 _lib = None
@@ -159,7 +163,7 @@ class CofError(RuntimeError):
 class Search:
     """One libcof context: query -> resident genome -> hits."""
 
-    def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0):
+    def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0, count_only=False):
         L = lib()
         n = len(guides)
         self._keep = [g.encode() for g in guides], [f.encode() for f in filters]
@@ -171,6 +175,7 @@ class Search:
         q.max_dna_bulges = (C.c_uint32 * n)(*(dnas or [0] * n))
         q.max_rna_bulges = (C.c_uint32 * n)(*(rnas or [0] * n))
         q.batch_nucl = batch_nucl
+        q.flags = COF_FLAG_COUNT_ONLY if count_only else 0
         self._q = q
         h = C.c_void_p()
         st = L.cof_open(device, C.byref(q), C.byref(h))

@@ -141,7 +141,16 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             cof::bulge_run(ctx->bulge, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
             st.n_raw_hits = praw.size() + braw.size();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
+            size_t first_bulge = ctx->hits.size();
             cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
+            if (!(ctx->q.flags & COF_FLAG_COUNT_ONLY)) {
+                st.bulge_hits = ctx->hits.size() - first_bulge;
+                for (size_t h = first_bulge; h < ctx->hits.size(); h++) {
+                    const cof_hit& x = ctx->hits[h];
+                    st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, x.pattern, x.mismatches, x.dna_bulge,
+                                                             x.rna_bulge, x.n_ops, x.ops[0], x.ops[1], x.ops[2]);
+                }
+            }
             st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             st.n_hits = ctx->hits.size();
             ctx->cached = true;

@@ -50,7 +50,8 @@ struct BGroup {
 };
 
 struct BPat {
-    uint8_t cls[64];          // Myers class per position: 0 A, 1 C, 2 G, 3 T, 4 wildcard (myers.rs:20-50)
+    uint64_t cls_bits[4];     // Myers class per position, 3 bits each, 21 per word:
+                              // 0 A, 1 C, 2 G, 3 T, 4 wildcard (myers.rs:20-50)
     uint64_t pa, pc, pg, pt;  // pattern bit4 planes over positions
     uint64_t fa, fc, fg, ft;  // effective-filter bit4 planes over positions
     uint64_t is_n;            // ASCII N / n positions
@@ -62,14 +63,22 @@ struct BTile {
     uint64_t floor;           // window floor: start of the virtual batch buffer
     uint64_t limit;           // PAM limit: end of the batch's last chunk
     uint32_t n;               // end positions in the tile
-    uint32_t pad;
+    uint32_t last_chunk;      // one past the batch's last chunk
 };
 
 struct BCand {
     uint64_t j;
     uint64_t floor;
     uint32_t pat;             // index into the BPat table
-    uint32_t pad;
+    uint32_t last_chunk;
+};
+
+// Chunk table for the device-side drop rule of convert_matches_myers
+// (search.rs:1338-1350).
+struct DChunk {
+    uint64_t nib_begin;
+    uint32_t size;
+    uint32_t chrom_start;     // chunk starts a chromosome
 };
 
 __device__ __forceinline__ uint32_t lds_nib(const uint32_t* w, uint32_t q) { return (w[q >> 3] >> ((q & 7u) * 4u)) & 0xFu; }
@@ -99,6 +108,74 @@ template <typename WORD>
 struct Masks {
     WORD m[5];
 };
+
+// bit 0 of nibble j -> bit j
+__device__ __forceinline__ uint32_t gather_bit0(uint32_t m) {
+    m &= 0x11111111u;
+    m = (m | (m >> 3)) & 0x03030303u;
+    m = (m | (m >> 6)) & 0x000F000Fu;
+    return (m | (m >> 12)) & 0xFFu;
+}
+
+// Myers match masks of the n-nibble window whose words (aligned to the window
+// start) are X[0..): per word, SWAR tests give 8 positions at once.
+template <typename WORD>
+__device__ __forceinline__ void window_masks(const uint32_t* X, uint32_t n, Masks<WORD>& mk, WORD* zero_mask,
+                                             WORD* n_mask, WORD* bA, WORD* bC, WORD* bG, WORD* bT) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) mk.m[c] = 0;
+    WORD z = 0, nn = 0, a = 0, cc = 0, g = 0, t = 0;
+#pragma unroll
+    for (int k = 0; k < (int)sizeof(WORD) * 2; k++) {
+        if ((uint32_t)k * 8 >= n) break;
+        uint32_t x = X[k];
+        uint32_t f = x & (x >> 1) & (x >> 2) & (x >> 3) & 0x11111111u;           // nibble == 0xF
+        uint32_t nz = ((((x & 0x77777777u) + 0x77777777u) | x) >> 3) & 0x11111111u;  // nibble != 0
+        uint32_t real = nz & ~f;
+        uint32_t sh = (uint32_t)k * 8;
+        mk.m[0] |= (WORD)gather_bit0((x >> 2) & real) << sh;
+        mk.m[1] |= (WORD)gather_bit0((x >> 1) & real) << sh;
+        mk.m[2] |= (WORD)gather_bit0((x >> 3) & real) << sh;
+        mk.m[3] |= (WORD)gather_bit0(x & real) << sh;
+        mk.m[4] |= (WORD)gather_bit0(nz) << sh;
+        if (zero_mask) {
+            z |= (WORD)gather_bit0(~nz) << sh;
+            nn |= (WORD)gather_bit0(f) << sh;
+            a |= (WORD)gather_bit0(x >> 2) << sh;
+            cc |= (WORD)gather_bit0(x >> 1) << sh;
+            g |= (WORD)gather_bit0(x >> 3) << sh;
+            t |= (WORD)gather_bit0(x) << sh;
+        }
+    }
+    WORD keep = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
+#pragma unroll
+    for (int c = 0; c < 5; c++) mk.m[c] &= keep;
+    if (zero_mask) {
+        *zero_mask = z & keep; *n_mask = nn & keep;
+        *bA = a & keep; *bC = cc & keep; *bG = g & keep; *bT = t & keep;
+    }
+}
+
+// match mask of class c (A C G T wildcard): a 5-way select, no indexing
+template <typename WORD>
+__device__ __forceinline__ WORD class_mask(const Masks<WORD>& mk, uint32_t c) {
+    WORD e01 = (c & 1u) ? mk.m[1] : mk.m[0];
+    WORD e23 = (c & 1u) ? mk.m[3] : mk.m[2];
+    WORD e = (c & 2u) ? e23 : e01;
+    return (c & 4u) ? mk.m[4] : e;
+}
+
+template <typename WORD>
+__device__ __forceinline__ void myers_step(WORD eq, WORD& VP, WORD& VN) {
+    WORD Xv = eq | VN;
+    WORD Xh = (((eq & VP) + VP) ^ VP) | eq;
+    WORD Ph = VN | ~(Xh | VP);
+    WORD Mh = VP & Xh;
+    Ph = (Ph << 1) | (WORD)1;
+    Mh = Mh << 1;
+    VP = Mh | ~(Xv | Ph);
+    VN = Ph & Xv;
+}
 
 template <typename WORD>
 __device__ __forceinline__ uint32_t popc_w(WORD x) {
@@ -169,38 +246,29 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
             uint64_t j = T.j0 + e;
             uint64_t ts = j + 1 >= T.floor + G.W ? j + 1 - G.W : T.floor;
             uint32_t n = (uint32_t)(j - ts + 1);
-            Masks<WORD> mk;
+            // window words aligned to its first nibble
+            uint32_t q = (uint32_t)(ts - base);
+            uint32_t qw = q >> 3, sh = (q & 7u) * 4u;
+            uint32_t X[sizeof(WORD) * 2];
 #pragma unroll
-            for (int x = 0; x < 5; x++) mk.m[x] = 0;
-            for (uint32_t a = 0; a < n; a++) {
-                uint32_t t = lds_nib(gw, (uint32_t)(ts + a - base));
-                WORD bit = (WORD)1 << a;
-                bool real = t != 0 && t != 0xFu;
-                mk.m[0] |= (real && (t & 4u)) ? bit : (WORD)0;
-                mk.m[1] |= (real && (t & 2u)) ? bit : (WORD)0;
-                mk.m[2] |= (real && (t & 8u)) ? bit : (WORD)0;
-                mk.m[3] |= (real && (t & 1u)) ? bit : (WORD)0;
-                mk.m[4] |= t != 0 ? bit : (WORD)0;
-            }
+            for (int k = 0; k < (int)sizeof(WORD) * 2; k++) X[k] = __builtin_amdgcn_alignbit(gw[qw + k + 1], gw[qw + k], sh);
+            Masks<WORD> mk;
+            window_masks<WORD>(X, n, mk, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
             WORD nmask = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
             for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
                 const BPat& P = pats[p];
+                const uint32_t L = P.L;
                 WORD VP = 0, VN = 0;
-                for (uint32_t i = 0; i < P.L; i++) {
-                    WORD eq = mk.m[P.cls[i]];
-                    WORD Xv = eq | VN;
-                    WORD Xh = (((eq & VP) + VP) ^ VP) | eq;
-                    WORD Ph = VN | ~(Xh | VP);
-                    WORD Mh = VP & Xh;
-                    Ph = (Ph << 1) | (WORD)1;
-                    Mh = Mh << 1;
-                    VP = Mh | ~(Xv | Ph);
-                    VN = Ph & Xv;
+                // pattern classes stream from scalar registers, 21 per word
+                for (uint32_t w = 0, i = 0; i < L; w++) {
+                    uint64_t cw = P.cls_bits[w];
+                    for (uint32_t k = 0; k < 21 && i < L; k++, i++, cw >>= 3)
+                        myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
                 }
-                int32_t score = (int32_t)P.L + (int32_t)popc_w<WORD>(VP & nmask) - (int32_t)popc_w<WORD>(VN & nmask);
+                int32_t score = (int32_t)L + (int32_t)popc_w<WORD>(VP & nmask) - (int32_t)popc_w<WORD>(VN & nmask);
                 if (active && score <= (int32_t)P.E) {
                     uint32_t idx = atomicAdd(q_count, 1u);
-                    if (idx < q_cap) queue[idx] = BCand{j, T.floor, p, 0u};
+                    if (idx < q_cap) queue[idx] = BCand{j, T.floor, p, T.last_chunk};
                 }
             }
         }
@@ -208,205 +276,263 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     }
 }
 
-// Hit sink: records and/or an order-independent checksum of the records.
+// Hit sink: records (count <= cap written), or, in count-only mode, the
+// device-side drop rule plus a kept count and checksum per wave.
 struct Sink {
     BulgeRaw* out;
     uint32_t* count;
     uint32_t cap;
-    unsigned long long* checksum;   // sum of mix64(record) (may be null)
+    int count_only;
+    const DChunk* chunks;
+    uint32_t n_chunks;
+    unsigned long long* kept;
+    unsigned long long* checksum;
+    unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop
 };
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
-    return x;
+// Reverses the order of the 2-bit ops held in NW words (op 0 lowest), so a
+// stack pushed from the alignment's end reads in alignment order.
+__device__ __forceinline__ uint64_t rev2(uint64_t x) {
+    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+    x = ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x & 0x00FF00FF00FF00FFull) << 8);
+    x = ((x >> 16) & 0x0000FFFF0000FFFFull) | ((x & 0x0000FFFF0000FFFFull) << 16);
+    return (x >> 32) | (x << 32);
 }
 
 template <typename WORD>
 __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __restrict__ genome,
                                                           const BCand* __restrict__ queue, uint32_t n_queue,
-                                                          const BPat* __restrict__ pats, Sink sink) {
-    // Myers columns of each lane's window: col i at [i][lane] (L <= window bits)
-    __shared__ WORD cvp[sizeof(WORD) * 8 + 1][64];
-    __shared__ WORD cvn[sizeof(WORD) * 8 + 1][64];
+                                                          uint32_t* __restrict__ q_next, const BPat* __restrict__ pats,
+                                                          uint32_t n_cols, Sink sink) {
+    // depth <= window length <= bits of WORD; ops 2 bits and child masks 4 bits per depth
+    constexpr int kOpsW = sizeof(WORD) / 4;        // u64 words of ops
+    constexpr int kTrsW = sizeof(WORD) / 2;        // u64 words of child masks
+    extern __shared__ unsigned char verify_lds[];
+    WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
+    WORD* cvn = cvp + (size_t)n_cols * 64;
     const uint32_t lane = threadIdx.x;
-    const uint32_t qi = blockIdx.x * 64u + lane;
-    if (qi >= n_queue) return;
-    const BCand C = queue[qi];
-    const BPat& P = pats[C.pat];
-    const uint32_t L = P.L, W = L + P.D;
-    const uint64_t ts = C.j + 1 >= C.floor + W ? C.j + 1 - W : C.floor;
-    const uint32_t n = (uint32_t)(C.j - ts + 1);
 
-    // window planes: Myers masks, padding, genome N, and bit4 planes
-    Masks<WORD> mk;
-    WORD z = 0, tA = 0, tC = 0, tG = 0, tT = 0, tN = 0;
+    // candidate state
+    uint64_t ts = 0;
+    WORD z = 0, tN = 0, tA = 0, tC = 0, tG = 0, tT = 0;   // window planes
+    WORD pa = 0, pc = 0, pg = 0, pt = 0, fa = 0, fc = 0, fg = 0, ft = 0, pn = 0;  // pattern planes
+    uint32_t L = 0, M = 0, D = 0, R = 0, E = 0, pid = 0, last_chunk = 0;
+    // DFS state
+    bool busy = false;
+    uint32_t i = 0, jj = 0, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
+    uint64_t ops[kOpsW], trs[kTrsW];
 #pragma unroll
-    for (int x = 0; x < 5; x++) mk.m[x] = 0;
-    for (uint32_t a = 0; a < n; a++) {
-        uint32_t t = glob_nib(genome, ts + a);
-        WORD bit = (WORD)1 << a;
-        bool real = t != 0 && t != 0xFu;
-        mk.m[0] |= (real && (t & 4u)) ? bit : (WORD)0;
-        mk.m[1] |= (real && (t & 2u)) ? bit : (WORD)0;
-        mk.m[2] |= (real && (t & 8u)) ? bit : (WORD)0;
-        mk.m[3] |= (real && (t & 1u)) ? bit : (WORD)0;
-        mk.m[4] |= t != 0 ? bit : (WORD)0;
-        z |= t == 0 ? bit : (WORD)0;
-        tN |= t == 0xFu ? bit : (WORD)0;
-        tA |= (t & 4u) ? bit : (WORD)0;
-        tC |= (t & 2u) ? bit : (WORD)0;
-        tG |= (t & 8u) ? bit : (WORD)0;
-        tT |= (t & 1u) ? bit : (WORD)0;
-    }
-    cvp[0][lane] = 0;
-    cvn[0][lane] = 0;
-    {
-        WORD VP = 0, VN = 0;
-        for (uint32_t i = 0; i < L; i++) {
-            WORD eq = mk.m[P.cls[i]];
-            WORD Xv = eq | VN;
-            WORD Xh = (((eq & VP) + VP) ^ VP) | eq;
-            WORD Ph = VN | ~(Xh | VP);
-            WORD Mh = VP & Xh;
-            Ph = (Ph << 1) | (WORD)1;
-            Mh = Mh << 1;
-            VP = Mh | ~(Xv | Ph);
-            VN = Ph & Xv;
-            cvp[i + 1][lane] = VP;
-            cvn[i + 1][lane] = VN;
-        }
-    }
-    auto text_at = [&](uint32_t a) -> uint32_t {   // window nibble a
-        uint32_t t = (uint32_t)((tT >> a) & 1) | (uint32_t)((tC >> a) & 1) << 1 | (uint32_t)((tA >> a) & 1) << 2 |
-                     (uint32_t)((tG >> a) & 1) << 3;
-        return t;
-    };
-    auto pat_at = [&](uint32_t k) -> uint32_t {
-        return (uint32_t)((P.pt >> k) & 1) | (uint32_t)((P.pc >> k) & 1) << 1 | (uint32_t)((P.pa >> k) & 1) << 2 |
-               (uint32_t)((P.pg >> k) & 1) << 3;
-    };
-    auto filt_at = [&](uint32_t k) -> uint32_t {
-        return (uint32_t)((P.ft >> k) & 1) | (uint32_t)((P.fc >> k) & 1) << 1 | (uint32_t)((P.fa >> k) & 1) << 2 |
-               (uint32_t)((P.fg >> k) & 1) << 3;
-    };
-    auto isn = [&](uint32_t k) -> bool { return (P.is_n >> k) & 1ull; };
-    // lower bound D[i][jj] of the remaining cost from node (i, jj)
-    auto lb = [&](uint32_t i, uint32_t jj) -> uint32_t {
-        WORD m = jj >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << jj) - 1);
-        return i + popc_w<WORD>(cvp[i][lane] & m) - popc_w<WORD>(cvn[i][lane] & m);
-    };
-    const uint32_t E = P.E, M = P.M, D = P.D, R = P.R;
-    // node admission (enumerate's entry checks, traceback.rs:349-354; padding
-    // cells are infinite in the reference DP, traceback.rs:297-300)
-    auto admit = [&](uint32_t i, uint32_t jj, uint32_t cost, uint32_t mm, uint32_t db, uint32_t rb) -> bool {
-        if (mm > M || db > D || rb > R) return false;
-        if (i >= 1 && jj >= 1 && ((z >> (jj - 1)) & 1)) return false;
-        return cost + lb(i, jj) <= E;
-    };
-    if (!admit(L, n, 0, 0, 0, 0)) return;
+    for (int k = 0; k < kOpsW; k++) ops[k] = 0;
+#pragma unroll
+    for (int k = 0; k < kTrsW; k++) trs[k] = 0;
+    uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0;
+    bool drained = false;
 
-    // DFS state: ops / next-transition per depth, 2 bits each
-    uint64_t ops[3] = {0, 0, 0}, trs[3] = {0, 0, 0};
-    auto get2 = [](const uint64_t* a, uint32_t d) -> uint32_t {
-        uint64_t w = d < 32 ? a[0] : d < 64 ? a[1] : a[2];
-        return (uint32_t)(w >> (2 * (d & 31))) & 3u;
+    auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
+    auto get2 = [&](uint32_t dd) -> uint32_t {
+        uint64_t w = ops[0];
+        if constexpr (kOpsW > 1) w = dd < 32 ? ops[0] : ops[1];
+        return (uint32_t)(w >> (2 * (dd & 31))) & 3u;
     };
-    auto set2 = [](uint64_t* a, uint32_t d, uint32_t v) {
-        uint64_t m = ~(3ull << (2 * (d & 31))), b = (uint64_t)v << (2 * (d & 31));
-        if (d < 32) a[0] = (a[0] & m) | b;
-        else if (d < 64) a[1] = (a[1] & m) | b;
-        else a[2] = (a[2] & m) | b;
+    auto set2 = [&](uint32_t dd, uint32_t v) {
+        uint64_t m = ~(3ull << (2 * (dd & 31))), b = (uint64_t)v << (2 * (dd & 31));
+        if constexpr (kOpsW > 1) {
+            if (dd < 32) ops[0] = (ops[0] & m) | b; else ops[1] = (ops[1] & m) | b;
+        } else {
+            ops[0] = (ops[0] & m) | b;
+        }
     };
-    uint32_t i = L, jj = n, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
-    set2(trs, 0, 0);
+    auto get4 = [&](uint32_t dd) -> uint32_t {
+        uint32_t w = dd >> 4;
+        uint64_t v = trs[0];
+#pragma unroll
+        for (int k = 1; k < kTrsW; k++) v = w == (uint32_t)k ? trs[k] : v;
+        return (uint32_t)(v >> (4 * (dd & 15))) & 15u;
+    };
+    auto set4 = [&](uint32_t dd, uint32_t val) {
+        uint32_t w = dd >> 4;
+        uint64_t m = ~(15ull << (4 * (dd & 15))), b = (uint64_t)val << (4 * (dd & 15));
+#pragma unroll
+        for (int k = 0; k < kTrsW; k++) trs[k] = w == (uint32_t)k ? ((trs[k] & m) | b) : trs[k];
+    };
+    auto text_at = [&](uint32_t a) -> uint32_t {
+        uint32_t t = bit(tT, a) | bit(tC, a) << 1 | bit(tA, a) << 2 | bit(tG, a) << 3;
+        return bit(tN, a) ? 0xFu : bit(z, a) ? 0u : t;
+    };
+    auto pat_at = [&](uint32_t k) -> uint32_t { return bit(pt, k) | bit(pc, k) << 1 | bit(pa, k) << 2 | bit(pg, k) << 3; };
+    auto filt_at = [&](uint32_t k) -> uint32_t { return bit(ft, k) | bit(fc, k) << 1 | bit(fa, k) << 2 | bit(fg, k) << 3; };
+    auto lbc = [&](uint32_t ii, WORD vp, WORD vn, uint32_t jx) -> uint32_t {
+        WORD m = jx >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << jx) - 1);
+        return ii + popc_w<WORD>(vp & m) - popc_w<WORD>(vn & m);
+    };
+    // Admissible children of node (i, jj) as a mask: 1 diagonal, 2 RNA bulge,
+    // 4 DNA bulge (enumerate(), traceback.rs:398-468).  Each child must pass
+    // enumerate()'s entry checks (caps, cost + DP bound <= E, no padding
+    // cell, traceback.rs:297-300,349-354); a diagonal step must also pass the
+    // filter cell of its pattern position, which is where the full PAM check
+    // of classify_myers_candidate (search.rs:638-671) would reject the leaf.
+    auto children = [&]() -> uint32_t {
+        if (i == 0) return 0u;
+        WORD vp1 = cvp[(i - 1) * 64 + lane], vn1 = cvn[(i - 1) * 64 + lane];
+        WORD vp0 = cvp[i * 64 + lane], vn0 = cvn[i * 64 + lane];
+        uint32_t v = 0;
+        const uint32_t n_im1 = bit(pn, i - 1), n_i = i < L ? bit(pn, i) : 0u, n_im2 = i >= 2 ? bit(pn, i - 2) : 0u;
+        if (jj > 0 && !bit(z, jj - 1)) {
+            uint32_t tv = text_at(jj - 1);
+            uint32_t mc = cmp_loose(pat_at(i - 1), tv) ? 0u : 1u;
+            bool pad_next = i - 1 >= 1 && jj - 1 >= 1 && bit(z, jj - 2);
+            if (!(mc && n_im1) && mm + mc <= M && !pad_next && pam_cell_ok(tv, filt_at(i - 1)) &&
+                cost + mc + lbc(i - 1, vp1, vn1, jj - 1) <= E)
+                v |= 1u;
+        }
+        if (rb < R && !n_im1 && !n_i && !n_im2 && !(i - 1 >= 1 && jj >= 1 && bit(z, jj - 1)) &&
+            cost + 1 + lbc(i - 1, vp1, vn1, jj) <= E)
+            v |= 2u;
+        if (db < D && jj > 0 && i < L && !(n_im1 && n_i) && !(jj - 1 >= 1 && bit(z, jj - 2)) && !bit(z, jj - 1) &&
+            cost + 1 + lbc(i, vp0, vn0, jj - 1) <= E)
+            v |= 4u;
+        return v;
+    };
+    // leaf (i == 0): emit the alignment
+    auto emit = [&]() {
+        uint64_t o0, o1 = 0;
+        if constexpr (kOpsW == 1) {
+            o0 = rev2(ops[0]) >> (2 * (32 - d));
+        } else {
+            // 128-bit reverse, then shift right by 2 * (64 - d)
+            uint64_t h = rev2(ops[0]), l = rev2(ops[1]);
+            uint32_t sh = 2 * (64 - d);
+            if (sh >= 64) { o0 = h >> (sh - 64); o1 = 0; }
+            else if (sh == 0) { o0 = l; o1 = h; }
+            else { o0 = (l >> sh) | (h << (64 - sh)); o1 = h >> sh; }
+        }
+        uint64_t st = ts + jj;
+        if (!sink.count_only) {
+            BulgeRaw r;
+            r.start = st; r.pattern = pid;
+            r.mm = (uint8_t)mm; r.db = (uint8_t)db; r.rb = (uint8_t)rb; r.n_ops = (uint8_t)d;
+            r.ops[0] = o0; r.ops[1] = o1; r.ops[2] = 0;
+            uint32_t idx = atomicAdd(sink.count, 1u);
+            if (idx < sink.cap) sink.out[idx] = r;
+        } else {
+            // convert_matches_myers' drop rule (search.rs:1338-1350) on the
+            // chunk of the alignment start
+            uint32_t lo = 0, hi = sink.n_chunks;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (sink.chunks[mid].nib_begin <= st) lo = mid + 1; else hi = mid;
+            }
+            uint32_t c = lo - 1;
+            bool end_chrom = c + 1 == last_chunk || sink.chunks[c + 1].chrom_start;
+            bool past_end = st - sink.chunks[c].nib_begin + (L + db - rb) > sink.chunks[c].size;
+            if (!(end_chrom && past_end)) {
+                my_kept++;
+                my_sum += bulge_hit_hash(st, pid, mm, db, rb, d, o0, o1, 0);
+            }
+        }
+    };
+
     for (;;) {
-        if (i == 0) {
-            // leaf: full PAM check along the alignment (search.rs:638-671)
-            bool ok = true;
-            uint32_t pi = 0, a = jj;
-            for (uint32_t k = 0; k < d && ok; k++) {
-                uint32_t op = get2(ops, d - 1 - k);
-                if (op <= 1) {
-                    uint32_t gv = (z >> a) & 1 ? 0u : ((tN >> a) & 1) ? 0xFu : text_at(a);
-                    ok = pam_cell_ok(gv, filt_at(pi));
-                    pi++;
-                    a++;
-                } else if (op == 2) {
-                    a++;
-                } else {
-                    pi++;
+        // refill idle lanes from the queue, a wave at a time
+        uint64_t idle = __ballot(!busy);
+        uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!drained && n_idle >= 8) {
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(q_next, n_idle);
+            b0 = __shfl(b0, 0);
+            if (b0 + n_idle >= n_queue) drained = true;
+            if (!busy) {
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                uint32_t qi = b0 + rank;
+                if (qi < n_queue) {
+                    const BCand C = queue[qi];
+                    const BPat& P = pats[C.pat];
+                    L = P.L; M = P.M; D = P.D; R = P.R; E = P.E; pid = P.pid;
+                    pa = (WORD)P.pa; pc = (WORD)P.pc; pg = (WORD)P.pg; pt = (WORD)P.pt;
+                    fa = (WORD)P.fa; fc = (WORD)P.fc; fg = (WORD)P.fg; ft = (WORD)P.ft;
+                    pn = (WORD)P.is_n;
+                    last_chunk = C.last_chunk;
+                    uint32_t W = L + D;
+                    ts = C.j + 1 >= C.floor + W ? C.j + 1 - W : C.floor;
+                    uint32_t n = (uint32_t)(C.j - ts + 1);
+                    uint64_t w0 = ts >> 3;
+                    uint32_t sh = (uint32_t)(ts & 7u) * 4u;
+                    uint32_t X[sizeof(WORD) * 2];
+                    uint32_t prev = genome[w0];
+#pragma unroll
+                    for (int k = 0; k < (int)sizeof(WORD) * 2; k++) {
+                        uint32_t nx = genome[w0 + k + 1];
+                        X[k] = __builtin_amdgcn_alignbit(nx, prev, sh);
+                        prev = nx;
+                    }
+                    Masks<WORD> mk;
+                    window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
+                    WORD VP = 0, VN = 0;
+                    cvp[lane] = 0;
+                    cvn[lane] = 0;
+                    for (uint32_t w = 0, ii = 0; ii < L; w++) {
+                        uint64_t cw = P.cls_bits[w];
+                        for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3) {
+                            myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
+                            cvp[(ii + 1) * 64 + lane] = VP;
+                            cvn[(ii + 1) * 64 + lane] = VN;
+                        }
+                    }
+                    i = L; jj = n; d = 0; cost = mm = db = rb = 0;
+                    // root admission: padding at the window end, DP bound
+                    busy = !bit(z, n - 1) && lbc(L, VP, VN, n) <= E;
+                    if (busy) set4(0, children());
                 }
             }
-            if (ok) {
-                BulgeRaw r;
-                r.start = ts + jj;
-                r.pattern = P.pid;
-                r.mm = (uint8_t)mm;
-                r.db = (uint8_t)db;
-                r.rb = (uint8_t)rb;
-                r.n_ops = (uint8_t)d;
-                r.ops[0] = r.ops[1] = r.ops[2] = 0;
-                for (uint32_t k = 0; k < d; k++) r.ops[k / 32] |= (uint64_t)get2(ops, d - 1 - k) << (2 * (k % 32));
-                uint32_t idx = atomicAdd(sink.count, 1u);
-                if (sink.out && idx < sink.cap) sink.out[idx] = r;
-                if (sink.checksum) {
-                    uint64_t h = mix64(r.start * 0x9E3779B97F4A7C15ull ^ ((uint64_t)r.pattern << 40) ^
-                                       ((uint64_t)r.mm << 24) ^ ((uint64_t)r.db << 16) ^ ((uint64_t)r.rb << 8) ^ r.n_ops);
-                    h = mix64(h ^ r.ops[0]) + mix64(h ^ r.ops[1] ^ 0x1234567ull) + mix64(h ^ r.ops[2] ^ 0x89ABCDEFull);
-                    atomicAdd(sink.checksum, (unsigned long long)h);
-                }
-            }
-            goto pop;
+            continue;
         }
-        {
-            uint32_t t = get2(trs, d);
+        if (drained && __ballot(busy) == 0) break;
+        my_slots++;
+        if (!busy) continue;
+        my_steps++;
+        uint32_t viable = get4(d);
+        if (viable) {
+            // push the next admissible child and expand it
+            uint32_t t = (uint32_t)__builtin_ctz(viable);
+            set4(d, viable & (viable - 1));
             if (t == 0) {
-                set2(trs, d, 1);
-                if (jj > 0) {                                   // diagonal
-                    uint32_t tv = (z >> (jj - 1)) & 1 ? 0u : ((tN >> (jj - 1)) & 1) ? 0xFu : text_at(jj - 1);
-                    uint32_t mc = cmp_loose(pat_at(i - 1), tv) ? 0u : 1u;
-                    if (!(mc && isn(i - 1)) && admit(i - 1, jj - 1, cost + mc, mm + mc, db, rb)) {
-                        set2(ops, d, mc);
-                        i--; jj--; cost += mc; mm += mc; d++;
-                        set2(trs, d, 0);
-                    }
-                }
-                continue;
+                uint32_t mc = cmp_loose(pat_at(i - 1), text_at(jj - 1)) ? 0u : 1u;
+                set2(d, mc);
+                i--; jj--; cost += mc; mm += mc;
+            } else if (t == 1) {
+                set2(d, 3);
+                i--; cost++; rb++;
+            } else {
+                set2(d, 2);
+                jj--; cost++; db++;
             }
-            if (t == 1) {
-                set2(trs, d, 2);
-                if (rb < R && R > 0) {                          // RNA bulge
-                    bool ok = !isn(i - 1) && !(i < L && isn(i)) && !(i >= 2 && isn(i - 2));
-                    if (ok && admit(i - 1, jj, cost + 1, mm, db, rb + 1)) {
-                        set2(ops, d, 3);
-                        i--; cost++; rb++; d++;
-                        set2(trs, d, 0);
-                    }
-                }
-                continue;
-            }
-            if (t == 2) {
-                set2(trs, d, 3);
-                if (db < D && D > 0 && jj > 0 && i < L) {       // DNA bulge
-                    bool ok = !(isn(i - 1) && isn(i));
-                    if (ok && admit(i, jj - 1, cost + 1, mm, db + 1, rb)) {
-                        set2(ops, d, 2);
-                        jj--; cost++; db++; d++;
-                        set2(trs, d, 0);
-                    }
-                }
-                continue;
-            }
-        }
-    pop:
-        if (d == 0) break;
-        d--;
-        {
-            uint32_t op = get2(ops, d);
+            d++;
+            if (i == 0) emit();
+            set4(d, children());
+        } else if (d == 0) {
+            busy = false;
+        } else {
+            d--;
+            uint32_t op = get2(d);
             if (op <= 1) { i++; jj++; cost -= op; mm -= op; }
             else if (op == 2) { jj++; cost--; db--; }
             else { i++; cost--; rb--; }
         }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        my_kept += __shfl_down(my_kept, off);
+        my_sum += __shfl_down(my_sum, off);
+        my_steps += __shfl_down(my_steps, off);
+        my_slots += __shfl_down(my_slots, off);
+    }
+    if (lane == 0) {
+        if (sink.count_only && my_kept) {
+            atomicAdd(sink.kept, (unsigned long long)my_kept);
+            atomicAdd(sink.checksum, (unsigned long long)my_sum);
+        }
+        atomicAdd(&sink.diag[0], (unsigned long long)my_steps);
+        atomicAdd(&sink.diag[1], (unsigned long long)my_slots);
     }
 }
 
@@ -427,12 +553,19 @@ struct BulgePlan {
     BTile* d_tiles = nullptr;
     size_t tiles_cap = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool records = true;
-    uint64_t last_checksum = 0, last_count = 0;
+    bool count_only = false;
+    DChunk* d_chunks = nullptr;
+    size_t chunks_cap = 0;
+    unsigned long long* d_kept = nullptr;
+    uint32_t* d_qnext = nullptr;
+    unsigned long long* d_diag = nullptr;
+    uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
+    uint32_t verify_blocks = 0;   // persistent verify grid
 };
 
 BulgePlan* bulge_plan_create(const Query& q) {
     auto* bp = new BulgePlan();
+    bp->count_only = (q.flags & COF_FLAG_COUNT_ONLY) != 0;
     if (!q.any_bulge) return bp;
     // group bulge patterns sharing the quick PAM check
     std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, std::vector<uint8_t>>, std::vector<uint32_t>> by_key;
@@ -466,7 +599,8 @@ BulgePlan* bulge_plan_create(const Query& q) {
             BPat P{};
             for (uint32_t i = 0; i < p.L; i++) {
                 char u = (char)toupper((unsigned char)p.ascii[i]);
-                P.cls[i] = u == 'A' ? 0 : u == 'C' ? 1 : u == 'G' ? 2 : u == 'T' ? 3 : 4;
+                uint64_t cls = u == 'A' ? 0 : u == 'C' ? 1 : u == 'G' ? 2 : u == 'T' ? 3 : 4;
+                P.cls_bits[i / 21] |= cls << (3 * (i % 21));
                 uint8_t v = p.b4[i], f = kCodec.enc_mixed[(uint8_t)p.eff_filter[i]];
                 P.pt |= (uint64_t)(v & 1) << i; P.pc |= (uint64_t)((v >> 1) & 1) << i;
                 P.pa |= (uint64_t)((v >> 2) & 1) << i; P.pg |= (uint64_t)((v >> 3) & 1) << i;
@@ -487,13 +621,23 @@ BulgePlan* bulge_plan_create(const Query& q) {
     up(bp->d_g32, bp->groups32);
     up(bp->d_g64, bp->groups64);
     up(bp->d_pats, bp->pats);
-    bp->q_cap = 1u << 22;
+    bp->q_cap = 1u << 27;
     HIPCHECK(hipMalloc(&bp->d_queue, (size_t)bp->q_cap * sizeof(BCand)));
     HIPCHECK(hipMalloc(&bp->d_qcount, sizeof(uint32_t)));
     bp->out_cap = 1u << 22;
     HIPCHECK(hipMalloc(&bp->d_out, (size_t)bp->out_cap * sizeof(BulgeRaw)));
     HIPCHECK(hipMalloc(&bp->d_ocount, sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&bp->d_checksum, sizeof(unsigned long long)));
+    HIPCHECK(hipMalloc(&bp->d_kept, sizeof(unsigned long long)));
+    HIPCHECK(hipMalloc(&bp->d_qnext, sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&bp->d_diag, 2 * sizeof(unsigned long long)));
+    for (const auto& P : bp->pats) bp->n_cols = std::max(bp->n_cols, P.L + 1);
+    {
+        int dev = 0, cus = 0;
+        HIPCHECK(hipGetDevice(&dev));
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        bp->verify_blocks = (uint32_t)std::max(1, cus) * 12;
+    }
     for (auto& e : bp->ev) HIPCHECK(hipEventCreate(&e));
     return bp;
 }
@@ -501,7 +645,8 @@ BulgePlan* bulge_plan_create(const Query& q) {
 void bulge_plan_destroy(BulgePlan* bp) {
     if (!bp) return;
     for (void* p : {(void*)bp->d_g32, (void*)bp->d_g64, (void*)bp->d_pats, (void*)bp->d_queue, (void*)bp->d_qcount,
-                    (void*)bp->d_out, (void*)bp->d_ocount, (void*)bp->d_checksum, (void*)bp->d_tiles})
+                    (void*)bp->d_out, (void*)bp->d_ocount, (void*)bp->d_checksum, (void*)bp->d_tiles,
+                    (void*)bp->d_chunks, (void*)bp->d_kept, (void*)bp->d_qnext, (void*)bp->d_diag})
         if (p) (void)hipFree(p);
     for (auto& e : bp->ev) if (e) (void)hipEventDestroy(e);
     delete bp;
@@ -509,11 +654,6 @@ void bulge_plan_destroy(BulgePlan* bp) {
 
 uint64_t bulge_pad_words() { return (uint64_t)kWinWords + 64; }
 
-void bulge_set_sink(BulgePlan* bp, bool records) { if (bp) bp->records = records; }
-void bulge_last_checksum(const BulgePlan* bp, uint64_t* count, uint64_t* checksum) {
-    *count = bp ? bp->last_count : 0;
-    *checksum = bp ? bp->last_checksum : 0;
-}
 
 // Runs the filter over every virtual batch's end range, then the traceback
 // over the queued candidates.  Tile ranges are processed in rounds so the
@@ -521,8 +661,6 @@ void bulge_last_checksum(const BulgePlan* bp, uint64_t* count, uint64_t* checksu
 void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches, const uint32_t* d_genome,
                hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats) {
     out.clear();
-    bp->last_checksum = 0;
-    bp->last_count = 0;
     if (!bp || bp->pats.empty()) return;
     // tiles of end positions per virtual batch (search.rs:1080-1092, 899-927)
     std::vector<BTile> tiles;
@@ -531,7 +669,7 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
         uint64_t a = g.chunks[b.first + (b.has_head ? 1 : 0)].nib_begin();
         uint64_t e = b.last < g.chunks.size() ? g.chunks[b.last].nib_begin() : g.n_nibbles();
         for (uint64_t j0 = a; j0 < e; j0 += kTileEnds)
-            tiles.push_back(BTile{j0, floor, e, (uint32_t)std::min<uint64_t>(kTileEnds, e - j0), 0u});
+            tiles.push_back(BTile{j0, floor, e, (uint32_t)std::min<uint64_t>(kTileEnds, e - j0), (uint32_t)b.last});
     }
     if (tiles.empty()) return;
     if (tiles.size() > bp->tiles_cap) {
@@ -541,12 +679,28 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
     }
     HIPCHECK(hipMemcpyAsync(bp->d_tiles, tiles.data(), tiles.size() * sizeof(BTile), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemsetAsync(bp->d_checksum, 0, sizeof(unsigned long long), st));
-    uint64_t total_hits = 0;
-    const size_t round_tiles = 1u << 14;
+    HIPCHECK(hipMemsetAsync(bp->d_kept, 0, sizeof(unsigned long long), st));
+    HIPCHECK(hipMemsetAsync(bp->d_diag, 0, 2 * sizeof(unsigned long long), st));
+    if (bp->count_only) {
+        std::vector<DChunk> dc(g.chunks.size());
+        for (size_t c = 0; c < g.chunks.size(); c++)
+            dc[c] = DChunk{g.chunks[c].nib_begin(), (uint32_t)g.chunks[c].size(), g.chunks[c].start == 0 ? 1u : 0u};
+        if (dc.size() > bp->chunks_cap) {
+            if (bp->d_chunks) (void)hipFree(bp->d_chunks);
+            bp->chunks_cap = dc.size();
+            HIPCHECK(hipMalloc(&bp->d_chunks, dc.size() * sizeof(DChunk)));
+        }
+        HIPCHECK(hipMemcpyAsync(bp->d_chunks, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice, st));
+    }
+    // rounds of tiles sized from the candidate density seen so far, so the
+    // queue rarely overflows; an overflowing round is split in halves
+    size_t next_tile = 0, round_tiles = 1u << 12;
     std::vector<std::pair<size_t, size_t>> rounds;
-    for (size_t t = 0; t < tiles.size(); t += round_tiles) rounds.push_back({t, std::min(tiles.size(), t + round_tiles)});
-    std::reverse(rounds.begin(), rounds.end());
-    while (!rounds.empty()) {
+    while (next_tile < tiles.size() || !rounds.empty()) {
+        if (rounds.empty()) {
+            rounds.push_back({next_tile, std::min(tiles.size(), next_tile + round_tiles)});
+            next_tile = rounds.back().second;
+        }
         auto [t0, t1] = rounds.back();
         rounds.pop_back();
         HIPCHECK(hipMemsetAsync(bp->d_qcount, 0, sizeof(uint32_t), st));
@@ -582,22 +736,29 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
             continue;
         }
         if (stats) stats->n_candidates += nq;
+        {
+            double per_tile = (double)nq / (double)(t1 - t0);
+            double fit = 0.5 * (double)bp->q_cap / std::max(per_tile, 1.0);
+            round_tiles = (size_t)std::max(1.0, std::min(fit, (double)(1u << 16)));
+        }
         if (nq == 0) continue;
         // verify; rerun with a larger record buffer when it overflows
         for (;;) {
             HIPCHECK(hipMemsetAsync(bp->d_ocount, 0, sizeof(uint32_t), st));
             HIPCHECK(hipEventRecord(bp->ev[2], st));
-            Sink sk{bp->records ? bp->d_out : nullptr, bp->d_ocount, bp->records ? bp->out_cap : 0u,
-                    bp->records ? nullptr : bp->d_checksum};
-            uint32_t blocks = (nq + 63) / 64;
-            // candidates of W <= 32 and W > 32 groups share the queue; the
-            // 64-bit kernel handles every window size
+            Sink sk{bp->d_out, bp->d_ocount, bp->out_cap, bp->count_only ? 1 : 0, bp->d_chunks,
+                    (uint32_t)g.chunks.size(), bp->d_kept, bp->d_checksum, bp->d_diag};
+            // persistent verify: lanes refill from the queue until it drains
+            HIPCHECK(hipMemsetAsync(bp->d_qnext, 0, sizeof(uint32_t), st));
+            uint32_t blocks = std::min<uint32_t>((nq + 63) / 64, bp->verify_blocks);
+            size_t lds32 = (size_t)bp->n_cols * 64 * sizeof(uint32_t) * 2;
+            size_t lds64 = (size_t)bp->n_cols * 64 * sizeof(uint64_t) * 2;
             if (bp->groups64.empty())
-                hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), 0, st, d_genome, bp->d_queue,
-                                   nq, bp->d_pats, sk);
+                hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, d_genome,
+                                   bp->d_queue, nq, bp->d_qnext, bp->d_pats, bp->n_cols, sk);
             else
-                hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), 0, st, d_genome, bp->d_queue,
-                                   nq, bp->d_pats, sk);
+                hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, d_genome,
+                                   bp->d_queue, nq, bp->d_qnext, bp->d_pats, bp->n_cols, sk);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipEventRecord(bp->ev[3], st));
             uint32_t no = 0;
@@ -605,14 +766,13 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
             HIPCHECK(hipStreamSynchronize(st));
             HIPCHECK(hipEventElapsedTime(&ms, bp->ev[2], bp->ev[3]));
             if (stats) stats->kernel_ms_verify += ms;
-            if (bp->records && no > bp->out_cap) {
+            if (!bp->count_only && no > bp->out_cap) {
                 (void)hipFree(bp->d_out);
                 bp->out_cap = no + no / 4;
                 HIPCHECK(hipMalloc(&bp->d_out, (size_t)bp->out_cap * sizeof(BulgeRaw)));
                 continue;
             }
-            total_hits += no;
-            if (bp->records && no) {
+            if (!bp->count_only && no) {
                 size_t old = out.size();
                 out.resize(old + no);
                 HIPCHECK(hipMemcpyAsync(out.data() + old, bp->d_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost, st));
@@ -621,13 +781,21 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
             break;
         }
     }
-    if (!bp->records) {
-        unsigned long long cs = 0;
-        HIPCHECK(hipMemcpyAsync(&cs, bp->d_checksum, sizeof(cs), hipMemcpyDeviceToHost, st));
+    if (stats) {
+        unsigned long long dg[2] = {0, 0};
+        HIPCHECK(hipMemcpyAsync(dg, bp->d_diag, sizeof dg, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        bp->last_checksum = cs;
+        stats->verify_steps = dg[0];
+        stats->verify_slots = dg[1];
     }
-    bp->last_count = total_hits;
+    if (bp->count_only && stats) {
+        unsigned long long cs = 0, kept = 0;
+        HIPCHECK(hipMemcpyAsync(&cs, bp->d_checksum, sizeof(cs), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&kept, bp->d_kept, sizeof(kept), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        stats->bulge_checksum = cs;
+        stats->bulge_hits = kept;
+    }
 }
 
 }  // namespace cof

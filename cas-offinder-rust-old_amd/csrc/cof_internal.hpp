@@ -95,6 +95,7 @@ struct Query {
     std::vector<Pattern> pats;  // n fwd then n rev
     uint32_t n_guides = 0;
     uint64_t batch_nucl = 1ull << 30;
+    uint32_t flags = 0;
     bool any_plain = false, any_bulge = false;
 };
 Query prepare_query(const cof_query* q);
@@ -114,6 +115,24 @@ struct BulgeRaw {
     uint8_t mm, db, rb, n_ops;
     uint64_t ops[3];          // forward order, 2 bits per op
 };
+
+#if defined(__HIPCC__)
+#define COF_HD __host__ __device__
+#else
+#define COF_HD
+#endif
+
+// Order-independent checksum term of one bulge hit (summed over hits).
+COF_HD inline uint64_t cof_mix64(uint64_t x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+    return x;
+}
+COF_HD inline uint64_t bulge_hit_hash(uint64_t start, uint32_t pattern, uint32_t mm, uint32_t db, uint32_t rb,
+                                      uint32_t n_ops, uint64_t ops0, uint64_t ops1, uint64_t ops2) {
+    uint64_t h = cof_mix64(start * 0x9E3779B97F4A7C15ull ^ ((uint64_t)pattern << 40) ^ ((uint64_t)mm << 24) ^
+                           ((uint64_t)db << 16) ^ ((uint64_t)rb << 8) ^ n_ops);
+    return cof_mix64(h ^ ops0) + cof_mix64(h ^ ops1 ^ 0x1234567ull) + cof_mix64(h ^ ops2 ^ 0x89ABCDEFull);
+}
 
 // Host-side conversion (chunk lookup, batch-seam and chromosome-end drop
 // rules, writer predicate) of raw device hits into final hits.

@@ -18,6 +18,8 @@ void plain_run(PlainPlan* pp, const uint32_t* d_genome, uint64_t scan_end, hipSt
 
 struct BulgePlan;
 BulgePlan* bulge_plan_create(const Query& q);
+// count-only mode: device applies the chromosome-end drop rule and folds hits
+// into stats->bulge_hits / bulge_checksum instead of returning records
 void bulge_plan_destroy(BulgePlan* bp);
 uint64_t bulge_pad_words();
 void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches, const uint32_t* d_genome,

@@ -20,6 +20,7 @@ Query prepare_query(const cof_query* q) {
     Query out;
     out.n_guides = q->n_guides;
     out.batch_nucl = q->batch_nucl ? q->batch_nucl : (1ull << 30);
+    out.flags = q->flags;
     if (out.batch_nucl % kChunkNucl != 0 || out.batch_nucl < 2 * kChunkNucl || out.batch_nucl > (1ull << 31))
         throw Error(COF_E_INVALID, "batch_nucl must be a multiple of 65536 in [131072, 2^31]");
     // The run a guide belongs to: guides of one reference run share filter

@@ -61,7 +61,15 @@ typedef struct {
     uint64_t batch_nucl;             /* virtual batch size in nt, multiple of 65536;
                                         0 = 1<<30 (reference GPU, search.rs:22,44-50);
                                         1<<22 reproduces the reference CPU path */
+    uint32_t flags;                  /* COF_FLAG_* */
 } cof_query;
+
+/* Bulge-path hits are counted and folded into an order-independent checksum
+ * on the device instead of being returned (for outputs too large to hold:
+ * a 3 Gbp bulge search can produce ~1e10 alignments).  cof_search then
+ * reports plain hits only in `out`; cof_stats carries the bulge count and
+ * checksum, which equal those of a record-mode search. */
+#define COF_FLAG_COUNT_ONLY 1u
 
 /* One chunk of the tight-packed genome (chrom_chunk.rs:1-12, search.rs:57-79). */
 typedef struct {
@@ -96,6 +104,11 @@ typedef struct {
     uint64_t n_raw_hits;       /* hits before the host-side seam/drop rules */
     uint64_t n_hits;           /* final hits */
     uint64_t genome_nucl;      /* nucleotides in the resident genome (incl. N) */
+    uint64_t bulge_hits;       /* final bulge-path hits (both modes) */
+    uint64_t bulge_checksum;   /* sum over final bulge hits of a 64-bit hash of
+                                  (start nibble, pattern, mm, db, rb, ops) */
+    uint64_t verify_steps;     /* diagnostics: DFS transitions of the traceback kernel */
+    uint64_t verify_slots;     /* diagnostics: lane slots its DFS loop ran (steps / slots = lane use) */
 } cof_stats;
 
 /* Replaces search_chunk_cuda{,_myers}'s set-up (search.rs:322-343,1193-1281):

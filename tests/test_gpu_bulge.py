@@ -104,3 +104,25 @@ def test_mixed_plain_and_bulge_query(tmp_path):
                   oracle_rows(tmp_path, fa, f + " 1 1", [g2 + "NNN"], 2, 1 << 30))
     got, _ = gpu_rows(tmp_path, fa, f, [g1 + "NNN", g2 + "NNN"], [4, 2], [0, 1], [0, 1], 1 << 30)
     assert got == want
+
+
+def test_count_only_equals_record_mode(tmp_path):
+    # the device-side drop rule + checksum equals conversion on the host;
+    # small chromosomes and 2-chunk batches put alignments across chromosome
+    # ends and batch seams
+    guides = ["GGCCGACCTGTCGCTGACGC", "TTGACCGATCGGATCCAGTT"]
+    sg = cofpy.SynthGenome(21, [65520 + 30, 300, 65520 * 2, 45_000, 23], n_frac=0.01, gap=(5, 100),
+                           repeats=(0.1, 3, 300, 0.1), plants=[(g + "GGG", 0, 20, 3, 25) for g in guides])
+    pats = [g + "NNN" for g in guides]
+    res = []
+    for count_only in (False, True):
+        s = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * 2, [4, 4], [2, 2], [2, 2], batch_nucl=1 << 17,
+                         count_only=count_only)
+        s.load_synth(sg)
+        n = s.search()
+        st = s.stats()
+        res.append((st.bulge_hits, st.bulge_checksum, n))
+        s.close()
+    assert res[0][0] > 1000
+    assert res[0][:2] == res[1][:2]
+    assert res[0][2] == res[0][0] and res[1][2] == 0
