@@ -132,6 +132,87 @@ def bulge_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def batched_guides(seed, n):
+    """10,000-guide mix: len U{17..30}, PAM in {NRG, NGG, NNGRRT, TTTN}, M U{0..6}, 20% with D, R in U{1,2}."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        glen = rng.randint(17, 30)
+        pam = rng.choice(["NRG", "NGG", "NNGRRT", "TTTN"])
+        g = "".join(rng.choice("ACGT") for _ in range(glen))
+        mm = rng.randint(0, 6)
+        d, r = (rng.randint(1, 2), rng.randint(1, 2)) if rng.random() < 0.2 else (0, 0)
+        if pam == "TTTN":
+            out.append(("NNNN" + g, pam + "N" * glen, mm, d, r, pam + g, 4, glen))
+        else:
+            out.append((g + "N" * len(pam), "N" * glen + pam, mm, d, r, g + pam, 0, glen))
+    return out
+
+
+def batched_workload(args):
+    """Batched search: all guides in one query vs one reference-format run per guide set."""
+    total = int(args.genome_bp)
+    lens = chrom_lengths(total, 24, args.seed)
+    guides = batched_guides(args.seed + 11, args.batched_guides)
+    plants = [(site, off, glen, min(mm, 3), 20) for (_, _, mm, _, _, site, off, glen) in guides[:2000]]
+    t = time.time()
+    sg = cofpy.SynthGenome(args.seed + 3, lens, n_frac=0.05, gap=(10_000, 1_000_000),
+                           repeats=(0.10, 20, 300, 0.10), plants=plants)
+    log(f"repeat-rich genome {total / 1e9:.3f} Gbp, {len(guides)} guides, generated in {time.time() - t:.1f} s")
+    pats, filts, mms, ds, rs = zip(*[(g[0], g[1], g[2], g[3], g[4]) for g in guides])
+    s = cofpy.Search(list(pats), list(filts), list(mms), list(ds), list(rs), count_only=True)
+    s.load_synth(sg)
+    res = {}
+    # batched: one query
+    times = []
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        n_plain = s.search()
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        if k >= args.warmup:
+            times.append(dt)
+        log(f"batched {'warmup' if k < args.warmup else 'step'}: {dt:.3f} s, plain hits {n_plain}, "
+            f"bulge hits {st.bulge_hits}, candidates {st.n_candidates}, plain {st.kernel_ms_plain:.1f} ms, "
+            f"filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms")
+    batched = sum(times) / len(times)
+    res["batched"] = {"seconds": round(batched, 3), "plain_hits": n_plain, "bulge_hits": st.bulge_hits,
+                      "bulge_checksum": st.bulge_checksum}
+    if args.compare_plain_schedule:
+        # plain schedule: one run per (filter, M, D, R) set
+        sets = {}
+        for g in guides:
+            sets.setdefault((g[1], g[2], g[3], g[4]), []).append(g)
+        t_sum, hp, hb, cs = 0.0, 0, 0, 0
+        for si, (key, gs) in enumerate(sets.items()):
+            if si % 100 == 0:
+                log(f"plain schedule: run {si}/{len(sets)}, {t_sum:.2f} s so far")
+            s.set_query([g[0] for g in gs], [key[0]] * len(gs), [key[1]] * len(gs), [key[2]] * len(gs),
+                        [key[3]] * len(gs), count_only=True)
+            t0 = time.perf_counter()
+            hp += s.search()
+            t_sum += time.perf_counter() - t0
+            st = s.stats()
+            hb += st.bulge_hits
+            cs = (cs + st.bulge_checksum) % (1 << 64)
+        log(f"plain schedule: {len(sets)} runs, {t_sum:.3f} s, plain hits {hp}, bulge hits {hb}")
+        res["plain_schedule"] = {"seconds": round(t_sum, 3), "runs": len(sets), "plain_hits": hp, "bulge_hits": hb}
+        res["speedup"] = round(t_sum / batched, 2)
+        res["identical_counts"] = hp == res["batched"]["plain_hits"] and hb == res["batched"]["bulge_hits"]
+    st = s.stats()
+    units = len(guides) * st.genome_nucl
+    s.close()
+    out = {"metric": "guide-Gbp/s at bit-exact hits (batched search)", "value": round(units / batched / 1e9, 3),
+           "unit": "guide-Gbp/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(batched * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u32 popcount / u32-u64 bit-parallel Myers", "data": "synthetic",
+           "config": {"workload": f"batched: {total / 1e9:.3f} Gbp repeat-rich synthetic, {len(guides)} guides len 17-30, "
+                                  "PAM {NRG,NGG,NNGRRT,TTTN}, M 0-6, 20% with D,R 1-2; bulge hits counted+checksummed",
+                      "genome_bp": st.genome_nucl}}
+    out.update(res)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,8 +222,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-bp", type=float, default=32e6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "bulge"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "bulge", "batched"], default="plain")
     ap.add_argument("--bulge-guides", type=int, default=100)
+    ap.add_argument("--batched-guides", type=int, default=10_000)
+    ap.add_argument("--compare-plain-schedule", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the search is single-GPU by design "
@@ -150,6 +233,8 @@ def main():
         sys.exit(2)
     if args.workload == "bulge":
         return bulge_workload(args)
+    if args.workload == "batched":
+        return batched_workload(args)
 
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)

@@ -15,7 +15,7 @@ COF_E_IO, COF_E_NOGENOME, COF_E_NOMEM, COF_E_UNSUPPORTED = 4, 5, 6, 7
 COF_FLAG_COUNT_ONLY = 1
 
 # Every symbol include/cof.h declares (checked by the CPU test suite).
-COF_SYMBOLS = ["cof_open", "cof_load_genome_file", "cof_load_genome", "cof_search",
+COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search",
                "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error"]
 
 
@@ -63,6 +63,7 @@ def lib():
             raise RuntimeError("libcof.so is not built (run __graft_entry__.build())")
         L = C.CDLL(path)
         L.cof_open.argtypes = [C.c_int, C.POINTER(CofQuery), C.POINTER(C.c_void_p)]
+        L.cof_set_query.argtypes = [C.c_void_p, C.POINTER(CofQuery)]
         L.cof_load_genome_file.argtypes = [C.c_void_p, C.c_char_p]
         L.cof_load_genome.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(CofChunk), C.c_uint64,
                                       C.POINTER(C.c_char_p), C.c_uint32]
@@ -74,7 +75,7 @@ def lib():
         L.cof_strerror.restype = C.c_char_p
         L.cof_last_error.argtypes = [C.c_void_p]
         L.cof_last_error.restype = C.c_char_p
-        for f in ("cof_open", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
+        for f in ("cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
                   "cof_get_stats"):
             getattr(L, f).restype = C.c_int
         _lib = L
@@ -165,6 +166,14 @@ class Search:
 
     def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0, count_only=False):
         L = lib()
+        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only)
+        h = C.c_void_p()
+        st = L.cof_open(device, C.byref(q), C.byref(h))
+        if st != COF_OK:
+            raise CofError(st, "cof_open")
+        self.h = h
+
+    def _make_query(self, guides, filters, mms, dnas, rnas, batch_nucl, count_only):
         n = len(guides)
         self._keep = [g.encode() for g in guides], [f.encode() for f in filters]
         q = CofQuery()
@@ -177,11 +186,11 @@ class Search:
         q.batch_nucl = batch_nucl
         q.flags = COF_FLAG_COUNT_ONLY if count_only else 0
         self._q = q
-        h = C.c_void_p()
-        st = L.cof_open(device, C.byref(q), C.byref(h))
-        if st != COF_OK:
-            raise CofError(st, "cof_open")
-        self.h = h
+        return q
+
+    def set_query(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, count_only=False):
+        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only)
+        self._check(lib().cof_set_query(self.h, C.byref(q)), "set_query")
 
     def _check(self, st, what):
         if st != COF_OK:

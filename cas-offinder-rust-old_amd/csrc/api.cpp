@@ -19,6 +19,7 @@ struct cof_ctx {
     uint64_t d_genome_words = 0;
     cof::PlainPlan* plain = nullptr;
     cof::BulgePlan* bulge = nullptr;
+    cof::Workspace ws;
     bool have_genome = false;
     bool cached = false;
     std::vector<cof_hit> hits;
@@ -74,6 +75,36 @@ cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out) {
         return s;
     }
     *out = ctx.release();
+    return COF_OK;
+}
+
+cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
+    if (!ctx) return COF_E_INVALID;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        cof::Query nq = cof::prepare_query(q);
+        cof::PlainPlan* np = cof::plain_plan_create(nq);
+        cof::BulgePlan* nb = nullptr;
+        try {
+            nb = cof::bulge_plan_create(nq);
+        } catch (...) {
+            cof::plain_plan_destroy(np);
+            throw;
+        }
+        cof::plain_plan_destroy(ctx->plain);
+        cof::bulge_plan_destroy(ctx->bulge);
+        ctx->plain = np;
+        ctx->bulge = nb;
+        bool batch_changed = nq.batch_nucl != ctx->q.batch_nucl;
+        ctx->q = std::move(nq);
+        if (ctx->have_genome && batch_changed)
+            ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
+        ctx->cached = false;
+        ctx->hits.clear();
+        ctx->aux.clear();
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
     return COF_OK;
 }
 
@@ -135,10 +166,10 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             ctx->hits.clear();
             ctx->aux.clear();
             std::vector<cof::PlainRaw> praw;
-            cof::plain_run(ctx->plain, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw, &st.kernel_ms_plain,
-                           &st.n_plain_launches);
+            cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw,
+                           &st.kernel_ms_plain, &st.n_plain_launches);
             std::vector<cof::BulgeRaw> braw;
-            cof::bulge_run(ctx->bulge, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
+            cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
             st.n_raw_hits = praw.size() + braw.size();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
             size_t first_bulge = ctx->hits.size();

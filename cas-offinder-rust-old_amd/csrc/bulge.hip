@@ -288,6 +288,7 @@ struct Sink {
     unsigned long long* kept;
     unsigned long long* checksum;
     unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop
+    uint32_t* error;                // set when a lane reaches an impossible state
 };
 
 // Reverses the order of the 2-bit ops held in NW words (op 0 lowest), so a
@@ -305,9 +306,11 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                                                           const BCand* __restrict__ queue, uint32_t n_queue,
                                                           uint32_t* __restrict__ q_next, const BPat* __restrict__ pats,
                                                           uint32_t n_cols, Sink sink) {
-    // depth <= window length <= bits of WORD; ops 2 bits and child masks 4 bits per depth
+    // depth d <= L + D = window length <= bits of WORD.  Ops are stored for
+    // depths 0..W-1 (2 bits each); child masks for depths 0..W (4 bits each,
+    // the leaf at depth W included).
     constexpr int kOpsW = sizeof(WORD) / 4;        // u64 words of ops
-    constexpr int kTrsW = sizeof(WORD) / 2;        // u64 words of child masks
+    constexpr int kTrsW = sizeof(WORD) / 2 + 1;    // u64 words of child masks
     extern __shared__ unsigned char verify_lds[];
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
     WORD* cvn = cvp + (size_t)n_cols * 64;
@@ -508,6 +511,12 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                 jj--; cost++; db++;
             }
             d++;
+            if (d > (uint32_t)sizeof(WORD) * 8 || i > L || jj > (uint32_t)sizeof(WORD) * 8) {
+                // impossible state: stop this lane and report instead of looping
+                atomicOr(sink.error, 1u);
+                busy = false;
+                continue;
+            }
             if (i == 0) emit();
             set4(d, children());
         } else if (d == 0) {
@@ -543,24 +552,8 @@ struct BulgePlan {
     std::vector<BPat> pats;
     BGroup *d_g32 = nullptr, *d_g64 = nullptr;
     BPat* d_pats = nullptr;
-    BCand* d_queue = nullptr;
-    uint32_t q_cap = 0;
-    uint32_t* d_qcount = nullptr;
-    BulgeRaw* d_out = nullptr;
-    uint32_t out_cap = 0;
-    uint32_t* d_ocount = nullptr;
-    unsigned long long* d_checksum = nullptr;
-    BTile* d_tiles = nullptr;
-    size_t tiles_cap = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool count_only = false;
-    DChunk* d_chunks = nullptr;
-    size_t chunks_cap = 0;
-    unsigned long long* d_kept = nullptr;
-    uint32_t* d_qnext = nullptr;
-    unsigned long long* d_diag = nullptr;
     uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
-    uint32_t verify_blocks = 0;   // persistent verify grid
 };
 
 BulgePlan* bulge_plan_create(const Query& q) {
@@ -621,34 +614,14 @@ BulgePlan* bulge_plan_create(const Query& q) {
     up(bp->d_g32, bp->groups32);
     up(bp->d_g64, bp->groups64);
     up(bp->d_pats, bp->pats);
-    bp->q_cap = 1u << 27;
-    HIPCHECK(hipMalloc(&bp->d_queue, (size_t)bp->q_cap * sizeof(BCand)));
-    HIPCHECK(hipMalloc(&bp->d_qcount, sizeof(uint32_t)));
-    bp->out_cap = 1u << 22;
-    HIPCHECK(hipMalloc(&bp->d_out, (size_t)bp->out_cap * sizeof(BulgeRaw)));
-    HIPCHECK(hipMalloc(&bp->d_ocount, sizeof(uint32_t)));
-    HIPCHECK(hipMalloc(&bp->d_checksum, sizeof(unsigned long long)));
-    HIPCHECK(hipMalloc(&bp->d_kept, sizeof(unsigned long long)));
-    HIPCHECK(hipMalloc(&bp->d_qnext, sizeof(uint32_t)));
-    HIPCHECK(hipMalloc(&bp->d_diag, 2 * sizeof(unsigned long long)));
     for (const auto& P : bp->pats) bp->n_cols = std::max(bp->n_cols, P.L + 1);
-    {
-        int dev = 0, cus = 0;
-        HIPCHECK(hipGetDevice(&dev));
-        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        bp->verify_blocks = (uint32_t)std::max(1, cus) * 12;
-    }
-    for (auto& e : bp->ev) HIPCHECK(hipEventCreate(&e));
     return bp;
 }
 
 void bulge_plan_destroy(BulgePlan* bp) {
     if (!bp) return;
-    for (void* p : {(void*)bp->d_g32, (void*)bp->d_g64, (void*)bp->d_pats, (void*)bp->d_queue, (void*)bp->d_qcount,
-                    (void*)bp->d_out, (void*)bp->d_ocount, (void*)bp->d_checksum, (void*)bp->d_tiles,
-                    (void*)bp->d_chunks, (void*)bp->d_kept, (void*)bp->d_qnext, (void*)bp->d_diag})
+    for (void* p : {(void*)bp->d_g32, (void*)bp->d_g64, (void*)bp->d_pats})
         if (p) (void)hipFree(p);
-    for (auto& e : bp->ev) if (e) (void)hipEventDestroy(e);
     delete bp;
 }
 
@@ -658,10 +631,11 @@ uint64_t bulge_pad_words() { return (uint64_t)kWinWords + 64; }
 // Runs the filter over every virtual batch's end range, then the traceback
 // over the queued candidates.  Tile ranges are processed in rounds so the
 // candidate queue stays bounded; a round whose queue overflows is split.
-void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches, const uint32_t* d_genome,
-               hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats) {
+void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
+               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats) {
     out.clear();
     if (!bp || bp->pats.empty()) return;
+    ws.ensure_events();
     // tiles of end positions per virtual batch (search.rs:1080-1092, 899-927)
     std::vector<BTile> tiles;
     for (const Batch& b : batches) {
@@ -672,26 +646,35 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
             tiles.push_back(BTile{j0, floor, e, (uint32_t)std::min<uint64_t>(kTileEnds, e - j0), (uint32_t)b.last});
     }
     if (tiles.empty()) return;
-    if (tiles.size() > bp->tiles_cap) {
-        if (bp->d_tiles) (void)hipFree(bp->d_tiles);
-        bp->tiles_cap = tiles.size();
-        HIPCHECK(hipMalloc(&bp->d_tiles, tiles.size() * sizeof(BTile)));
-    }
-    HIPCHECK(hipMemcpyAsync(bp->d_tiles, tiles.data(), tiles.size() * sizeof(BTile), hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemsetAsync(bp->d_checksum, 0, sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(bp->d_kept, 0, sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(bp->d_diag, 0, 2 * sizeof(unsigned long long), st));
+    ws_grow(&ws.tiles, &ws.tiles_cap, tiles.size(), sizeof(BTile));
+    BTile* d_tiles = (BTile*)ws.tiles;
+    HIPCHECK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BTile), hipMemcpyHostToDevice, st));
+    uint32_t* q_count = ws.counters;
+    uint32_t* o_count = ws.counters + 1;
+    uint32_t* q_next = ws.counters + 2;
+    unsigned long long* kept = ws.sums;
+    unsigned long long* checksum = ws.sums + 1;
+    unsigned long long* diag = ws.sums + 2;
+    HIPCHECK(hipMemsetAsync(ws.sums, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHECK(hipMemsetAsync(ws.counters + 3, 0, sizeof(uint32_t), st));
     if (bp->count_only) {
         std::vector<DChunk> dc(g.chunks.size());
         for (size_t c = 0; c < g.chunks.size(); c++)
             dc[c] = DChunk{g.chunks[c].nib_begin(), (uint32_t)g.chunks[c].size(), g.chunks[c].start == 0 ? 1u : 0u};
-        if (dc.size() > bp->chunks_cap) {
-            if (bp->d_chunks) (void)hipFree(bp->d_chunks);
-            bp->chunks_cap = dc.size();
-            HIPCHECK(hipMalloc(&bp->d_chunks, dc.size() * sizeof(DChunk)));
-        }
-        HIPCHECK(hipMemcpyAsync(bp->d_chunks, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice, st));
+        ws_grow(&ws.chunks, &ws.chunks_cap, dc.size(), sizeof(DChunk));
+        HIPCHECK(hipMemcpyAsync(ws.chunks, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice, st));
     }
+    if (!ws.queue) {
+        size_t cap = 0;
+        ws_grow(&ws.queue, &cap, 1u << 22, sizeof(BCand));
+        ws.q_cap = (uint32_t)cap;
+    }
+    if (!ws.bulge_out && !bp->count_only) {
+        size_t cap = 0;
+        ws_grow(&ws.bulge_out, &cap, 1u << 20, sizeof(BulgeRaw));
+        ws.bulge_cap = (uint32_t)cap;
+    }
+    const uint32_t verify_blocks = (uint32_t)std::max(1, ws.cus) * 12;
     // rounds of tiles sized from the candidate density seen so far, so the
     // queue rarely overflows; an overflowing round is split in halves
     size_t next_tile = 0, round_tiles = 1u << 12;
@@ -703,98 +686,105 @@ void bulge_run(BulgePlan* bp, const Genome& g, const std::vector<Batch>& batches
         }
         auto [t0, t1] = rounds.back();
         rounds.pop_back();
-        HIPCHECK(hipMemsetAsync(bp->d_qcount, 0, sizeof(uint32_t), st));
-        HIPCHECK(hipEventRecord(bp->ev[0], st));
+        BCand* d_queue = (BCand*)ws.queue;
+        HIPCHECK(hipMemsetAsync(q_count, 0, sizeof(uint32_t), st));
+        HIPCHECK(hipEventRecord(ws.ev[0], st));
         uint32_t nt = (uint32_t)(t1 - t0);
         if (!bp->groups32.empty())
             hipLaunchKernelGGL(bulge_filter_kernel<uint32_t>, dim3(nt), dim3(kFilterThreads), 0, st, d_genome,
-                               bp->d_tiles + t0, bp->d_g32, (uint32_t)bp->groups32.size(), bp->d_pats, bp->d_queue,
-                               bp->d_qcount, bp->q_cap);
+                               d_tiles + t0, bp->d_g32, (uint32_t)bp->groups32.size(), bp->d_pats, d_queue, q_count,
+                               ws.q_cap);
         if (!bp->groups64.empty())
             hipLaunchKernelGGL(bulge_filter_kernel<uint64_t>, dim3(nt), dim3(kFilterThreads), 0, st, d_genome,
-                               bp->d_tiles + t0, bp->d_g64, (uint32_t)bp->groups64.size(), bp->d_pats, bp->d_queue,
-                               bp->d_qcount, bp->q_cap);
+                               d_tiles + t0, bp->d_g64, (uint32_t)bp->groups64.size(), bp->d_pats, d_queue, q_count,
+                               ws.q_cap);
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipEventRecord(bp->ev[1], st));
+        HIPCHECK(hipEventRecord(ws.ev[1], st));
         uint32_t nq = 0;
-        HIPCHECK(hipMemcpyAsync(&nq, bp->d_qcount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&nq, q_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, bp->ev[0], bp->ev[1]));
+        HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
         if (stats) stats->kernel_ms_filter += ms;
-        if (nq > bp->q_cap) {
+        if (nq > ws.q_cap) {
+            if (ws.q_cap < (1u << 27)) {
+                // grow the queue (up to 128 Mi candidates) and redo the round
+                size_t cap = ws.q_cap;
+                ws_grow(&ws.queue, &cap, std::min<size_t>(1u << 27, (size_t)nq + nq / 4), sizeof(BCand));
+                ws.q_cap = (uint32_t)cap;
+                rounds.push_back({t0, t1});
+                continue;
+            }
             if (t1 - t0 > 1) {           // split the round (search.rs:1140-1152)
                 size_t mid = t0 + (t1 - t0) / 2;
                 rounds.push_back({mid, t1});
                 rounds.push_back({t0, mid});
                 continue;
             }
-            (void)hipFree(bp->d_queue);  // a single tile overflows: grow the queue
-            bp->q_cap = nq + nq / 4;
-            HIPCHECK(hipMalloc(&bp->d_queue, (size_t)bp->q_cap * sizeof(BCand)));
-            rounds.push_back({t0, t1});
-            continue;
+            throw Error(COF_E_NOMEM, "one filter tile produced more than 2^27 candidates");
         }
         if (stats) stats->n_candidates += nq;
         {
             double per_tile = (double)nq / (double)(t1 - t0);
-            double fit = 0.5 * (double)bp->q_cap / std::max(per_tile, 1.0);
+            double fit = 0.5 * (double)(1u << 27) / std::max(per_tile, 1.0);
             round_tiles = (size_t)std::max(1.0, std::min(fit, (double)(1u << 16)));
         }
         if (nq == 0) continue;
         // verify; rerun with a larger record buffer when it overflows
         for (;;) {
-            HIPCHECK(hipMemsetAsync(bp->d_ocount, 0, sizeof(uint32_t), st));
-            HIPCHECK(hipEventRecord(bp->ev[2], st));
-            Sink sk{bp->d_out, bp->d_ocount, bp->out_cap, bp->count_only ? 1 : 0, bp->d_chunks,
-                    (uint32_t)g.chunks.size(), bp->d_kept, bp->d_checksum, bp->d_diag};
+            HIPCHECK(hipMemsetAsync(ws.counters + 1, 0, 2 * sizeof(uint32_t), st));
+            HIPCHECK(hipEventRecord(ws.ev[2], st));
+            Sink sk{(BulgeRaw*)ws.bulge_out, o_count, ws.bulge_cap, bp->count_only ? 1 : 0, (const DChunk*)ws.chunks,
+                    (uint32_t)g.chunks.size(), kept, checksum, diag, ws.counters + 3};
             // persistent verify: lanes refill from the queue until it drains
-            HIPCHECK(hipMemsetAsync(bp->d_qnext, 0, sizeof(uint32_t), st));
-            uint32_t blocks = std::min<uint32_t>((nq + 63) / 64, bp->verify_blocks);
+            uint32_t blocks = std::min<uint32_t>((nq + 63) / 64, verify_blocks);
             size_t lds32 = (size_t)bp->n_cols * 64 * sizeof(uint32_t) * 2;
             size_t lds64 = (size_t)bp->n_cols * 64 * sizeof(uint64_t) * 2;
             if (bp->groups64.empty())
                 hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, d_genome,
-                                   bp->d_queue, nq, bp->d_qnext, bp->d_pats, bp->n_cols, sk);
+                                   d_queue, nq, q_next, bp->d_pats, bp->n_cols, sk);
             else
                 hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, d_genome,
-                                   bp->d_queue, nq, bp->d_qnext, bp->d_pats, bp->n_cols, sk);
+                                   d_queue, nq, q_next, bp->d_pats, bp->n_cols, sk);
             HIPCHECK(hipGetLastError());
-            HIPCHECK(hipEventRecord(bp->ev[3], st));
+            HIPCHECK(hipEventRecord(ws.ev[3], st));
             uint32_t no = 0;
-            HIPCHECK(hipMemcpyAsync(&no, bp->d_ocount, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipMemcpyAsync(&no, o_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             HIPCHECK(hipStreamSynchronize(st));
-            HIPCHECK(hipEventElapsedTime(&ms, bp->ev[2], bp->ev[3]));
+            HIPCHECK(hipEventElapsedTime(&ms, ws.ev[2], ws.ev[3]));
             if (stats) stats->kernel_ms_verify += ms;
-            if (!bp->count_only && no > bp->out_cap) {
-                (void)hipFree(bp->d_out);
-                bp->out_cap = no + no / 4;
-                HIPCHECK(hipMalloc(&bp->d_out, (size_t)bp->out_cap * sizeof(BulgeRaw)));
+            if (!bp->count_only && no > ws.bulge_cap) {
+                size_t cap = ws.bulge_cap;
+                ws_grow(&ws.bulge_out, &cap, (size_t)no + no / 4, sizeof(BulgeRaw));
+                ws.bulge_cap = (uint32_t)cap;
                 continue;
             }
             if (!bp->count_only && no) {
                 size_t old = out.size();
                 out.resize(old + no);
-                HIPCHECK(hipMemcpyAsync(out.data() + old, bp->d_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost, st));
+                HIPCHECK(hipMemcpyAsync(out.data() + old, ws.bulge_out, (size_t)no * sizeof(BulgeRaw),
+                                        hipMemcpyDeviceToHost, st));
                 HIPCHECK(hipStreamSynchronize(st));
             }
             break;
         }
     }
-    if (stats) {
-        unsigned long long dg[2] = {0, 0};
-        HIPCHECK(hipMemcpyAsync(dg, bp->d_diag, sizeof dg, hipMemcpyDeviceToHost, st));
+    {
+        uint32_t err = 0;
+        HIPCHECK(hipMemcpyAsync(&err, ws.counters + 3, sizeof err, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        stats->verify_steps = dg[0];
-        stats->verify_slots = dg[1];
+        if (err) throw Error(COF_E_HIP, "bulge traceback reached an impossible state");
     }
-    if (bp->count_only && stats) {
-        unsigned long long cs = 0, kept = 0;
-        HIPCHECK(hipMemcpyAsync(&cs, bp->d_checksum, sizeof(cs), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipMemcpyAsync(&kept, bp->d_kept, sizeof(kept), hipMemcpyDeviceToHost, st));
+    if (stats) {
+        unsigned long long sums[4] = {0, 0, 0, 0};
+        HIPCHECK(hipMemcpyAsync(sums, ws.sums, sizeof sums, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        stats->bulge_checksum = cs;
-        stats->bulge_hits = kept;
+        stats->verify_steps = sums[2];
+        stats->verify_slots = sums[3];
+        if (bp->count_only) {
+            stats->bulge_hits = sums[0];
+            stats->bulge_checksum = sums[1];
+        }
     }
 }
 

@@ -29,6 +29,7 @@
 #include <map>
 
 #include "cof_internal.hpp"
+#include "engine.hpp"
 #include "hip_check.hpp"
 
 namespace cof {
@@ -260,11 +261,7 @@ struct PlainPlan {
     GroupDev* d_groups = nullptr;
     PatRec* d_pats = nullptr;
     uint32_t* d_full = nullptr;
-    PlainRaw* d_out = nullptr;
-    uint32_t* d_count = nullptr;
-    uint32_t out_cap = 0;
     uint32_t max_L = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 PlainPlan* plain_plan_create(const Query& q) {
@@ -347,11 +344,6 @@ PlainPlan* plain_plan_create(const Query& q) {
         };
         up(pp->d_pats, recs);
         up(pp->d_full, full);
-        pp->out_cap = 1u << 22;
-        HIPCHECK(hipMalloc(&pp->d_out, (size_t)pp->out_cap * sizeof(PlainRaw)));
-        HIPCHECK(hipMalloc(&pp->d_count, sizeof(uint32_t)));
-        HIPCHECK(hipEventCreate(&pp->ev0));
-        HIPCHECK(hipEventCreate(&pp->ev1));
     }
     return pp;
 }
@@ -361,10 +353,6 @@ void plain_plan_destroy(PlainPlan* pp) {
     (void)hipFree(pp->d_groups);
     (void)hipFree(pp->d_pats);
     (void)hipFree(pp->d_full);
-    (void)hipFree(pp->d_out);
-    (void)hipFree(pp->d_count);
-    if (pp->ev0) (void)hipEventDestroy(pp->ev0);
-    if (pp->ev1) (void)hipEventDestroy(pp->ev1);
     delete pp;
 }
 
@@ -375,14 +363,46 @@ uint64_t plain_pad_words() { return kTileWords + 16; }
 
 // Scans start positions [0, scan_end) of the resident genome.  Device words
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
-void plain_run(PlainPlan* pp, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
+Workspace::~Workspace() {
+    for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums})
+        if (p) (void)hipFree(p);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+}
+
+void Workspace::ensure_events() {
+    if (ev[0]) return;
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipMalloc(&counters, 4 * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&sums, 4 * sizeof(unsigned long long)));
+    HIPCHECK(hipMalloc(&plain_count, sizeof(uint32_t)));
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+}
+
+void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem) {
+    if (need <= *cap && *ptr) return;
+    if (*ptr) (void)hipFree(*ptr);
+    *ptr = nullptr;
+    HIPCHECK(hipMalloc(ptr, need * elem));
+    *cap = need;
+}
+
+void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
                std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches) {
     out.clear();
     if (plain_plan_empty(pp) || scan_end == 0) return;
+    ws.ensure_events();
+    if (!ws.plain_out) {
+        size_t cap = 0;
+        ws_grow(&ws.plain_out, &cap, 1u << 20, sizeof(PlainRaw));
+        ws.plain_cap = (uint32_t)cap;
+    }
     uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
     for (;;) {
-        HIPCHECK(hipMemsetAsync(pp->d_count, 0, sizeof(uint32_t), st));
-        HIPCHECK(hipEventRecord(pp->ev0, st));
+        PlainRaw* d_out = (PlainRaw*)ws.plain_out;
+        HIPCHECK(hipMemsetAsync(ws.plain_count, 0, sizeof(uint32_t), st));
+        HIPCHECK(hipEventRecord(ws.ev[0], st));
         for (const auto& l : pp->launches) {
             // grids of at most 2^20 tiles keep the tile index in 32 bits
             for (uint64_t t0 = 0; t0 < n_tiles; t0 += (1u << 20)) {
@@ -392,8 +412,8 @@ void plain_run(PlainPlan* pp, const uint32_t* d_genome, uint64_t scan_end, hipSt
 #define COF_PLAIN_LAUNCH(WW)                                                                              \
     case WW:                                                                                              \
         hipLaunchKernelGGL(plain_scan_kernel<WW>, dim3(nb), dim3(kThreads), 0, st, d_genome, scan_end, first, g, \
-                           l.n_groups, pp->d_pats, pp->d_full, pp->d_out,                                 \
-                           pp->d_count, pp->out_cap);                                                    \
+                           l.n_groups, pp->d_pats, pp->d_full, d_out,                                     \
+                           ws.plain_count, ws.plain_cap);                                                \
         break;
                 switch (l.W) {
                     COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
@@ -405,22 +425,23 @@ void plain_run(PlainPlan* pp, const uint32_t* d_genome, uint64_t scan_end, hipSt
                 if (n_launches) (*n_launches)++;
             }
         }
-        HIPCHECK(hipEventRecord(pp->ev1, st));
+        HIPCHECK(hipEventRecord(ws.ev[1], st));
         uint32_t cnt = 0;
-        HIPCHECK(hipMemcpyAsync(&cnt, pp->d_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&cnt, ws.plain_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, pp->ev0, pp->ev1));
+        HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
         if (kernel_ms) *kernel_ms += ms;
-        if (cnt > pp->out_cap) {
+        if (cnt > ws.plain_cap) {
             // too many hits for the record buffer: grow it and scan again
-            (void)hipFree(pp->d_out);
-            pp->out_cap = (uint32_t)std::min<uint64_t>((uint64_t)cnt + cnt / 4 + 1024, 0xFFFFFFF0ull);
-            HIPCHECK(hipMalloc(&pp->d_out, (size_t)pp->out_cap * sizeof(PlainRaw)));
+            size_t cap = ws.plain_cap;
+            ws_grow(&ws.plain_out, &cap, std::min<uint64_t>((uint64_t)cnt + cnt / 4 + 1024, 0xFFFFFFF0ull),
+                    sizeof(PlainRaw));
+            ws.plain_cap = (uint32_t)cap;
             continue;
         }
         out.resize(cnt);
-        if (cnt) HIPCHECK(hipMemcpyAsync(out.data(), pp->d_out, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
+        if (cnt) HIPCHECK(hipMemcpyAsync(out.data(), d_out, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         return;
     }

@@ -13,6 +13,7 @@
 // (search.rs:1316-1374) is applied and rows are formatted (main.rs:145-157).
 #include <algorithm>
 #include <cstdio>
+#include <thread>
 
 #include "cof_internal.hpp"
 
@@ -59,10 +60,32 @@ bool plain_predicate(const Genome& g, const Pattern& p, uint64_t pos, uint64_t v
 
 }  // namespace
 
+namespace {
+// Runs fn(begin, end, out, aux) over up to 16 slices of [0, n) in parallel and
+// concatenates the slices' outputs in order.
+template <typename Fn>
+void parallel_convert(size_t n, std::vector<cof_hit>& out, std::vector<HitAux>& aux, Fn fn) {
+    unsigned nt = n < 65536 ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<cof_hit>> po(nt);
+    std::vector<std::vector<HitAux>> pa(nt);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t]() { fn(n * t / nt, n * (t + 1) / nt, po[t], pa[t]); });
+    for (auto& x : th) x.join();
+    for (unsigned t = 0; t < nt; t++) {
+        out.insert(out.end(), po[t].begin(), po[t].end());
+        aux.insert(aux.end(), pa[t].begin(), pa[t].end());
+    }
+}
+}  // namespace
+
 void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs,
-                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux) {
+                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out_all, std::vector<HitAux>& aux_all) {
     BatchIndex bi(g, bs);
-    for (const auto& r : raw) {
+    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, std::vector<cof_hit>& out,
+                                                       std::vector<HitAux>& aux) {
+    for (size_t ri = b0; ri < b1; ri++) {
+        const PlainRaw& r = raw[ri];
         const Pattern& p = q.pats[r.pattern];
         size_t c = g.chunk_of(r.pos);
         const Chunk& ch = g.chunks[c];
@@ -102,12 +125,16 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
         out.push_back(h);
         aux.push_back({r.pos, view_end});
     }
+    });
 }
 
 void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs,
-                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux) {
+                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out_all, std::vector<HitAux>& aux_all) {
     BatchIndex bi(g, bs);
-    for (const auto& r : raw) {
+    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, std::vector<cof_hit>& out,
+                                                       std::vector<HitAux>& aux) {
+    for (size_t ri = b0; ri < b1; ri++) {
+        const BulgeRaw& r = raw[ri];
         const Pattern& p = q.pats[r.pattern];
         uint64_t span = (uint64_t)p.L + r.db - r.rb;
         uint64_t j = r.start + span - 1;
@@ -134,6 +161,7 @@ void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs
         out.push_back(h);
         aux.push_back({r.start, bi.end_nib[b]});
     }
+    });
 }
 
 // Rows of main.rs:130-247.

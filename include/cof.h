@@ -115,6 +115,11 @@ typedef struct {
  * validates the query, builds per-pattern tables and uploads them. */
 cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out);
 
+/* Replaces the query of an open context; the resident genome stays loaded, so
+ * many queries can run against one upload (each reference run re-reads the
+ * genome, main.rs:106-121). */
+cof_status cof_set_query(cof_ctx* ctx, const cof_query* q);
+
 /* Makes a genome resident in HBM from a FASTA file, a 2bit file or a folder of
  * *.fa files, with the reference readers' chunking (read_fasta.rs:9-71,
  * read_2bit.rs:33-118, read_fasta_folder.rs:8-23, main.rs:106-121). */

@@ -126,3 +126,41 @@ def test_count_only_equals_record_mode(tmp_path):
     assert res[0][0] > 1000
     assert res[0][:2] == res[1][:2]
     assert res[0][2] == res[0][0] and res[1][2] == 0
+
+
+def test_set_query_reuses_resident_genome(tmp_path):
+    # a re-queried context gives the same rows as a fresh one
+    g1, g2 = "GGCCGACCTGTCGCTGACGC", "ACCTGTCGCTGACGCGGCCG"
+    sg = cofpy.SynthGenome(13, [90_000, 30_000], plants=[(g1 + "AGG", 0, 20, 3, 20), (g2 + "TTTA", 0, 20, 2, 20)])
+    f1, f2 = "NNNNNNNNNNNNNNNNNNNNNRG", "NNNNNNNNNNNNNNNNNNNNNNGRRT"
+    a = cofpy.Search([g1 + "NNN"], [f1], [4], [1], [1])
+    a.load_synth(sg)
+    a.search()
+    a.set_query([g2 + "NNNNNN"], [f2], [3])
+    a.search()
+    a.write_tsv(tmp_path / "a.tsv")
+    b = cofpy.Search([g2 + "NNNNNN"], [f2], [3])
+    b.load_synth(sg)
+    b.search()
+    b.write_tsv(tmp_path / "b.tsv")
+    assert cofpy.read_rows(tmp_path / "a.tsv") == cofpy.read_rows(tmp_path / "b.tsv")
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("glen,pam,D,R", [(25, "NNGRRT", 1, 2), (26, "NNGRRT", 0, 1), (27, "NGG", 2, 1),
+                                          (58, "NGG", 3, 1), (59, "NRG", 2, 2)])
+def test_window_width_edges(tmp_path, glen, pam, D, R):
+    # L + D = 32 and 64 are the last windows of the 32- and 64-bit kernels
+    rng = random.Random(glen * 3 + D)
+    g = "".join(rng.choice("ACGT") for _ in range(glen))
+    filt = "N" * glen + pam
+    sg = cofpy.SynthGenome(31 + glen, [30_000, 12_000], n_frac=0.01, gap=(5, 50),
+                           plants=[(g + pam, 0, glen, 3, 25)])
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    M = 3 if glen < 40 else 1
+    want = oracle_rows(tmp_path, fa, f"{filt} {D} {R}", [g + "N" * len(pam)], M, 1 << 22)
+    got, _ = gpu_rows(tmp_path, fa, filt, [g + "N" * len(pam)], [M], [D], [R], 1 << 22)
+    assert len(want) > 0
+    assert got == want
