@@ -99,8 +99,11 @@ def bulge_workload(args):
     log(f"repeat-rich genome {total / 1e9:.2f} Gbp generated in {time.time() - t:.1f} s")
     pats = [g + "NNN" for g in guides]
     n = len(pats)
-    s = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * n, [4] * n, [2] * n, [2] * n, count_only=True)
+    s = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * n, [4] * n, [2] * n, [2] * n, count_only=True,
+                     flags=schedule_flags(args))
     s.load_synth(sg)
+    if args.autotune:
+        log(f"autotune: {s.autotune()}")
     for w in range(args.warmup):
         s.search()
         log(f"warmup {w + 1} done")
@@ -113,13 +116,14 @@ def bulge_workload(args):
         filt += st.kernel_ms_filter
         ver += st.kernel_ms_verify
         log(f"step {k + 1}: {times[-1]:.3f} s, candidates {st.n_candidates}, hits {st.bulge_hits}, "
-            f"filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms, "
+            f"bulge path {st.bulge_ms:.1f} ms ({st.n_pipeline_rounds} pipeline rounds, {st.n_redo_rounds} redone; "
+            f"synchronous rounds: filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms), "
             f"dfs steps {st.verify_steps} ({st.verify_steps / max(st.n_candidates, 1):.1f}/cand), "
             f"lane use {st.verify_steps / max(st.verify_slots, 1):.3f}")
     st = s.stats()
     step = sum(times) / len(times)
     units = n * st.genome_nucl
-    out = {"metric": "guide-Gbp/s at bit-exact hits (bulge search, plain schedule)",
+    out = {"metric": "guide-Gbp/s at bit-exact hits (bulge search)",
            "value": round(units / step / 1e9, 3), "unit": "guide-Gbp/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(step * 1e3, 1), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "u32/u64 (bit-parallel Myers, u8 DP bound)", "data": "synthetic",
@@ -127,9 +131,14 @@ def bulge_workload(args):
                                   f"at 10% divergence, {n} guides NRG, M4 D2 R2, count+checksum sink",
                       "genome_bp": st.genome_nucl},
            "candidates": st.n_candidates, "bulge_hits": st.bulge_hits, "checksum": st.bulge_checksum,
+           "schedule": args.schedule, "tuning": s.tuning(), "pipeline_rounds": st.n_pipeline_rounds,
            "kernel_ms_filter": round(filt / args.steps, 1), "kernel_ms_verify": round(ver / args.steps, 1)}
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def schedule_flags(args):
+    return {"graph": 0, "eager": cofpy.COF_FLAG_NO_GRAPH, "sync": cofpy.COF_FLAG_SYNC_ROUNDS}[args.schedule]
 
 
 def batched_guides(seed, n):
@@ -160,8 +169,11 @@ def batched_workload(args):
                            repeats=(0.10, 20, 300, 0.10), plants=plants)
     log(f"repeat-rich genome {total / 1e9:.3f} Gbp, {len(guides)} guides, generated in {time.time() - t:.1f} s")
     pats, filts, mms, ds, rs = zip(*[(g[0], g[1], g[2], g[3], g[4]) for g in guides])
-    s = cofpy.Search(list(pats), list(filts), list(mms), list(ds), list(rs), count_only=True)
+    s = cofpy.Search(list(pats), list(filts), list(mms), list(ds), list(rs), count_only=True,
+                     flags=schedule_flags(args))
     s.load_synth(sg)
+    if args.autotune:
+        log(f"autotune: {s.autotune()}")
     res = {}
     # batched: one query
     times = []
@@ -174,7 +186,7 @@ def batched_workload(args):
             times.append(dt)
         log(f"batched {'warmup' if k < args.warmup else 'step'}: {dt:.3f} s, plain hits {n_plain}, "
             f"bulge hits {st.bulge_hits}, candidates {st.n_candidates}, plain {st.kernel_ms_plain:.1f} ms, "
-            f"filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms")
+            f"bulge path {st.bulge_ms:.1f} ms ({st.n_pipeline_rounds} pipeline rounds)")
     batched = sum(times) / len(times)
     res["batched"] = {"seconds": round(batched, 3), "plain_hits": n_plain, "bulge_hits": st.bulge_hits,
                       "bulge_checksum": st.bulge_checksum}
@@ -198,7 +210,9 @@ def batched_workload(args):
         log(f"plain schedule: {len(sets)} runs, {t_sum:.3f} s, plain hits {hp}, bulge hits {hb}")
         res["plain_schedule"] = {"seconds": round(t_sum, 3), "runs": len(sets), "plain_hits": hp, "bulge_hits": hb}
         res["speedup"] = round(t_sum / batched, 2)
+        res["plain_schedule"]["bulge_checksum"] = cs
         res["identical_counts"] = hp == res["batched"]["plain_hits"] and hb == res["batched"]["bulge_hits"]
+        res["identical_checksum"] = cs == res["batched"]["bulge_checksum"]
     st = s.stats()
     units = len(guides) * st.genome_nucl
     s.close()
@@ -226,6 +240,9 @@ def main():
     ap.add_argument("--bulge-guides", type=int, default=100)
     ap.add_argument("--batched-guides", type=int, default=10_000)
     ap.add_argument("--compare-plain-schedule", action="store_true")
+    ap.add_argument("--schedule", choices=["graph", "eager", "sync"], default="graph",
+                    help="bulge rounds: hipGraph pipeline, eager pipeline, or synchronous rounds")
+    ap.add_argument("--autotune", action="store_true", help="run the autotuner on a 64 Mnt prefix first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the search is single-GPU by design "

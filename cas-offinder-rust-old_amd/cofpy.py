@@ -13,10 +13,28 @@ BUILD = os.path.join(HERE, "_build")
 COF_OK, COF_E_INVALID, COF_E_HIP, COF_E_OVERFLOW = 0, 1, 2, 3
 COF_E_IO, COF_E_NOGENOME, COF_E_NOMEM, COF_E_UNSUPPORTED = 4, 5, 6, 7
 COF_FLAG_COUNT_ONLY = 1
+COF_FLAG_NO_GRAPH = 2
+COF_FLAG_SYNC_ROUNDS = 4
+COF_FLAG_TILE_STATS = 8
 
 # Every symbol include/cof.h declares (checked by the CPU test suite).
 COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search",
-               "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error"]
+               "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error",
+               "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats"]
+
+
+class CofTuning(C.Structure):
+    _fields_ = [("plain_threads", C.c_uint32), ("plain_pat_chunk", C.c_uint32), ("bulge_first_tiles", C.c_uint32),
+                ("bulge_round_tiles", C.c_uint32), ("verify_blocks_per_cu", C.c_uint32),
+                ("bulge_queue_cap", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CofTileStat(C.Structure):
+    _fields_ = [("first_end", C.c_uint64), ("n_ends", C.c_uint32), ("candidates", C.c_uint32),
+                ("dfs_steps", C.c_uint64)]
 
 
 class CofQuery(C.Structure):
@@ -47,7 +65,8 @@ class CofStats(C.Structure):
                 ("n_plain_launches", C.c_uint64), ("n_candidates", C.c_uint64),
                 ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64),
                 ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
-                ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64)]
+                ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
+                ("n_redo_rounds", C.c_uint64)]
 
 
 _lib = None
@@ -75,8 +94,12 @@ def lib():
         L.cof_strerror.restype = C.c_char_p
         L.cof_last_error.argtypes = [C.c_void_p]
         L.cof_last_error.restype = C.c_char_p
+        L.cof_set_tuning.argtypes = [C.c_void_p, C.POINTER(CofTuning)]
+        L.cof_get_tuning.argtypes = [C.c_void_p, C.POINTER(CofTuning)]
+        L.cof_autotune.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(CofTuning)]
+        L.cof_get_tile_stats.argtypes = [C.c_void_p, C.POINTER(CofTileStat), C.c_uint64, C.POINTER(C.c_uint64)]
         for f in ("cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
-                  "cof_get_stats"):
+                  "cof_get_stats", "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -164,16 +187,17 @@ class CofError(RuntimeError):
 class Search:
     """One libcof context: query -> resident genome -> hits."""
 
-    def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0, count_only=False):
+    def __init__(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, device=0, count_only=False,
+                 flags=0):
         L = lib()
-        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only)
+        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only, flags)
         h = C.c_void_p()
         st = L.cof_open(device, C.byref(q), C.byref(h))
         if st != COF_OK:
             raise CofError(st, "cof_open")
         self.h = h
 
-    def _make_query(self, guides, filters, mms, dnas, rnas, batch_nucl, count_only):
+    def _make_query(self, guides, filters, mms, dnas, rnas, batch_nucl, count_only, flags=0):
         n = len(guides)
         self._keep = [g.encode() for g in guides], [f.encode() for f in filters]
         q = CofQuery()
@@ -184,12 +208,12 @@ class Search:
         q.max_dna_bulges = (C.c_uint32 * n)(*(dnas or [0] * n))
         q.max_rna_bulges = (C.c_uint32 * n)(*(rnas or [0] * n))
         q.batch_nucl = batch_nucl
-        q.flags = COF_FLAG_COUNT_ONLY if count_only else 0
+        q.flags = (COF_FLAG_COUNT_ONLY if count_only else 0) | flags
         self._q = q
         return q
 
-    def set_query(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, count_only=False):
-        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only)
+    def set_query(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, count_only=False, flags=0):
+        q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only, flags)
         self._check(lib().cof_set_query(self.h, C.byref(q)), "set_query")
 
     def _check(self, st, what):
@@ -224,6 +248,30 @@ class Search:
         s = CofStats()
         lib().cof_get_stats(self.h, C.byref(s))
         return s
+
+    def set_tuning(self, **kw):
+        t = CofTuning(**kw)
+        self._check(lib().cof_set_tuning(self.h, C.byref(t)), "set_tuning")
+
+    def tuning(self):
+        t = CofTuning()
+        self._check(lib().cof_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def tile_stats(self):
+        """numpy structured array (first_end, n_ends, candidates, dfs_steps) of the last bulge search."""
+        import numpy as np
+        n = C.c_uint64()
+        self._check(lib().cof_get_tile_stats(self.h, None, 0, C.byref(n)), "tile_stats")
+        buf = (CofTileStat * max(1, n.value))()
+        self._check(lib().cof_get_tile_stats(self.h, buf, n.value, C.byref(n)), "tile_stats")
+        dt = np.dtype([("first_end", "<u8"), ("n_ends", "<u4"), ("candidates", "<u4"), ("dfs_steps", "<u8")])
+        return np.frombuffer(bytes(buf), dtype=dt, count=n.value).copy()
+
+    def autotune(self, sample_nucl=0):
+        t = CofTuning()
+        self._check(lib().cof_autotune(self.h, sample_nucl, C.byref(t)), "autotune")
+        return t.as_dict()
 
     def close(self):
         if getattr(self, "h", None):

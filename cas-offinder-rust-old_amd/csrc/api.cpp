@@ -20,6 +20,9 @@ struct cof_ctx {
     cof::PlainPlan* plain = nullptr;
     cof::BulgePlan* bulge = nullptr;
     cof::Workspace ws;
+    std::vector<uint32_t> tile_order;   // bulge tile processing order (cost model)
+    cof::PlainTune plain_tune;
+    cof::BulgeTune bulge_tune;
     bool have_genome = false;
     bool cached = false;
     std::vector<cof_hit> hits;
@@ -52,6 +55,88 @@ void upload_genome(cof_ctx* ctx) {
     ctx->cached = false;
     ctx->stats.genome_nucl = 0;
     for (auto& c : ctx->g.chunks) ctx->stats.genome_nucl += c.size();
+}
+
+cof_tuning get_tuning(const cof_ctx* ctx) {
+    cof_tuning t{};
+    t.plain_threads = ctx->plain_tune.threads;
+    t.plain_pat_chunk = ctx->plain_tune.pat_chunk;
+    t.bulge_first_tiles = ctx->bulge_tune.first_tiles;
+    t.bulge_round_tiles = ctx->bulge_tune.round_tiles;
+    t.verify_blocks_per_cu = ctx->bulge_tune.verify_per_cu;
+    t.bulge_queue_cap = ctx->bulge_tune.queue_cap;
+    return t;
+}
+
+void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
+    cof::PlainTune pt;
+    cof::BulgeTune bt;
+    if (t.plain_threads) {
+        if (t.plain_threads != 128 && t.plain_threads != 256 && t.plain_threads != 512)
+            throw cof::Error(COF_E_INVALID, "plain_threads must be 128, 256 or 512");
+        pt.threads = t.plain_threads;
+    }
+    pt.pat_chunk = t.plain_pat_chunk;
+    if (t.bulge_first_tiles) bt.first_tiles = t.bulge_first_tiles;
+    bt.round_tiles = t.bulge_round_tiles;
+    if (t.verify_blocks_per_cu) {
+        if (t.verify_blocks_per_cu > 64) throw cof::Error(COF_E_INVALID, "verify_blocks_per_cu must be <= 64");
+        bt.verify_per_cu = t.verify_blocks_per_cu;
+    }
+    if (t.bulge_queue_cap) {
+        if (t.bulge_queue_cap < 64 || t.bulge_queue_cap > (1u << 27))
+            throw cof::Error(COF_E_INVALID, "bulge_queue_cap must be in [64, 2^27]");
+        bt.queue_cap = t.bulge_queue_cap;
+    }
+    ctx->plain_tune = pt;
+    ctx->bulge_tune = bt;
+    ctx->cached = false;
+}
+
+// Times each candidate shape on a genome prefix and keeps the fastest:
+// plain-scan workgroup size x patterns per workgroup (kernel time of a scan
+// of the prefix), then traceback workgroups per CU (wall time of the bulge
+// path over the prefix's batches).
+void autotune(cof_ctx* ctx, uint64_t sample) {
+    const uint64_t n = std::min<uint64_t>(sample, ctx->g.n_nibbles());
+    if (!cof::plain_plan_empty(ctx->plain)) {
+        std::vector<cof::PlainRaw> raw;
+        double best = 1e300;
+        cof::PlainTune best_t = ctx->plain_tune;
+        for (uint32_t th : {128u, 256u, 512u}) {
+            for (uint32_t pc : {0u, 16u, 64u}) {
+                cof::PlainTune t;
+                t.threads = th;
+                t.pat_chunk = pc;
+                double ms = 0, warm = 0;
+                uint64_t nl = 0;
+                cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, n, ctx->stream, raw, &warm, &nl, t);
+                for (int r = 0; r < 2; r++)
+                    cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, n, ctx->stream, raw, &ms, &nl, t);
+                if (ms < best) { best = ms; best_t = t; }
+            }
+        }
+        ctx->plain_tune = best_t;
+    }
+    if (ctx->bulge && cof::bulge_plan_size(ctx->bulge)) {
+        // the batches of the prefix's chunks
+        size_t nc = 0;
+        while (nc < ctx->g.chunks.size() && ctx->g.chunks[nc].nib_begin() < n) nc++;
+        std::vector<cof::Batch> sb = cof::make_batches(std::max<size_t>(nc, 1), ctx->q.batch_nucl);
+        std::vector<cof::BulgeRaw> raw;
+        double best = 1e300;
+        cof::BulgeTune best_t = ctx->bulge_tune;
+        for (uint32_t v : {4u, 8u, 12u, 16u, 24u}) {
+            cof::BulgeTune t = ctx->bulge_tune;
+            t.verify_per_cu = v;
+            t.flags = ctx->q.flags;
+            cof_stats st{};
+            cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, sb, ctx->d_genome, ctx->stream, raw, &st, t);
+            if (st.bulge_ms < best) { best = st.bulge_ms; best_t.verify_per_cu = v; }
+        }
+        ctx->bulge_tune = best_t;
+    }
+    ctx->cached = false;
 }
 
 }  // namespace
@@ -167,9 +252,12 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             ctx->aux.clear();
             std::vector<cof::PlainRaw> praw;
             cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw,
-                           &st.kernel_ms_plain, &st.n_plain_launches);
+                           &st.kernel_ms_plain, &st.n_plain_launches, ctx->plain_tune);
             std::vector<cof::BulgeRaw> braw;
-            cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st);
+            cof::BulgeTune bt = ctx->bulge_tune;
+            bt.flags = ctx->q.flags;
+            bt.order = ctx->tile_order.empty() ? nullptr : &ctx->tile_order;
+            cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
             st.n_raw_hits = praw.size() + braw.size();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
             size_t first_bulge = ctx->hits.size();
@@ -178,7 +266,8 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
                 st.bulge_hits = ctx->hits.size() - first_bulge;
                 for (size_t h = first_bulge; h < ctx->hits.size(); h++) {
                     const cof_hit& x = ctx->hits[h];
-                    st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, x.pattern, x.mismatches, x.dna_bulge,
+                    st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, cof::pattern_id_of(ctx->q.pats[x.pattern]),
+                                                             x.mismatches, x.dna_bulge,
                                                              x.rna_bulge, x.n_ops, x.ops[0], x.ops[1], x.ops[2]);
                 }
             }
@@ -213,6 +302,57 @@ cof_status cof_write_tsv(cof_ctx* ctx, const char* path) {
 cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out) {
     if (!ctx || !out) return COF_E_INVALID;
     *out = ctx->stats;
+    return COF_OK;
+}
+
+cof_status cof_set_tuning(cof_ctx* ctx, const cof_tuning* t) {
+    if (!ctx || !t) return COF_E_INVALID;
+    try {
+        set_tuning(ctx, *t);
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_get_tile_stats(const cof_ctx* ctx, cof_tile_stat* out, uint64_t cap, uint64_t* n) {
+    if (!ctx || (!out && cap)) return COF_E_INVALID;
+    try {
+        const cof::Workspace& ws = ctx->ws;
+        const uint64_t nt = ws.tile_j0.size();
+        if (n) *n = nt;
+        const uint64_t k = std::min<uint64_t>(cap, nt);
+        if (k) {
+            std::vector<uint32_t> cand(k);
+            std::vector<unsigned long long> steps(k);
+            HIPCHECK(hipSetDevice(ctx->device));
+            HIPCHECK(hipMemcpy(cand.data(), ws.tile_cand, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(steps.data(), ws.tile_steps, k * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            for (uint64_t t = 0; t < k; t++) out[t] = cof_tile_stat{ws.tile_j0[t], ws.tile_n[t], cand[t], steps[t]};
+        }
+        if (cap < nt && out) return COF_E_OVERFLOW;
+    } catch (const cof::Error& e) {
+        return COF_E_HIP;
+    }
+    return COF_OK;
+}
+
+cof_status cof_get_tuning(const cof_ctx* ctx, cof_tuning* out) {
+    if (!ctx || !out) return COF_E_INVALID;
+    *out = get_tuning(ctx);
+    return COF_OK;
+}
+
+cof_status cof_autotune(cof_ctx* ctx, uint64_t sample_nucl, cof_tuning* chosen) {
+    if (!ctx) return COF_E_INVALID;
+    if (!ctx->have_genome) return COF_E_NOGENOME;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        autotune(ctx, sample_nucl ? sample_nucl : (64ull << 20));
+        if (chosen) *chosen = get_tuning(ctx);
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
     return COF_OK;
 }
 

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <tuple>
 
@@ -37,6 +38,7 @@ namespace cof {
 namespace {
 
 constexpr int kFilterThreads = 256;
+constexpr int kStage = 512;                  // staged filter survivors per wave
 constexpr int kTileEnds = 4096;          // end positions per filter workgroup
 constexpr int kMaxW = 64;                // text window (L + D) supported
 constexpr int kMaxR = 16;
@@ -55,6 +57,7 @@ struct BPat {
     uint64_t pa, pc, pg, pt;  // pattern bit4 planes over positions
     uint64_t fa, fc, fg, ft;  // effective-filter bit4 planes over positions
     uint64_t is_n;            // ASCII N / n positions
+    uint64_t hash_id;         // pattern_id_of (checksum key)
     uint32_t L, M, D, R, E, pid;
 };
 
@@ -71,6 +74,8 @@ struct BCand {
     uint64_t floor;
     uint32_t pat;             // index into the BPat table
     uint32_t last_chunk;
+    uint32_t tile;            // filter tile (work accounting for the cost model)
+    uint32_t pad;
 };
 
 // Chunk table for the device-side drop rule of convert_matches_myers
@@ -187,19 +192,47 @@ template <typename WORD>
 __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     const uint32_t* __restrict__ genome, const BTile* __restrict__ tiles, const BGroup* __restrict__ groups,
     uint32_t n_groups, const BPat* __restrict__ pats, BCand* __restrict__ queue, uint32_t* __restrict__ q_count,
-    uint32_t q_cap) {
+    uint32_t q_cap, const uint32_t* __restrict__ tile_ids, uint32_t* __restrict__ tile_cand) {
     __shared__ uint32_t gw[kWinWords];
     __shared__ uint16_t cand[kTileEnds];
     __shared__ uint32_t n_cand;
+    // per-wave staging of Myers survivors: (end offset << 20 | pattern - group
+    // base); a wave reserves queue space once per flush, not per pattern
+    __shared__ uint32_t stage[kFilterThreads / 64][kStage];
 
-    const BTile T = tiles[blockIdx.x];
+    const uint32_t tile_id = tile_ids[blockIdx.x];
+    const BTile T = tiles[tile_id];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    uint32_t my_cand = 0;
+    uint32_t* my_stage = stage[tid >> 6];
+    uint32_t n_st = 0;   // staged entries of this wave (wave-uniform)
     // LDS window: from the lowest text start any end in the tile can use
     const uint64_t lo = T.j0 > T.floor + (kMaxW + 8) ? T.j0 - (kMaxW + 8) : T.floor;
     const uint64_t wlo = lo >> 3;
     const uint64_t base = wlo * 8;
     for (uint32_t i = tid; i < (uint32_t)kWinWords; i += kFilterThreads) gw[i] = genome[wlo + i];
     __syncthreads();
+
+    // copies the wave's staged survivors to the queue; once the queue count
+    // exceeds q_cap it stops growing (the host redoes such a round), so the
+    // 32-bit count cannot wrap
+    auto flush = [&](uint32_t pat_base) {
+        __builtin_amdgcn_wave_barrier();
+        uint32_t qb = q_cap;
+        if (lane == 0) {
+            if (__hip_atomic_load(q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= q_cap)
+                qb = atomicAdd(q_count, n_st);
+            my_cand += n_st;
+        }
+        qb = __shfl(qb, 0);
+        for (uint32_t i = lane; i < n_st; i += 64) {
+            const uint32_t v = my_stage[i];
+            if (qb + i < q_cap)
+                queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, pat_base + (v & 0xFFFFFu), T.last_chunk, tile_id, 0u};
+        }
+        n_st = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const BGroup& G = groups[gi];
@@ -266,14 +299,23 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                         myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
                 }
                 int32_t score = (int32_t)L + (int32_t)popc_w<WORD>(VP & nmask) - (int32_t)popc_w<WORD>(VN & nmask);
-                if (active && score <= (int32_t)P.E) {
-                    uint32_t idx = atomicAdd(q_count, 1u);
-                    if (idx < q_cap) queue[idx] = BCand{j, T.floor, p, T.last_chunk};
+                const bool pass = active && score <= (int32_t)P.E;
+                const uint64_t pb = __ballot(pass);
+                if (pb) {
+                    const uint32_t off =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
+                    if (pass) my_stage[n_st + off] = (e << 20) | (p - G.pat_begin);
+                    n_st += (uint32_t)__popcll(pb);
+                    if (n_st > (uint32_t)kStage - 64u) flush(G.pat_begin);
                 }
             }
         }
+        if (n_st) flush(G.pat_begin);
         __syncthreads();
     }
+    // candidates of this tile (one workgroup per tile)
+    for (int off = 32; off > 0; off >>= 1) my_cand += __shfl_down(my_cand, off);
+    if (lane == 0 && my_cand) atomicAdd(&tile_cand[tile_id], my_cand);
 }
 
 // Hit sink: records (count <= cap written), or, in count-only mode, the
@@ -289,7 +331,20 @@ struct Sink {
     unsigned long long* checksum;
     unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop
     uint32_t* error;                // set when a lane reaches an impossible state
+    unsigned long long* tile_steps; // DFS transitions per filter tile (cost model targets)
 };
+
+// Clears tile_cand of the tiles order[0, n) before they are filtered again.
+__global__ void clear_tile_cand_kernel(const uint32_t* __restrict__ order, uint32_t n, uint32_t* __restrict__ tile_cand) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tile_cand[order[i]] = 0;
+}
+
+// Clears n 32-bit words (graph-safe replacement for a memset node).
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
 
 // Reverses the order of the 2-bit ops held in NW words (op 0 lowest), so a
 // stack pushed from the alignment's end reads in alignment order.
@@ -303,9 +358,19 @@ __device__ __forceinline__ uint64_t rev2(uint64_t x) {
 
 template <typename WORD>
 __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __restrict__ genome,
-                                                          const BCand* __restrict__ queue, uint32_t n_queue,
+                                                          const BCand* __restrict__ queue,
+                                                          const uint32_t* __restrict__ q_count, uint32_t q_cap,
+                                                          uint32_t* __restrict__ overflow,
                                                           uint32_t* __restrict__ q_next, const BPat* __restrict__ pats,
                                                           uint32_t n_cols, Sink sink) {
+    // the candidate count comes from device memory so rounds need no host
+    // round trip; an overflowed queue is skipped and redone by the host
+    const uint32_t n_queue = *q_count;
+    if (n_queue > q_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (blockIdx.x * 64u >= n_queue) return;
     // depth d <= L + D = window length <= bits of WORD.  Ops are stored for
     // depths 0..W-1 (2 bits each); child masks for depths 0..W (4 bits each,
     // the leaf at depth W included).
@@ -321,6 +386,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     WORD z = 0, tN = 0, tA = 0, tC = 0, tG = 0, tT = 0;   // window planes
     WORD pa = 0, pc = 0, pg = 0, pt = 0, fa = 0, fc = 0, fg = 0, ft = 0, pn = 0;  // pattern planes
     uint32_t L = 0, M = 0, D = 0, R = 0, E = 0, pid = 0, last_chunk = 0;
+    uint64_t hash_id = 0;
     // DFS state
     bool busy = false;
     uint32_t i = 0, jj = 0, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
@@ -330,6 +396,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < kTrsW; k++) trs[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0;
+    uint32_t cand_steps = 0, cand_tile = 0;
     bool drained = false;
 
     auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
@@ -431,7 +498,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             bool past_end = st - sink.chunks[c].nib_begin + (L + db - rb) > sink.chunks[c].size;
             if (!(end_chrom && past_end)) {
                 my_kept++;
-                my_sum += bulge_hit_hash(st, pid, mm, db, rb, d, o0, o1, 0);
+                my_sum += bulge_hit_hash(st, hash_id, mm, db, rb, d, o0, o1, 0);
             }
         }
     };
@@ -451,7 +518,9 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                 if (qi < n_queue) {
                     const BCand C = queue[qi];
                     const BPat& P = pats[C.pat];
-                    L = P.L; M = P.M; D = P.D; R = P.R; E = P.E; pid = P.pid;
+                    cand_tile = C.tile;
+                    cand_steps = 0;
+                    L = P.L; M = P.M; D = P.D; R = P.R; E = P.E; pid = P.pid; hash_id = P.hash_id;
                     pa = (WORD)P.pa; pc = (WORD)P.pc; pg = (WORD)P.pg; pt = (WORD)P.pt;
                     fa = (WORD)P.fa; fc = (WORD)P.fc; fg = (WORD)P.fg; ft = (WORD)P.ft;
                     pn = (WORD)P.is_n;
@@ -494,6 +563,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         my_slots++;
         if (!busy) continue;
         my_steps++;
+        cand_steps++;
         uint32_t viable = get4(d);
         if (viable) {
             // push the next admissible child and expand it
@@ -521,6 +591,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             set4(d, children());
         } else if (d == 0) {
             busy = false;
+            if (sink.tile_steps) atomicAdd(&sink.tile_steps[cand_tile], (unsigned long long)cand_steps);
         } else {
             d--;
             uint32_t op = get2(d);
@@ -586,6 +657,7 @@ BulgePlan* bulge_plan_create(const Query& q) {
         G.pam_len = (uint32_t)pf.size();
         for (size_t k = 0; k < pf.size(); k++) G.pam_filter[k] = pf[k];
         G.pat_begin = (uint32_t)bp->pats.size();
+        if (kv.second.size() >= (1u << 20)) throw Error(COF_E_UNSUPPORTED, "more than 2^20 patterns share one filter");
         G.pat_count = (uint32_t)kv.second.size();
         for (uint32_t pi : kv.second) {
             const Pattern& p = q.pats[pi];
@@ -602,6 +674,7 @@ BulgePlan* bulge_plan_create(const Query& q) {
                 if (is_n_char(p.ascii[i])) P.is_n |= 1ull << i;
             }
             P.L = p.L; P.M = p.M; P.D = p.D; P.R = p.R; P.E = p.M + p.D + p.R; P.pid = pi;
+            P.hash_id = pattern_id_of(p);
             bp->pats.push_back(P);
         }
         (G.W <= 32 ? bp->groups32 : bp->groups64).push_back(G);
@@ -625,18 +698,80 @@ void bulge_plan_destroy(BulgePlan* bp) {
     delete bp;
 }
 
+size_t bulge_plan_size(const BulgePlan* bp) { return bp ? bp->pats.size() : 0; }
+
 uint64_t bulge_pad_words() { return (uint64_t)kWinWords + 64; }
 
 
-// Runs the filter over every virtual batch's end range, then the traceback
-// over the queued candidates.  Tile ranges are processed in rounds so the
-// candidate queue stays bounded; a round whose queue overflows is split.
+namespace {
+
+void fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+struct RoundCtx {
+    BulgePlan* bp;
+    Workspace& ws;
+    const Genome& g;
+    const uint32_t* d_genome;
+    uint32_t n_tiles;
+    uint32_t verify_blocks;
+    Sink base_sink;
+};
+
+// filter of tiles order[t0, t1) into `queue`, count at *q_count
+void launch_filter(RoundCtx& c, const uint32_t* d_order, size_t t0, size_t t1, BCand* queue, uint32_t* q_count,
+                   uint32_t q_cap, hipStream_t st) {
+    uint32_t nt = (uint32_t)(t1 - t0);
+    const BTile* tiles = (const BTile*)c.ws.tiles;
+    uint32_t* tile_cand = c.ws.tile_cand;
+    if (!c.bp->groups32.empty())
+        hipLaunchKernelGGL(bulge_filter_kernel<uint32_t>, dim3(nt), dim3(kFilterThreads), 0, st, c.d_genome, tiles,
+                           c.bp->d_g32, (uint32_t)c.bp->groups32.size(), c.bp->d_pats, queue, q_count, q_cap,
+                           d_order + t0, tile_cand);
+    if (!c.bp->groups64.empty())
+        hipLaunchKernelGGL(bulge_filter_kernel<uint64_t>, dim3(nt), dim3(kFilterThreads), 0, st, c.d_genome, tiles,
+                           c.bp->d_g64, (uint32_t)c.bp->groups64.size(), c.bp->d_pats, queue, q_count, q_cap,
+                           d_order + t0, tile_cand);
+}
+
+void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uint32_t q_cap, uint32_t* overflow,
+                   uint32_t* q_next, uint32_t blocks, hipStream_t st) {
+    size_t lds32 = (size_t)c.bp->n_cols * 64 * sizeof(uint32_t) * 2;
+    size_t lds64 = (size_t)c.bp->n_cols * 64 * sizeof(uint64_t) * 2;
+    if (c.bp->groups64.empty())
+        hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, c.d_genome, queue, q_count,
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, c.base_sink);
+    else
+        hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, c.d_genome, queue, q_count,
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, c.base_sink);
+}
+
+}  // namespace
+
+// Runs the filter over every virtual batch's end range (tiles of 4 Ki end
+// positions, search.rs:1080-1092, 899-927), then the traceback over the
+// queued candidates.
+//
+// Schedule: a first round of tiles runs synchronously and measures the
+// candidate density; the remaining tiles are cut into rounds expected to fill
+// half a queue and run as a pipeline — filter of round r+1 on one stream while
+// round r verifies on another, two queues alternating — captured into one
+// hipGraph unless disabled.  The verify kernel reads its count from device
+// memory and skips a round whose queue overflowed; such rounds are redone
+// synchronously in halves (the reference bisects the same way,
+// search.rs:1140-1152).  `order` (optional) is the tile processing order.
 void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats) {
+               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats,
+               const BulgeTune& tune) {
+    const std::vector<uint32_t>* order = tune.order;
+    const uint32_t flags = tune.flags;
     out.clear();
+    ws.tile_j0.clear();
+    ws.tile_n.clear();
     if (!bp || bp->pats.empty()) return;
     ws.ensure_events();
-    // tiles of end positions per virtual batch (search.rs:1080-1092, 899-927)
     std::vector<BTile> tiles;
     for (const Batch& b : batches) {
         uint64_t floor = g.chunks[b.first].nib_begin();
@@ -646,17 +781,24 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             tiles.push_back(BTile{j0, floor, e, (uint32_t)std::min<uint64_t>(kTileEnds, e - j0), (uint32_t)b.last});
     }
     if (tiles.empty()) return;
-    ws_grow(&ws.tiles, &ws.tiles_cap, tiles.size(), sizeof(BTile));
-    BTile* d_tiles = (BTile*)ws.tiles;
-    HIPCHECK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BTile), hipMemcpyHostToDevice, st));
-    uint32_t* q_count = ws.counters;
-    uint32_t* o_count = ws.counters + 1;
-    uint32_t* q_next = ws.counters + 2;
-    unsigned long long* kept = ws.sums;
-    unsigned long long* checksum = ws.sums + 1;
-    unsigned long long* diag = ws.sums + 2;
+    const uint32_t n_tiles = (uint32_t)tiles.size();
+    std::vector<uint32_t> ord(n_tiles);
+    if (order && order->size() == n_tiles) ord = *order;
+    else for (uint32_t t = 0; t < n_tiles; t++) ord[t] = t;
+    ws.last_tiles = n_tiles;
+    ws.tile_j0.resize(n_tiles);
+    ws.tile_n.resize(n_tiles);
+    for (uint32_t t = 0; t < n_tiles; t++) { ws.tile_j0[t] = tiles[t].j0; ws.tile_n[t] = tiles[t].n; }
+    ws_grow(&ws.tiles, &ws.tiles_cap, n_tiles, sizeof(BTile));
+    ws_grow((void**)&ws.tile_order, &ws.tile_order_cap, n_tiles, sizeof(uint32_t));
+    ws_grow((void**)&ws.tile_cand, &ws.tile_cand_cap, n_tiles, sizeof(uint32_t));
+    ws_grow((void**)&ws.tile_steps, &ws.tile_steps_cap, n_tiles, sizeof(unsigned long long));
+    HIPCHECK(hipMemcpyAsync(ws.tiles, tiles.data(), n_tiles * sizeof(BTile), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ws.tile_order, ord.data(), n_tiles * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemsetAsync(ws.tile_cand, 0, n_tiles * sizeof(uint32_t), st));
+    HIPCHECK(hipMemsetAsync(ws.tile_steps, 0, n_tiles * sizeof(unsigned long long), st));
     HIPCHECK(hipMemsetAsync(ws.sums, 0, 4 * sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(ws.counters + 3, 0, sizeof(uint32_t), st));
+    HIPCHECK(hipMemsetAsync(ws.counters, 0, 4 * sizeof(uint32_t), st));
     if (bp->count_only) {
         std::vector<DChunk> dc(g.chunks.size());
         for (size_t c = 0; c < g.chunks.size(); c++)
@@ -664,121 +806,179 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         ws_grow(&ws.chunks, &ws.chunks_cap, dc.size(), sizeof(DChunk));
         HIPCHECK(hipMemcpyAsync(ws.chunks, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice, st));
     }
-    if (!ws.queue) {
-        size_t cap = 0;
-        ws_grow(&ws.queue, &cap, 1u << 22, sizeof(BCand));
-        ws.q_cap = (uint32_t)cap;
-    }
+    const uint32_t cap_max = std::max<uint32_t>(64, tune.queue_cap);
+    auto ensure_queue = [&](int k, uint32_t cap) {
+        size_t c = ws.qs_cap[k];
+        ws_grow(&ws.qs[k], &c, cap, sizeof(BCand));
+        ws.qs_cap[k] = c;
+    };
     if (!ws.bulge_out && !bp->count_only) {
         size_t cap = 0;
         ws_grow(&ws.bulge_out, &cap, 1u << 20, sizeof(BulgeRaw));
         ws.bulge_cap = (uint32_t)cap;
     }
-    const uint32_t verify_blocks = (uint32_t)std::max(1, ws.cus) * 12;
-    // rounds of tiles sized from the candidate density seen so far, so the
-    // queue rarely overflows; an overflowing round is split in halves
-    size_t next_tile = 0, round_tiles = 1u << 12;
-    std::vector<std::pair<size_t, size_t>> rounds;
-    while (next_tile < tiles.size() || !rounds.empty()) {
-        if (rounds.empty()) {
-            rounds.push_back({next_tile, std::min(tiles.size(), next_tile + round_tiles)});
-            next_tile = rounds.back().second;
-        }
-        auto [t0, t1] = rounds.back();
-        rounds.pop_back();
-        BCand* d_queue = (BCand*)ws.queue;
-        HIPCHECK(hipMemsetAsync(q_count, 0, sizeof(uint32_t), st));
+    if (!ws.st2) HIPCHECK(hipStreamCreate(&ws.st2));
+    if (!ws.st3) HIPCHECK(hipStreamCreate(&ws.st3));
+    RoundCtx rc{bp, ws, g, d_genome, n_tiles, (uint32_t)std::max(1, ws.cus) * std::max(1u, tune.verify_per_cu),
+                Sink{}};
+    rc.base_sink = Sink{(BulgeRaw*)ws.bulge_out, ws.counters + 1, ws.bulge_cap, bp->count_only ? 1 : 0,
+                        (const DChunk*)ws.chunks, (uint32_t)g.chunks.size(), ws.sums, ws.sums + 1, ws.sums + 2,
+                        ws.counters + 3, (flags & COF_FLAG_TILE_STATS) ? ws.tile_steps : nullptr};
+    const uint32_t* d_order = ws.tile_order;
+    auto t_start = std::chrono::steady_clock::now();
+
+    uint64_t cand_sync = 0;
+    // synchronous round over order[t0, t1) with queue 0 (capacity q_cap):
+    // filter, count, traceback, timed with events.  Returns the count; a
+    // count above q_cap means the traceback was not run.
+    auto sync_round = [&](size_t t0, size_t t1, uint32_t q_cap) -> uint32_t {
+        uint32_t* cnt = ws.counters + 2;       // queue count of this round
+        HIPCHECK(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st));
+        HIPCHECK(hipMemsetAsync(ws.round_ctr, 0, 2 * sizeof(uint32_t), st));   // cursor, overflow
+        hipLaunchKernelGGL(clear_tile_cand_kernel, dim3((uint32_t)((t1 - t0 + 255) / 256)), dim3(256), 0, st,
+                           d_order + t0, (uint32_t)(t1 - t0), ws.tile_cand);
         HIPCHECK(hipEventRecord(ws.ev[0], st));
-        uint32_t nt = (uint32_t)(t1 - t0);
-        if (!bp->groups32.empty())
-            hipLaunchKernelGGL(bulge_filter_kernel<uint32_t>, dim3(nt), dim3(kFilterThreads), 0, st, d_genome,
-                               d_tiles + t0, bp->d_g32, (uint32_t)bp->groups32.size(), bp->d_pats, d_queue, q_count,
-                               ws.q_cap);
-        if (!bp->groups64.empty())
-            hipLaunchKernelGGL(bulge_filter_kernel<uint64_t>, dim3(nt), dim3(kFilterThreads), 0, st, d_genome,
-                               d_tiles + t0, bp->d_g64, (uint32_t)bp->groups64.size(), bp->d_pats, d_queue, q_count,
-                               ws.q_cap);
+        launch_filter(rc, d_order, t0, t1, (BCand*)ws.qs[0], cnt, q_cap, st);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(ws.ev[1], st));
         uint32_t nq = 0;
-        HIPCHECK(hipMemcpyAsync(&nq, q_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&nq, cnt, sizeof nq, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
         if (stats) stats->kernel_ms_filter += ms;
-        if (nq > ws.q_cap) {
-            if (ws.q_cap < (1u << 27)) {
-                // grow the queue (up to 128 Mi candidates) and redo the round
-                size_t cap = ws.q_cap;
-                ws_grow(&ws.queue, &cap, std::min<size_t>(1u << 27, (size_t)nq + nq / 4), sizeof(BCand));
-                ws.q_cap = (uint32_t)cap;
-                rounds.push_back({t0, t1});
-                continue;
-            }
-            if (t1 - t0 > 1) {           // split the round (search.rs:1140-1152)
-                size_t mid = t0 + (t1 - t0) / 2;
-                rounds.push_back({mid, t1});
-                rounds.push_back({t0, mid});
-                continue;
-            }
-            throw Error(COF_E_NOMEM, "one filter tile produced more than 2^27 candidates");
-        }
+        if (nq > q_cap) return nq;
+        cand_sync += nq;
         if (stats) stats->n_candidates += nq;
-        {
-            double per_tile = (double)nq / (double)(t1 - t0);
-            double fit = 0.5 * (double)(1u << 27) / std::max(per_tile, 1.0);
-            round_tiles = (size_t)std::max(1.0, std::min(fit, (double)(1u << 16)));
-        }
-        if (nq == 0) continue;
-        // verify; rerun with a larger record buffer when it overflows
-        for (;;) {
-            HIPCHECK(hipMemsetAsync(ws.counters + 1, 0, 2 * sizeof(uint32_t), st));
+        if (nq) {
             HIPCHECK(hipEventRecord(ws.ev[2], st));
-            Sink sk{(BulgeRaw*)ws.bulge_out, o_count, ws.bulge_cap, bp->count_only ? 1 : 0, (const DChunk*)ws.chunks,
-                    (uint32_t)g.chunks.size(), kept, checksum, diag, ws.counters + 3};
-            // persistent verify: lanes refill from the queue until it drains
-            uint32_t blocks = std::min<uint32_t>((nq + 63) / 64, verify_blocks);
-            size_t lds32 = (size_t)bp->n_cols * 64 * sizeof(uint32_t) * 2;
-            size_t lds64 = (size_t)bp->n_cols * 64 * sizeof(uint64_t) * 2;
-            if (bp->groups64.empty())
-                hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, d_genome,
-                                   d_queue, nq, q_next, bp->d_pats, bp->n_cols, sk);
-            else
-                hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, d_genome,
-                                   d_queue, nq, q_next, bp->d_pats, bp->n_cols, sk);
+            launch_verify(rc, (const BCand*)ws.qs[0], cnt, q_cap, ws.round_ctr + 1, ws.round_ctr,
+                          std::min<uint32_t>((nq + 63) / 64, rc.verify_blocks), st);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipEventRecord(ws.ev[3], st));
-            uint32_t no = 0;
-            HIPCHECK(hipMemcpyAsync(&no, o_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
+            HIPCHECK(hipEventSynchronize(ws.ev[3]));
             HIPCHECK(hipEventElapsedTime(&ms, ws.ev[2], ws.ev[3]));
             if (stats) stats->kernel_ms_verify += ms;
-            if (!bp->count_only && no > ws.bulge_cap) {
-                size_t cap = ws.bulge_cap;
-                ws_grow(&ws.bulge_out, &cap, (size_t)no + no / 4, sizeof(BulgeRaw));
-                ws.bulge_cap = (uint32_t)cap;
-                continue;
+        }
+        return nq;
+    };
+    // synchronous rounds, halving a range whose queue overflowed
+    auto run_sync = [&](size_t t0, size_t t1, uint32_t q_cap) {
+        std::vector<std::pair<size_t, size_t>> stack{{t0, t1}};
+        while (!stack.empty()) {
+            auto [a0, a1] = stack.back();
+            stack.pop_back();
+            uint32_t nq = sync_round(a0, a1, q_cap);
+            if (nq > q_cap) {
+                if (a1 - a0 == 1) throw Error(COF_E_NOMEM, "one filter tile produced more candidates than the queue holds");
+                if (stats) stats->n_redo_rounds++;
+                size_t mid = a0 + (a1 - a0) / 2;
+                stack.push_back({mid, a1});
+                stack.push_back({a0, mid});
             }
-            if (!bp->count_only && no) {
-                size_t old = out.size();
-                out.resize(old + no);
-                HIPCHECK(hipMemcpyAsync(out.data() + old, ws.bulge_out, (size_t)no * sizeof(BulgeRaw),
-                                        hipMemcpyDeviceToHost, st));
+        }
+    };
+
+    // 1. first round, synchronous: measures the candidate density
+    const size_t first = std::min<size_t>(n_tiles, std::max<uint32_t>(1, tune.first_tiles));
+    const uint32_t cap0 = std::min<uint32_t>(cap_max, 1u << 24);
+    ensure_queue(0, cap0);
+    run_sync(0, first, cap0);
+    if (first < n_tiles) {
+        // 2. the rest in rounds expected to fill half a queue, at least ~4
+        //    rounds (so filter and traceback overlap) unless rounds would fall
+        //    below 4 Mi candidates
+        const size_t remaining = n_tiles - first;
+        const double per_tile = std::max(1.0, (double)cand_sync / (double)first);
+        const double target = std::min(0.5 * cap_max, std::max(per_tile * remaining / 4.0, (double)(1u << 22)));
+        size_t round_tiles = tune.round_tiles ? tune.round_tiles
+                                              : (size_t)std::max(1.0, std::min(target / per_tile, (double)remaining));
+        uint32_t q_cap = cap_max;
+        if (!tune.round_tiles) {
+            uint64_t want = 1;
+            while (want < (uint64_t)(2.0 * target)) want <<= 1;
+            q_cap = (uint32_t)std::min<uint64_t>(cap_max, want);
+        }
+        ensure_queue(0, q_cap);
+        ensure_queue(1, q_cap);
+        std::vector<std::pair<size_t, size_t>> rounds;
+        for (size_t t0 = first; t0 < n_tiles; t0 += round_tiles)
+            rounds.push_back({t0, std::min<size_t>(n_tiles, t0 + round_tiles)});
+        const uint32_t R = (uint32_t)rounds.size();
+        if (flags & COF_FLAG_SYNC_ROUNDS) {
+            for (auto& r : rounds) run_sync(r.first, r.second, q_cap);
+        } else {
+            // 3. pipeline: filter of round r on `st`, its traceback on
+            //    st2/st3 alternately (consecutive tracebacks overlap, so one
+            //    round's tail does not idle the GPU); queue r%2 is refilled by
+            //    round r+2 only after round r's traceback ended
+            ws_grow((void**)&ws.rounds, &ws.rounds_cap, 3 * (size_t)R, sizeof(uint32_t));
+            ws.ensure_round_events(R);
+            const bool use_graph = !(flags & COF_FLAG_NO_GRAPH);
+            hipGraph_t graph = nullptr;
+            if (use_graph) HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+            fill_u32(ws.rounds, 3 * R, 0, st);
+            for (uint32_t r = 0; r < R; r++) {
+                uint32_t* rq = ws.rounds + 3 * r;          // count, cursor, overflow
+                const int buf = (int)(r & 1);
+                hipStream_t vs = buf ? ws.st3 : ws.st2;
+                if (r >= 2) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * (r - 2) + 1], 0));
+                launch_filter(rc, d_order, rounds[r].first, rounds[r].second, (BCand*)ws.qs[buf], rq, q_cap, st);
+                HIPCHECK(hipEventRecord(ws.rev[2 * r], st));
+                HIPCHECK(hipStreamWaitEvent(vs, ws.rev[2 * r], 0));
+                launch_verify(rc, (const BCand*)ws.qs[buf], rq, q_cap, rq + 2, rq + 1, rc.verify_blocks, vs);
+                HIPCHECK(hipEventRecord(ws.rev[2 * r + 1], vs));
+            }
+            for (uint32_t r = R >= 2 ? R - 2 : 0; r < R; r++) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * r + 1], 0));
+            if (use_graph) {
+                HIPCHECK(hipStreamEndCapture(st, &graph));
+                hipGraphExec_t exec = nullptr;
+                HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                HIPCHECK(hipGraphLaunch(exec, st));
                 HIPCHECK(hipStreamSynchronize(st));
+                HIPCHECK(hipGraphExecDestroy(exec));
+                HIPCHECK(hipGraphDestroy(graph));
             }
-            break;
+            HIPCHECK(hipStreamSynchronize(st));
+            HIPCHECK(hipGetLastError());
+            std::vector<uint32_t> rs(3 * (size_t)R);
+            HIPCHECK(hipMemcpy(rs.data(), ws.rounds, rs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (uint32_t r = 0; r < R; r++) {
+                if (rs[3 * r] > q_cap) {
+                    // the traceback skipped this round; redo it (in halves if needed)
+                    if (stats) stats->n_redo_rounds++;
+                    run_sync(rounds[r].first, rounds[r].second, q_cap);
+                } else if (stats) {
+                    stats->n_candidates += rs[3 * r];
+                }
+            }
+            HIPCHECK(hipStreamSynchronize(st));
+            if (stats) stats->n_pipeline_rounds += R;
         }
     }
-    {
-        uint32_t err = 0;
-        HIPCHECK(hipMemcpyAsync(&err, ws.counters + 3, sizeof err, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        if (err) throw Error(COF_E_HIP, "bulge traceback reached an impossible state");
+    if (stats) {
+        stats->bulge_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    uint32_t err = 0, no = 0;
+    HIPCHECK(hipMemcpy(&err, ws.counters + 3, sizeof err, hipMemcpyDeviceToHost));
+    if (err) throw Error(COF_E_HIP, "bulge traceback reached an impossible state");
+    if (!bp->count_only) {
+        HIPCHECK(hipMemcpy(&no, ws.counters + 1, sizeof no, hipMemcpyDeviceToHost));
+        if (no > ws.bulge_cap) {
+            // record buffer too small: grow it and search again
+            size_t cap = ws.bulge_cap;
+            ws_grow(&ws.bulge_out, &cap, (size_t)no + no / 4, sizeof(BulgeRaw));
+            ws.bulge_cap = (uint32_t)cap;
+            if (stats) stats->n_candidates = 0;
+            bulge_run(bp, ws, g, batches, d_genome, st, out, stats, tune);
+            return;
+        }
+        out.resize(no);
+        if (no) HIPCHECK(hipMemcpy(out.data(), ws.bulge_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost));
     }
     if (stats) {
         unsigned long long sums[4] = {0, 0, 0, 0};
-        HIPCHECK(hipMemcpyAsync(sums, ws.sums, sizeof sums, hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipMemcpy(sums, ws.sums, sizeof sums, hipMemcpyDeviceToHost));
         stats->verify_steps = sums[2];
         stats->verify_slots = sums[3];
         if (bp->count_only) {

@@ -127,11 +127,20 @@ COF_HD inline uint64_t cof_mix64(uint64_t x) {
     x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
     return x;
 }
-COF_HD inline uint64_t bulge_hit_hash(uint64_t start, uint32_t pattern, uint32_t mm, uint32_t db, uint32_t rb,
+// pattern_id: a hash of the pattern's sequence and strand (pattern_id_of), so
+// the checksum does not depend on how a query numbers its patterns
+COF_HD inline uint64_t bulge_hit_hash(uint64_t start, uint64_t pattern_id, uint32_t mm, uint32_t db, uint32_t rb,
                                       uint32_t n_ops, uint64_t ops0, uint64_t ops1, uint64_t ops2) {
-    uint64_t h = cof_mix64(start * 0x9E3779B97F4A7C15ull ^ ((uint64_t)pattern << 40) ^ ((uint64_t)mm << 24) ^
+    uint64_t h = cof_mix64(start * 0x9E3779B97F4A7C15ull ^ pattern_id ^ ((uint64_t)mm << 24) ^
                            ((uint64_t)db << 16) ^ ((uint64_t)rb << 8) ^ n_ops);
     return cof_mix64(h ^ ops0) + cof_mix64(h ^ ops1 ^ 0x1234567ull) + cof_mix64(h ^ ops2 ^ 0x89ABCDEFull);
+}
+
+inline uint64_t pattern_id_of(const Pattern& p) {
+    uint64_t h = 0xcbf29ce484222325ull;                 // FNV-1a over the bit4 codes
+    for (uint8_t v : p.b4) { h ^= v; h *= 0x100000001b3ull; }
+    return cof_mix64(h ^ (p.forward ? 0x5bd1e995ull : 0x27d4eb2full) ^ ((uint64_t)p.M << 48) ^
+                     ((uint64_t)p.D << 52) ^ ((uint64_t)p.R << 56));
 }
 
 // Host-side conversion (chunk lookup, batch-seam and chromosome-end drop

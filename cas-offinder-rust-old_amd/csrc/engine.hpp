@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "cof_internal.hpp"
 
 namespace cof {
@@ -25,11 +27,45 @@ struct Workspace {
     unsigned long long* sums = nullptr;  // [0] kept, [1] checksum, [2..3] diagnostics
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus = 0;
+    // bulge pipeline
+    void* qs[2] = {nullptr, nullptr};          // double-buffered candidate queues
+    size_t qs_cap[2] = {0, 0};
+    uint32_t* tile_order = nullptr;            // tile processing order
+    size_t tile_order_cap = 0;
+    uint32_t* tile_cand = nullptr;             // candidates per tile (last search)
+    size_t tile_cand_cap = 0;
+    unsigned long long* tile_steps = nullptr;  // DFS transitions per tile (last search)
+    size_t tile_steps_cap = 0;
+    uint32_t last_tiles = 0;
+    std::vector<uint64_t> tile_j0;             // host copy of the last search's tiles
+    std::vector<uint32_t> tile_n;
+    uint32_t* rounds = nullptr;                // per round: queue count, cursor, overflow
+    size_t rounds_cap = 0;
+    uint32_t* round_ctr = nullptr;             // synchronous round: cursor, overflow
+    std::vector<hipEvent_t> rev;               // per round: filtered, verified
+    hipStream_t st2 = nullptr, st3 = nullptr;  // traceback streams of the pipeline
     ~Workspace();
     void ensure_events();
+    void ensure_round_events(uint32_t rounds);
 };
 // grows *ptr to hold `need` elements of `elem` bytes (contents not kept)
 void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem);
+
+// Launch shape of the plain scan (results do not depend on it).
+struct PlainTune {
+    uint32_t threads = 256;     // workgroup size; a workgroup scans 64 starts per thread
+    uint32_t pat_chunk = 0;     // patterns per workgroup (grid.y slices a group); 0 = whole group
+};
+
+// Schedule of the bulge path (results do not depend on it).
+struct BulgeTune {
+    uint32_t first_tiles = 1024;    // filter tiles in the synchronous first round
+    uint32_t round_tiles = 0;       // tiles per pipeline round; 0 = sized from the first round
+    uint32_t verify_per_cu = 12;    // persistent traceback workgroups per CU
+    uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
+    uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
+    const std::vector<uint32_t>* order = nullptr;   // tile processing order (identity if null)
+};
 
 struct PlainPlan;
 PlainPlan* plain_plan_create(const Query& q);
@@ -38,7 +74,7 @@ bool plain_plan_empty(const PlainPlan* pp);
 uint64_t plain_tile_nucl();
 uint64_t plain_pad_words();
 void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
-               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches);
+               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
 
 struct BulgePlan;
 BulgePlan* bulge_plan_create(const Query& q);
@@ -46,7 +82,9 @@ BulgePlan* bulge_plan_create(const Query& q);
 // into stats->bulge_hits / bulge_checksum instead of returning records
 void bulge_plan_destroy(BulgePlan* bp);
 uint64_t bulge_pad_words();
+size_t bulge_plan_size(const BulgePlan* bp);   // patterns of the plan
 void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats);
+               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats,
+               const BulgeTune& tune);
 
 }  // namespace cof

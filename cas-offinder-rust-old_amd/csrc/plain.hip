@@ -9,14 +9,14 @@
 // filtering on the host as the reference does, the device evaluates the whole
 // predicate, so only final hits leave the GPU.
 //
-// Design (one workgroup = one tile of T consecutive start positions):
+// Design (one workgroup = one tile of 64 x THREADS consecutive start
+// positions, optionally one slice of a group's patterns per grid row):
 //   1. the tile's genome words (+ halo) are loaded coalesced into LDS with the
 //      N-mask (0xF -> 0, search.rs:512-531) applied on the fly; a SWAR test
 //      flags tiles holding IUPAC genome letters (R, Y, ...);
-//   2. per pattern group (patterns sharing one effective filter), every lane
-//      tests the PAM of one start position with a 16-bit nibble lookup per
-//      filter position, and passing starts are compacted into an LDS list with
-//      a wave64 ballot + mbcnt prefix;
+//   2. per pattern group (patterns sharing one effective filter), every thread
+//      tests the PAM of its 64 starts, 8 per word (SWAR over nibbles), and the
+//      passing starts are compacted into an LDS list by a block-wide scan;
 //   3. each lane takes one compacted start, extracts its compare window with
 //      v_alignbit funnel shifts, and loops over the group's patterns (uniform
 //      scalar loads) doing AND + v_bcnt per 32-bit word.
@@ -36,11 +36,10 @@ namespace cof {
 
 namespace {
 
-constexpr int kTile = 16384;                 // start positions per workgroup
-constexpr int kThreads = 256;
+constexpr int kMaxThreads = 512;             // largest workgroup (tile = 64 starts per thread)
 constexpr int kMaxL = 64;                    // plain pattern length supported
 constexpr int kHaloWords = (kMaxL + 8) / 8 + 2;
-constexpr int kTileWords = kTile / 8 + kHaloWords;
+constexpr int kMaxTileWords = kMaxThreads * 64 / 8 + kHaloWords;
 constexpr int kMaxPam = 64;
 
 struct GroupDev {
@@ -95,13 +94,17 @@ __device__ __forceinline__ uint32_t recount(const uint32_t* gw, uint32_t i, cons
     return amm;
 }
 
-template <int W>
-__global__ __launch_bounds__(kThreads) void plain_scan_kernel(
+template <int W, int THREADS>
+__global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
     const GroupDev* __restrict__ groups, uint32_t n_groups,
     const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
-    PlainRaw* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap) {
+    PlainRaw* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
+    uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
+    constexpr int kTile = THREADS * 64;
+    constexpr int kTileWords = kTile / 8 + kHaloWords;
+    constexpr int kThreads = THREADS;
     __shared__ uint32_t gw[kTileWords];
     __shared__ uint16_t cand[kTile];
     __shared__ uint32_t wave_sum[kThreads / 64];
@@ -128,6 +131,13 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const GroupDev& G = groups[gi];
+        uint32_t pb = G.pat_begin, pe = G.pat_begin + G.pat_count;
+        if (pat_chunk) {
+            const uint32_t lo = blockIdx.y * pat_chunk;   // uniform over the workgroup
+            if (lo >= G.pat_count) continue;
+            pb += lo;
+            pe = min(pe, pb + pat_chunk);
+        }
         // 2. PAM test, 8 positions per word: nibble j of X_k is the genome at
         //    start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble is
         //    in the filter's plain-base set iff it overlaps the filter code.
@@ -189,8 +199,7 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
                 uint32_t win[W];
 #pragma unroll
                 for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                const uint32_t pe = G.pat_begin + G.pat_count;
-                uint32_t p = G.pat_begin;
+                uint32_t p = pb;
                 // four patterns per iteration: their records are fetched with
                 // one batch of scalar loads, and one branch covers the four hits
                 for (; p + 4 <= pe; p += 4) {
@@ -225,7 +234,7 @@ __global__ __launch_bounds__(kThreads) void plain_scan_kernel(
                     }
                 }
             } else {
-                for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
+                for (uint32_t p = pb; p < pe; p++) {
                     const uint32_t* fw = full_words + (size_t)p * 8;
                     int32_t pop = 0;
                     uint32_t amm = 0;
@@ -255,6 +264,7 @@ struct PlainPlan {
     struct Launch {
         int W;
         uint32_t group_begin, n_groups;
+        uint32_t max_pats;         // largest group of the launch
     };
     std::vector<GroupDev> groups;
     std::vector<Launch> launches;
@@ -331,7 +341,9 @@ PlainPlan* plain_plan_create(const Query& q) {
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
-        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size()};
+        uint32_t mp = 0;
+        for (const GroupDev& G : kv.second) mp = std::max(mp, G.pat_count);
+        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp};
         pp->groups.insert(pp->groups.end(), kv.second.begin(), kv.second.end());
         pp->launches.push_back(l);
     }
@@ -358,15 +370,53 @@ void plain_plan_destroy(PlainPlan* pp) {
 
 bool plain_plan_empty(const PlainPlan* pp) { return !pp || pp->launches.empty(); }
 
-uint64_t plain_tile_nucl() { return kTile; }
-uint64_t plain_pad_words() { return kTileWords + 16; }
+uint64_t plain_tile_nucl() { return (uint64_t)kMaxThreads * 64; }
+uint64_t plain_pad_words() { return kMaxTileWords + 16; }
+
+namespace {
+
+template <int W>
+void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
+                  uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const uint32_t* full,
+                  PlainRaw* out, uint32_t* count, uint32_t cap) {
+    switch (t.threads) {
+        case 128:
+            hipLaunchKernelGGL((plain_scan_kernel<W, 128>), grid, dim3(128), 0, st, d_genome, scan_end, first, g,
+                               n_groups, pats, full, out, count, cap, t.pat_chunk);
+            break;
+        case 256:
+            hipLaunchKernelGGL((plain_scan_kernel<W, 256>), grid, dim3(256), 0, st, d_genome, scan_end, first, g,
+                               n_groups, pats, full, out, count, cap, t.pat_chunk);
+            break;
+        case 512:
+            hipLaunchKernelGGL((plain_scan_kernel<W, 512>), grid, dim3(512), 0, st, d_genome, scan_end, first, g,
+                               n_groups, pats, full, out, count, cap, t.pat_chunk);
+            break;
+        default: throw Error(COF_E_INVALID, "plain workgroup size must be 128, 256 or 512");
+    }
+}
+
+}  // namespace
 
 // Scans start positions [0, scan_end) of the resident genome.  Device words
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
 Workspace::~Workspace() {
-    for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums})
+    for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
+                    qs[0], qs[1], (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
+                    (void*)round_ctr})
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : rev) if (e) (void)hipEventDestroy(e);
+    if (st2) (void)hipStreamDestroy(st2);
+    if (st3) (void)hipStreamDestroy(st3);
+}
+
+void Workspace::ensure_round_events(uint32_t r) {
+    while (rev.size() < 2 * (size_t)r) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        rev.push_back(e);
+    }
 }
 
 void Workspace::ensure_events() {
@@ -375,6 +425,7 @@ void Workspace::ensure_events() {
     HIPCHECK(hipMalloc(&counters, 4 * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&sums, 4 * sizeof(unsigned long long)));
     HIPCHECK(hipMalloc(&plain_count, sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&round_ctr, 2 * sizeof(uint32_t)));
     int dev = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -389,7 +440,7 @@ void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem) {
 }
 
 void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
-               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches) {
+               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
     out.clear();
     if (plain_plan_empty(pp) || scan_end == 0) return;
     ws.ensure_events();
@@ -398,6 +449,7 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
         ws_grow(&ws.plain_out, &cap, 1u << 20, sizeof(PlainRaw));
         ws.plain_cap = (uint32_t)cap;
     }
+    const uint64_t kTile = (uint64_t)tune.threads * 64;
     uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
     for (;;) {
         PlainRaw* d_out = (PlainRaw*)ws.plain_out;
@@ -409,11 +461,12 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
                 uint32_t nb = (uint32_t)std::min<uint64_t>(n_tiles - t0, 1u << 20);
                 uint64_t first = t0 * kTile;
                 const GroupDev* g = pp->d_groups + l.group_begin;
-#define COF_PLAIN_LAUNCH(WW)                                                                              \
-    case WW:                                                                                              \
-        hipLaunchKernelGGL(plain_scan_kernel<WW>, dim3(nb), dim3(kThreads), 0, st, d_genome, scan_end, first, g, \
-                           l.n_groups, pp->d_pats, pp->d_full, d_out,                                     \
-                           ws.plain_count, ws.plain_cap);                                                \
+                uint32_t ny = tune.pat_chunk ? (l.max_pats + tune.pat_chunk - 1) / tune.pat_chunk : 1u;
+                dim3 grid(nb, std::max(1u, ny));
+#define COF_PLAIN_LAUNCH(WW)                                                                                   \
+    case WW:                                                                                                   \
+        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_full, d_out, \
+                         ws.plain_count, ws.plain_cap);                                                       \
         break;
                 switch (l.W) {
                     COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)

@@ -70,6 +70,18 @@ typedef struct {
  * reports plain hits only in `out`; cof_stats carries the bulge count and
  * checksum, which equal those of a record-mode search. */
 #define COF_FLAG_COUNT_ONLY 1u
+/* Bulge path schedule (results are identical in every mode):
+ * default — after a first synchronous round that measures candidate density,
+ *   tile rounds run as a two-stream filter/traceback pipeline with
+ *   double-buffered candidate queues, captured into one hipGraph;
+ * COF_FLAG_NO_GRAPH — the same pipeline launched eagerly;
+ * COF_FLAG_SYNC_ROUNDS — one stream, filter then traceback per round with a
+ *   host synchronisation in between (the reference's batch loop shape). */
+#define COF_FLAG_NO_GRAPH 2u
+#define COF_FLAG_SYNC_ROUNDS 4u
+/* Also account traceback DFS transitions per filter tile (cof_tile_stat.dfs_steps;
+ * one atomic per candidate, so it costs time). */
+#define COF_FLAG_TILE_STATS 8u
 
 /* One chunk of the tight-packed genome (chrom_chunk.rs:1-12, search.rs:57-79). */
 typedef struct {
@@ -97,8 +109,9 @@ typedef struct {
 typedef struct {
     double search_ms;          /* wall time of the last search: device work, D2H of hits, conversion */
     double kernel_ms_plain;    /* summed HIP-event time of plain-scan launches */
-    double kernel_ms_filter;   /* summed HIP-event time of PAM+Myers filter launches */
-    double kernel_ms_verify;   /* summed HIP-event time of DP traceback launches */
+    double kernel_ms_filter;   /* summed HIP-event time of PAM+Myers filter launches of the
+                                  bulge path's synchronous rounds (all rounds with COF_FLAG_SYNC_ROUNDS) */
+    double kernel_ms_verify;   /* the same for DP traceback launches */
     uint64_t n_plain_launches;
     uint64_t n_candidates;     /* Myers survivors sent to the DP traceback */
     uint64_t n_raw_hits;       /* hits before the host-side seam/drop rules */
@@ -109,7 +122,36 @@ typedef struct {
                                   (start nibble, pattern, mm, db, rb, ops) */
     uint64_t verify_steps;     /* diagnostics: DFS transitions of the traceback kernel */
     uint64_t verify_slots;     /* diagnostics: lane slots its DFS loop ran (steps / slots = lane use) */
+    uint64_t n_pipeline_rounds;/* bulge rounds run by the pipeline */
+    double bulge_ms;           /* wall time of the bulge path (filter + traceback, all rounds) */
+    uint64_t n_redo_rounds;    /* bulge rounds whose queue overflowed and were redone in halves */
 } cof_stats;
+
+/* Launch shapes and schedule.  No reference counterpart: the reference fixes
+ * them at kernel-compile time (`-D` constants, search.rs:322-343,1193-1281).
+ * Results never depend on them; 0 selects the default of each field. */
+typedef struct {
+    uint32_t plain_threads;        /* plain-scan workgroup: 128, 256 (default) or 512 threads,
+                                      64 start positions per thread */
+    uint32_t plain_pat_chunk;      /* patterns per plain-scan workgroup; a larger group is split
+                                      over grid rows (0: whole group) */
+    uint32_t bulge_first_tiles;    /* filter tiles (4096 end positions each) in the synchronous
+                                      first round of the bulge path (default 1024) */
+    uint32_t bulge_round_tiles;    /* tiles per later round (0: sized from the first round's
+                                      candidate density: about 4 rounds, each within half a queue) */
+    uint32_t verify_blocks_per_cu; /* persistent traceback workgroups per CU (default 12) */
+    uint32_t bulge_queue_cap;      /* candidate queue capacity per round (default 2^27 = 4 GiB of
+                                      32-byte records, min 64); a round that overflows it is redone
+                                      in halves */
+} cof_tuning;
+
+/* Per filter tile of the last bulge search (tiles in genome order). */
+typedef struct {
+    uint64_t first_end;    /* first alignment end (nibble index in the resident genome) */
+    uint32_t n_ends;       /* end positions in the tile (<= 4096) */
+    uint32_t candidates;   /* Myers survivors sent to the traceback */
+    uint64_t dfs_steps;    /* traceback DFS transitions spent on them (COF_FLAG_TILE_STATS, else 0) */
+} cof_tile_stat;
 
 /* Replaces search_chunk_cuda{,_myers}'s set-up (search.rs:322-343,1193-1281):
  * validates the query, builds per-pattern tables and uploads them. */
@@ -147,6 +189,20 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
 cof_status cof_write_tsv(cof_ctx* ctx, const char* path);
 
 cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out);
+
+/* Sets the launch shapes (validated; COF_E_INVALID names the bad field).
+ * Build-only, no reference counterpart. */
+cof_status cof_set_tuning(cof_ctx* ctx, const cof_tuning* t);
+/* Per-tile work of the last bulge search: writes min(cap, tiles) records,
+ * *n = number of tiles.  Build-only, no reference counterpart. */
+cof_status cof_get_tile_stats(const cof_ctx* ctx, cof_tile_stat* out, uint64_t cap, uint64_t* n);
+/* Effective launch shapes, defaults filled in. */
+cof_status cof_get_tuning(const cof_ctx* ctx, cof_tuning* out);
+/* Times candidate shapes on the first `sample_nucl` nucleotides of the
+ * resident genome (0: 64 Mnt) with the current query, keeps the fastest and
+ * returns it in `chosen` (may be NULL).  Needs a genome.  Build-only, no
+ * reference counterpart (the reference compiles one shape per run). */
+cof_status cof_autotune(cof_ctx* ctx, uint64_t sample_nucl, cof_tuning* chosen);
 void cof_close(cof_ctx* ctx);
 const char* cof_strerror(cof_status s);
 const char* cof_last_error(const cof_ctx* ctx);

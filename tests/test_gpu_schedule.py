@@ -1,0 +1,178 @@
+"""Schedules and launch shapes never change results.
+
+The bulge path's rounds run as a two-stream filter/traceback pipeline captured
+into a hipGraph (default), the same pipeline launched eagerly, or synchronous
+rounds; the plain scan takes several workgroup sizes and pattern slices; the
+autotuner picks among them.  Every variant must give the oracle's rows
+(reference semantics: search.rs:1054-1191 for the bulge batch loop,
+search.rs:322-366 for the plain launch).  Small rounds and a small queue force
+many rounds, device-side overflow detection and the redo-in-halves path.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import cofpy
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+FILT = "N" * 20 + "NRG"
+
+
+def _bulge_genome(tmp_path):
+    rng = random.Random(2024)
+    guides = ["".join(rng.choice("ACGT") for _ in range(20)) for _ in range(3)]
+    plants = [(g + "AGG", 0, 20, 3, 20) for g in guides]
+    sg = cofpy.SynthGenome(4242, [700_000, 500_000, 130_001], n_frac=0.02, gap=(50, 2000),
+                           repeats=(0.1, 6, 300, 0.1), plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    return fa, [g + "NNN" for g in guides]
+
+
+def _oracle(tmp_path, fa, pats, M, D, R):
+    in_text = f"{fa}\n{FILT} {D} {R}\n" + "".join(f"{p} {M}\n" for p in pats)
+    oracle.run(in_text, tmp_path / "ref.tsv", batch_nucl=1 << 30)
+    return cofpy.read_rows(tmp_path / "ref.tsv")
+
+
+def _rows(s, tmp_path):
+    s.search()
+    s.write_tsv(tmp_path / "gpu.tsv")
+    return cofpy.read_rows(tmp_path / "gpu.tsv")
+
+
+def _bulge_search(fa, pats, flags=0, count_only=False, **tune):
+    n = len(pats)
+    s = cofpy.Search(pats, [FILT] * n, [3] * n, [1] * n, [1] * n, batch_nucl=1 << 30, count_only=count_only,
+                     flags=flags)
+    if tune:
+        s.set_tuning(**tune)
+    s.load_file(fa)
+    return s
+
+
+@pytest.mark.parametrize("flags", [0, cofpy.COF_FLAG_NO_GRAPH, cofpy.COF_FLAG_SYNC_ROUNDS])
+def test_bulge_schedules_match_oracle(tmp_path, flags):
+    fa, pats = _bulge_genome(tmp_path)
+    want = _oracle(tmp_path, fa, pats, 3, 1, 1)
+    assert len(want) > 0
+    s = _bulge_search(fa, pats, flags | cofpy.COF_FLAG_TILE_STATS, bulge_first_tiles=16, bulge_round_tiles=24)
+    got = _rows(s, tmp_path)
+    st = s.stats()
+    assert got == want
+    if flags & cofpy.COF_FLAG_SYNC_ROUNDS:
+        assert st.n_pipeline_rounds == 0
+    else:
+        assert st.n_pipeline_rounds >= 10
+    # per-tile accounting adds up to the totals
+    ts = s.tile_stats()
+    assert int(ts["candidates"].sum()) == st.n_candidates
+    assert int(ts["dfs_steps"].sum()) == st.verify_steps
+    assert np.all(np.diff(ts["first_end"].astype(np.int64)) > 0)
+    # without the flag, candidates are still counted but DFS steps are not
+    n = len(pats)
+    s.set_query(pats, [FILT] * n, [3] * n, [1] * n, [1] * n, batch_nucl=1 << 30, flags=flags)
+    assert _rows(s, tmp_path) == want
+    ts2 = s.tile_stats()
+    assert np.array_equal(ts2["candidates"], ts["candidates"])
+    assert int(ts2["dfs_steps"].sum()) == 0
+    s.close()
+
+
+def test_graph_replays_interleaved_with_eager(tmp_path):
+    # each search builds and launches a fresh graph; eager searches in between
+    # share the context's workspace, counters and queues
+    fa, pats = _bulge_genome(tmp_path)
+    want = _oracle(tmp_path, fa, pats, 3, 1, 1)
+    n = len(pats)
+    s = _bulge_search(fa, pats, 0, bulge_first_tiles=8, bulge_round_tiles=16)
+    sums = set()
+    for it in range(3):
+        for flags in (0, cofpy.COF_FLAG_SYNC_ROUNDS):
+            s.set_query(pats, [FILT] * n, [3] * n, [1] * n, [1] * n, batch_nucl=1 << 30, flags=flags)
+            assert _rows(s, tmp_path) == want, (it, flags)
+            st = s.stats()
+            sums.add((st.bulge_hits, st.bulge_checksum))
+            s.set_query(pats, [FILT] * n, [3] * n, [1] * n, [1] * n, batch_nucl=1 << 30, flags=flags,
+                        count_only=True)
+            s.search()
+            st = s.stats()
+            sums.add((st.bulge_hits, st.bulge_checksum))
+    assert len(sums) == 1
+    s.close()
+
+
+@pytest.mark.parametrize("flags", [0, cofpy.COF_FLAG_SYNC_ROUNDS])
+def test_queue_overflow_redo(tmp_path, flags):
+    fa, pats = _bulge_genome(tmp_path)
+    want = _oracle(tmp_path, fa, pats, 3, 1, 1)
+    s = _bulge_search(fa, pats, flags)
+    assert _rows(s, tmp_path) == want
+    cand = s.tile_stats()["candidates"].astype(np.int64)
+    # a queue that holds the densest tile but not a whole round of them
+    cap = max(64, int(cand.max()))
+    # the shortest round length (first round and pipeline rounds alike) at
+    # which some round overflows
+    rt = next(r for r in range(8, len(cand) + 1)
+              if max(cand[i:i + r].sum() for i in range(0, len(cand), r)) > cap)
+    s.set_tuning(bulge_first_tiles=rt, bulge_round_tiles=rt, bulge_queue_cap=cap)
+    assert _rows(s, tmp_path) == want
+    assert s.stats().n_redo_rounds > 0
+    s.close()
+
+
+def _plain_case(tmp_path, n_guides=40):
+    rng = random.Random(77)
+    guides = ["".join(rng.choice("ACGT") for _ in range(20)) for _ in range(n_guides)]
+    plants = [(g + "TGG", 0, 20, 4, 3) for g in guides]
+    sg = cofpy.SynthGenome(91, [400_000, 250_007], n_frac=0.02, gap=(30, 900), iupac_frac=0.0005,
+                           plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    pats = [g + "NNN" for g in guides]
+    in_text = f"{fa}\n{FILT}\n" + "".join(f"{p} 4\n" for p in pats)
+    oracle.run(in_text, tmp_path / "ref.tsv", batch_nucl=1 << 22)
+    return fa, pats, cofpy.read_rows(tmp_path / "ref.tsv")
+
+
+@pytest.mark.parametrize("threads,chunk", [(128, 0), (256, 16), (512, 7), (512, 0), (128, 64)])
+def test_plain_launch_shapes(tmp_path, threads, chunk):
+    fa, pats, want = _plain_case(tmp_path)
+    assert len(want) > 0
+    s = cofpy.Search(pats, [FILT] * len(pats), [4] * len(pats), batch_nucl=1 << 22)
+    s.set_tuning(plain_threads=threads, plain_pat_chunk=chunk)
+    s.load_file(fa)
+    assert s.tuning()["plain_threads"] == threads
+    assert _rows(s, tmp_path) == want
+    s.close()
+
+
+def test_autotune_keeps_results(tmp_path):
+    fa, pats, want = _plain_case(tmp_path)
+    n = len(pats)
+    s = cofpy.Search(pats, [FILT] * n, [4] * n, batch_nucl=1 << 22)
+    s.load_file(fa)
+    t = s.autotune(1 << 20)
+    assert t["plain_threads"] in (128, 256, 512)
+    assert s.tuning() == t
+    assert _rows(s, tmp_path) == want
+    s.close()
+    fa, bpats = _bulge_genome(tmp_path)
+    bwant = _oracle(tmp_path, fa, bpats, 3, 1, 1)
+    s = _bulge_search(fa, bpats)
+    t = s.autotune(1 << 20)
+    assert 1 <= t["verify_blocks_per_cu"] <= 64
+    assert _rows(s, tmp_path) == bwant
+    s.close()
+
+
+def test_invalid_tuning_rejected():
+    s = cofpy.Search(["A" * 20 + "NNN"], [FILT], [2])
+    for kw in ({"plain_threads": 100}, {"verify_blocks_per_cu": 65}, {"bulge_queue_cap": 10}):
+        with pytest.raises(cofpy.CofError) as e:
+            s.set_tuning(**kw)
+        assert e.value.status == cofpy.COF_E_INVALID
+    s.close()
