@@ -331,8 +331,31 @@ struct Sink {
     unsigned long long* checksum;
     unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop
     uint32_t* error;                // set when a lane reaches an impossible state
-    unsigned long long* tile_steps; // DFS transitions per filter tile (cost model targets)
+    uint32_t* cand_steps;           // DFS transitions per queue entry (null: not recorded)
 };
+
+// tile_steps[queue[i].tile] += steps[i] over the round's queue; a wave whose
+// 64 entries share one tile (the usual case: a filter workgroup writes one
+// tile's survivors contiguously) adds them with one atomic.
+__global__ void tile_steps_reduce_kernel(const BCand* __restrict__ queue, const uint32_t* __restrict__ q_count,
+                                         uint32_t q_cap, const uint32_t* __restrict__ steps,
+                                         unsigned long long* __restrict__ tile_steps) {
+    const uint32_t n = *q_count;
+    if (n > q_cap) return;   // skipped round: redone synchronously
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((i & ~63u) >= n) return;
+    const bool valid = i < n;
+    const uint32_t tile = valid ? queue[i].tile : 0xFFFFFFFFu;
+    const unsigned long long v = valid ? steps[i] : 0ull;
+    const uint32_t t0 = __shfl(tile, 0);
+    if (__ballot(valid && tile != t0) == 0) {
+        unsigned long long sum = v;
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if ((threadIdx.x & 63u) == 0) atomicAdd(&tile_steps[t0], sum);
+    } else if (valid) {
+        atomicAdd(&tile_steps[tile], v);
+    }
+}
 
 // Clears tile_cand of the tiles order[0, n) before they are filtered again.
 __global__ void clear_tile_cand_kernel(const uint32_t* __restrict__ order, uint32_t n, uint32_t* __restrict__ tile_cand) {
@@ -396,7 +419,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < kTrsW; k++) trs[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0;
-    uint32_t cand_steps = 0, cand_tile = 0;
+    uint32_t cand_steps = 0, cand_idx = 0;
     bool drained = false;
 
     auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
@@ -518,7 +541,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                 if (qi < n_queue) {
                     const BCand C = queue[qi];
                     const BPat& P = pats[C.pat];
-                    cand_tile = C.tile;
+                    cand_idx = qi;
                     cand_steps = 0;
                     L = P.L; M = P.M; D = P.D; R = P.R; E = P.E; pid = P.pid; hash_id = P.hash_id;
                     pa = (WORD)P.pa; pc = (WORD)P.pc; pg = (WORD)P.pg; pt = (WORD)P.pt;
@@ -555,6 +578,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     // root admission: padding at the window end, DP bound
                     busy = !bit(z, n - 1) && lbc(L, VP, VN, n) <= E;
                     if (busy) set4(0, children());
+                    else if (sink.cand_steps) sink.cand_steps[qi] = 0;
                 }
             }
             continue;
@@ -591,7 +615,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             set4(d, children());
         } else if (d == 0) {
             busy = false;
-            if (sink.tile_steps) atomicAdd(&sink.tile_steps[cand_tile], (unsigned long long)cand_steps);
+            if (sink.cand_steps) sink.cand_steps[cand_idx] = cand_steps;
         } else {
             d--;
             uint32_t op = get2(d);
@@ -736,16 +760,23 @@ void launch_filter(RoundCtx& c, const uint32_t* d_order, size_t t0, size_t t1, B
                            d_order + t0, tile_cand);
 }
 
+// Traceback of one queue; with `steps` (q_cap entries) it also records the
+// DFS transitions of each entry and adds them to the workspace's tile_steps.
 void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uint32_t q_cap, uint32_t* overflow,
-                   uint32_t* q_next, uint32_t blocks, hipStream_t st) {
+                   uint32_t* q_next, uint32_t blocks, hipStream_t st, uint32_t* steps) {
     size_t lds32 = (size_t)c.bp->n_cols * 64 * sizeof(uint32_t) * 2;
     size_t lds64 = (size_t)c.bp->n_cols * 64 * sizeof(uint64_t) * 2;
+    Sink sink = c.base_sink;
+    sink.cand_steps = steps;
     if (c.bp->groups64.empty())
         hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, c.d_genome, queue, q_count,
-                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, c.base_sink);
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, sink);
     else
         hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, c.d_genome, queue, q_count,
-                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, c.base_sink);
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, sink);
+    if (steps)
+        hipLaunchKernelGGL(tile_steps_reduce_kernel, dim3((q_cap + 255) / 256), dim3(256), 0, st, queue, q_count, q_cap,
+                           (const uint32_t*)steps, c.ws.tile_steps);
 }
 
 }  // namespace
@@ -823,7 +854,14 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
                 Sink{}};
     rc.base_sink = Sink{(BulgeRaw*)ws.bulge_out, ws.counters + 1, ws.bulge_cap, bp->count_only ? 1 : 0,
                         (const DChunk*)ws.chunks, (uint32_t)g.chunks.size(), ws.sums, ws.sums + 1, ws.sums + 2,
-                        ws.counters + 3, (flags & COF_FLAG_TILE_STATS) ? ws.tile_steps : nullptr};
+                        ws.counters + 3, nullptr};
+    const bool want_steps = (flags & COF_FLAG_TILE_STATS) != 0;
+    auto ensure_steps = [&](int k, uint32_t cap) {
+        if (!want_steps) return;
+        size_t c = ws.qsteps_cap[k];
+        ws_grow((void**)&ws.qsteps[k], &c, cap, sizeof(uint32_t));
+        ws.qsteps_cap[k] = c;
+    };
     const uint32_t* d_order = ws.tile_order;
     auto t_start = std::chrono::steady_clock::now();
 
@@ -853,7 +891,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         if (nq) {
             HIPCHECK(hipEventRecord(ws.ev[2], st));
             launch_verify(rc, (const BCand*)ws.qs[0], cnt, q_cap, ws.round_ctr + 1, ws.round_ctr,
-                          std::min<uint32_t>((nq + 63) / 64, rc.verify_blocks), st);
+                          std::min<uint32_t>((nq + 63) / 64, rc.verify_blocks), st, want_steps ? ws.qsteps[0] : nullptr);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipEventRecord(ws.ev[3], st));
             HIPCHECK(hipEventSynchronize(ws.ev[3]));
@@ -883,6 +921,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     const size_t first = std::min<size_t>(n_tiles, std::max<uint32_t>(1, tune.first_tiles));
     const uint32_t cap0 = std::min<uint32_t>(cap_max, 1u << 24);
     ensure_queue(0, cap0);
+    ensure_steps(0, cap0);
     run_sync(0, first, cap0);
     if (first < n_tiles) {
         // 2. the rest in rounds expected to fill half a queue, at least ~4
@@ -901,6 +940,8 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         }
         ensure_queue(0, q_cap);
         ensure_queue(1, q_cap);
+        ensure_steps(0, q_cap);
+        ensure_steps(1, q_cap);
         std::vector<std::pair<size_t, size_t>> rounds;
         for (size_t t0 = first; t0 < n_tiles; t0 += round_tiles)
             rounds.push_back({t0, std::min<size_t>(n_tiles, t0 + round_tiles)});
@@ -926,7 +967,8 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
                 launch_filter(rc, d_order, rounds[r].first, rounds[r].second, (BCand*)ws.qs[buf], rq, q_cap, st);
                 HIPCHECK(hipEventRecord(ws.rev[2 * r], st));
                 HIPCHECK(hipStreamWaitEvent(vs, ws.rev[2 * r], 0));
-                launch_verify(rc, (const BCand*)ws.qs[buf], rq, q_cap, rq + 2, rq + 1, rc.verify_blocks, vs);
+                launch_verify(rc, (const BCand*)ws.qs[buf], rq, q_cap, rq + 2, rq + 1, rc.verify_blocks, vs,
+                              want_steps ? ws.qsteps[buf] : nullptr);
                 HIPCHECK(hipEventRecord(ws.rev[2 * r + 1], vs));
             }
             for (uint32_t r = R >= 2 ? R - 2 : 0; r < R; r++) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * r + 1], 0));

@@ -30,6 +30,8 @@ struct Workspace {
     // bulge pipeline
     void* qs[2] = {nullptr, nullptr};          // double-buffered candidate queues
     size_t qs_cap[2] = {0, 0};
+    uint32_t* qsteps[2] = {nullptr, nullptr};  // DFS transitions per queue entry (tile stats only)
+    size_t qsteps_cap[2] = {0, 0};
     uint32_t* tile_order = nullptr;            // tile processing order
     size_t tile_order_cap = 0;
     uint32_t* tile_cand = nullptr;             // candidates per tile (last search)
