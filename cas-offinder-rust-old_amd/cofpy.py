@@ -20,7 +20,12 @@ COF_FLAG_TILE_STATS = 8
 # Every symbol include/cof.h declares (checked by the CPU test suite).
 COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search",
                "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error",
-               "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats"]
+               "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
+               "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
+               "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model"]
+MLP_IN = 12
+MLP_PARAMS = 5057
+COF_MLP_EAGER = 1
 
 
 class CofTuning(C.Structure):
@@ -34,7 +39,7 @@ class CofTuning(C.Structure):
 
 class CofTileStat(C.Structure):
     _fields_ = [("first_end", C.c_uint64), ("n_ends", C.c_uint32), ("candidates", C.c_uint32),
-                ("dfs_steps", C.c_uint64)]
+                ("dfs_steps", C.c_uint64), ("chunk", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 class CofQuery(C.Structure):
@@ -98,8 +103,21 @@ def lib():
         L.cof_get_tuning.argtypes = [C.c_void_p, C.POINTER(CofTuning)]
         L.cof_autotune.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(CofTuning)]
         L.cof_get_tile_stats.argtypes = [C.c_void_p, C.POINTER(CofTileStat), C.c_uint64, C.POINTER(C.c_uint64)]
+        fp = C.POINTER(C.c_float)
+        L.cof_mlp_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.cof_mlp_destroy.argtypes = [C.c_void_p]
+        L.cof_mlp_destroy.restype = None
+        L.cof_mlp_get_params.argtypes = [C.c_void_p, fp]
+        L.cof_mlp_set_params.argtypes = [C.c_void_p, fp]
+        L.cof_mlp_forward.argtypes = [C.c_void_p, fp, C.c_uint64, fp]
+        L.cof_mlp_loss_grad.argtypes = [C.c_void_p, fp, fp, C.c_uint64, fp, fp]
+        L.cof_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32, fp]
+        L.cof_chunk_features.argtypes = [C.c_void_p, fp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.cof_set_cost_model.argtypes = [C.c_void_p, C.c_void_p]
         for f in ("cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
-                  "cof_get_stats", "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats"):
+                  "cof_get_stats", "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
+                  "cof_mlp_create", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward", "cof_mlp_loss_grad",
+                  "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -210,7 +228,15 @@ class Search:
         q.batch_nucl = batch_nucl
         q.flags = (COF_FLAG_COUNT_ONLY if count_only else 0) | flags
         self._q = q
+        self._qargs = (list(guides), list(filters), list(mms), dnas, rnas, batch_nucl)
+        self.flags = q.flags
         return q
+
+    def set_flags(self, flags):
+        """Same query with other COF_FLAG_* bits (COF_FLAG_COUNT_ONLY included)."""
+        g, f, m, d, r, b = self._qargs
+        q = self._make_query(g, f, m, d, r, b, False, flags)
+        self._check(lib().cof_set_query(self.h, C.byref(q)), "set_query")
 
     def set_query(self, guides, filters, mms, dnas=None, rnas=None, batch_nucl=0, count_only=False, flags=0):
         q = self._make_query(guides, filters, mms, dnas, rnas, batch_nucl, count_only, flags)
@@ -265,8 +291,22 @@ class Search:
         self._check(lib().cof_get_tile_stats(self.h, None, 0, C.byref(n)), "tile_stats")
         buf = (CofTileStat * max(1, n.value))()
         self._check(lib().cof_get_tile_stats(self.h, buf, n.value, C.byref(n)), "tile_stats")
-        dt = np.dtype([("first_end", "<u8"), ("n_ends", "<u4"), ("candidates", "<u4"), ("dfs_steps", "<u8")])
+        dt = np.dtype([("first_end", "<u8"), ("n_ends", "<u4"), ("candidates", "<u4"), ("dfs_steps", "<u8"),
+                       ("chunk", "<u4"), ("_pad", "<u4")])
         return np.frombuffer(bytes(buf), dtype=dt, count=n.value).copy()
+
+    def chunk_features(self):
+        """numpy float32 [n_chunks, 12]: cost-model features of every genome chunk for this query."""
+        import numpy as np
+        n = C.c_uint64()
+        self._check(lib().cof_chunk_features(self.h, None, 0, C.byref(n)), "chunk_features")
+        out = np.zeros((n.value, MLP_IN), dtype=np.float32)
+        self._check(lib().cof_chunk_features(self.h, _fptr(out), n.value, C.byref(n)), "chunk_features")
+        return out
+
+    def set_cost_model(self, mlp):
+        """Order bulge work by the model's predictions (None: static genome order)."""
+        self._check(lib().cof_set_cost_model(self.h, mlp.h if mlp is not None else None), "set_cost_model")
 
     def autotune(self, sample_nucl=0):
         t = CofTuning()
@@ -276,6 +316,70 @@ class Search:
     def close(self):
         if getattr(self, "h", None):
             lib().cof_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """Device-resident FP32 MLP 12 -> 64 -> 64 -> 1 (the bulge scheduler's cost model)."""
+
+    def __init__(self, seed=1, device=0):
+        h = C.c_void_p()
+        st = lib().cof_mlp_create(device, seed, C.byref(h))
+        if st != COF_OK:
+            raise CofError(st, "mlp_create")
+        self.h = h
+
+    def _check(self, st, what):
+        if st != COF_OK:
+            raise CofError(st, what)
+
+    def params(self):
+        import numpy as np
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(lib().cof_mlp_get_params(self.h, _fptr(p)), "get_params")
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._check(lib().cof_mlp_set_params(self.h, _fptr(p)), "set_params")
+
+    def forward(self, x):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.zeros(len(x), dtype=np.float32)
+        self._check(lib().cof_mlp_forward(self.h, _fptr(x), len(x), _fptr(y)), "forward")
+        return y
+
+    def loss_grad(self, x, t):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().cof_mlp_loss_grad(self.h, _fptr(x), _fptr(t), len(x), C.byref(loss), _fptr(g)), "loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, steps, lr=1e-3, eager=False):
+        import numpy as np
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().cof_mlp_train(self.h, _fptr(x), _fptr(t), len(x), steps, lr,
+                                        COF_MLP_EAGER if eager else 0, C.byref(loss)), "train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cof_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

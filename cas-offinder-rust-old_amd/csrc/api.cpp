@@ -2,12 +2,18 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <memory>
 
 #include "cof_internal.hpp"
+#include "costmodel.hpp"
 #include "engine.hpp"
 #include "hip_check.hpp"
+
+struct cof_mlp {
+    std::unique_ptr<cof::Mlp> m;
+};
 
 struct cof_ctx {
     int device = 0;
@@ -20,8 +26,12 @@ struct cof_ctx {
     cof::PlainPlan* plain = nullptr;
     cof::BulgePlan* bulge = nullptr;
     cof::Workspace ws;
-    std::vector<uint32_t> tile_order;   // bulge tile processing order (cost model)
     cof::PlainTune plain_tune;
+    // cost model: parameters (empty: static order), device copies, features
+    std::vector<float> model;
+    cof::FeaturePlan* feat = nullptr;
+    cof::DevBuf d_model, d_feat, d_feat_chunks, d_pred;
+    std::vector<float> chunk_cost;
     cof::BulgeTune bulge_tune;
     bool have_genome = false;
     bool cached = false;
@@ -55,6 +65,29 @@ void upload_genome(cof_ctx* ctx) {
     ctx->cached = false;
     ctx->stats.genome_nucl = 0;
     for (auto& c : ctx->g.chunks) ctx->stats.genome_nucl += c.size();
+}
+
+// chunk features of the resident genome for the query, on the device
+void compute_features(cof_ctx* ctx) {
+    if (!ctx->feat) ctx->feat = cof::feature_plan_create(ctx->q);
+    const size_t nc = ctx->g.chunks.size();
+    ctx->d_feat.ensure(std::max<size_t>(1, nc) * cof::kMlpIn * sizeof(float));
+    cof::chunk_features(ctx->feat, ctx->g, ctx->d_genome, ctx->stream, ctx->d_feat_chunks, (float*)ctx->d_feat.p);
+}
+
+// predicted traceback work per chunk: exp(model(features)) - 1
+void predict_chunk_cost(cof_ctx* ctx) {
+    const size_t nc = ctx->g.chunks.size();
+    compute_features(ctx);
+    ctx->d_pred.ensure(std::max<size_t>(1, nc) * sizeof(float));
+    cof::mlp_forward_dev((const float*)ctx->d_model.p, (const float*)ctx->d_feat.p, (uint32_t)nc,
+                         (float*)ctx->d_pred.p, ctx->stream);
+    ctx->chunk_cost.resize(nc);
+    if (nc)
+        HIPCHECK(hipMemcpyAsync(ctx->chunk_cost.data(), ctx->d_pred.p, nc * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (float& c : ctx->chunk_cost) c = std::expm1(std::min(c, 60.f));
 }
 
 cof_tuning get_tuning(const cof_ctx* ctx) {
@@ -178,6 +211,8 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
         }
         cof::plain_plan_destroy(ctx->plain);
         cof::bulge_plan_destroy(ctx->bulge);
+        cof::feature_plan_destroy(ctx->feat);
+        ctx->feat = nullptr;
         ctx->plain = np;
         ctx->bulge = nb;
         bool batch_changed = nq.batch_nucl != ctx->q.batch_nucl;
@@ -256,7 +291,10 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             std::vector<cof::BulgeRaw> braw;
             cof::BulgeTune bt = ctx->bulge_tune;
             bt.flags = ctx->q.flags;
-            bt.order = ctx->tile_order.empty() ? nullptr : &ctx->tile_order;
+            if (!ctx->model.empty() && ctx->bulge && cof::bulge_plan_size(ctx->bulge)) {
+                predict_chunk_cost(ctx);
+                bt.chunk_cost = &ctx->chunk_cost;
+            }
             cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
             st.n_raw_hits = praw.size() + braw.size();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
@@ -328,11 +366,119 @@ cof_status cof_get_tile_stats(const cof_ctx* ctx, cof_tile_stat* out, uint64_t c
             HIPCHECK(hipSetDevice(ctx->device));
             HIPCHECK(hipMemcpy(cand.data(), ws.tile_cand, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
             HIPCHECK(hipMemcpy(steps.data(), ws.tile_steps, k * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-            for (uint64_t t = 0; t < k; t++) out[t] = cof_tile_stat{ws.tile_j0[t], ws.tile_n[t], cand[t], steps[t]};
+            for (uint64_t t = 0; t < k; t++)
+                out[t] = cof_tile_stat{ws.tile_j0[t], ws.tile_n[t], cand[t], steps[t], ws.tile_chunk[t], 0u};
         }
         if (cap < nt && out) return COF_E_OVERFLOW;
     } catch (const cof::Error& e) {
         return COF_E_HIP;
+    }
+    return COF_OK;
+}
+
+cof_status cof_chunk_features(cof_ctx* ctx, float* out, uint64_t cap_rows, uint64_t* n_rows) {
+    if (!ctx || (!out && cap_rows)) return COF_E_INVALID;
+    if (!ctx->have_genome) return COF_E_NOGENOME;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        const uint64_t nc = ctx->g.chunks.size();
+        if (n_rows) *n_rows = nc;
+        if (!out) return COF_OK;
+        compute_features(ctx);
+        const uint64_t k = std::min(cap_rows, nc);
+        if (k)
+            HIPCHECK(hipMemcpyAsync(out, ctx->d_feat.p, k * cof::kMlpIn * sizeof(float), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        if (cap_rows < nc) return COF_E_OVERFLOW;
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_set_cost_model(cof_ctx* ctx, const cof_mlp* m) {
+    if (!ctx) return COF_E_INVALID;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        if (!m) {
+            ctx->model.clear();
+        } else {
+            ctx->model.resize(cof::kMlpParams);
+            m->m->get_params(ctx->model.data());
+            HIPCHECK(hipSetDevice(ctx->device));
+            ctx->d_model.ensure(cof::kMlpParams * sizeof(float));
+            HIPCHECK(hipMemcpy(ctx->d_model.p, ctx->model.data(), cof::kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+        }
+        ctx->cached = false;
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_mlp_create(int hip_device, uint64_t seed, cof_mlp** out) {
+    if (!out) return COF_E_INVALID;
+    *out = nullptr;
+    try {
+        auto m = std::make_unique<cof_mlp>();
+        m->m = std::make_unique<cof::Mlp>(hip_device, seed);
+        *out = m.release();
+    } catch (const cof::Error& e) {
+        return e.code;
+    }
+    return COF_OK;
+}
+
+void cof_mlp_destroy(cof_mlp* m) { delete m; }
+
+cof_status cof_mlp_get_params(const cof_mlp* m, float* out) {
+    if (!m || !out) return COF_E_INVALID;
+    try {
+        m->m->get_params(out);
+    } catch (const cof::Error& e) {
+        return e.code;
+    }
+    return COF_OK;
+}
+
+cof_status cof_mlp_set_params(cof_mlp* m, const float* p) {
+    if (!m || !p) return COF_E_INVALID;
+    try {
+        m->m->set_params(p);
+    } catch (const cof::Error& e) {
+        return e.code;
+    }
+    return COF_OK;
+}
+
+cof_status cof_mlp_forward(cof_mlp* m, const float* x, uint64_t n_rows, float* y) {
+    if (!m || (n_rows && (!x || !y)) || n_rows > 0xFFFFFFFFull) return COF_E_INVALID;
+    try {
+        m->m->forward(x, n_rows, y);
+    } catch (const cof::Error& e) {
+        return e.code;
+    }
+    return COF_OK;
+}
+
+cof_status cof_mlp_loss_grad(cof_mlp* m, const float* x, const float* t, uint64_t n_rows, float* loss, float* grad) {
+    if (!m || !x || !t || !n_rows || n_rows > 0xFFFFFFFFull) return COF_E_INVALID;
+    try {
+        m->m->loss_grad(x, t, n_rows, loss, grad);
+    } catch (const cof::Error& e) {
+        return e.code;
+    }
+    return COF_OK;
+}
+
+cof_status cof_mlp_train(cof_mlp* m, const float* x, const float* t, uint64_t n_rows, uint32_t steps, float lr,
+                         uint32_t flags, float* loss) {
+    if (!m || !x || !t || !n_rows || n_rows > 0xFFFFFFFFull || !(lr > 0)) return COF_E_INVALID;
+    try {
+        m->m->train(x, t, n_rows, steps, lr, (flags & COF_MLP_EAGER) != 0, loss);
+    } catch (const cof::Error& e) {
+        return e.code;
     }
     return COF_OK;
 }
@@ -361,6 +507,7 @@ void cof_close(cof_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     cof::plain_plan_destroy(ctx->plain);
     cof::bulge_plan_destroy(ctx->bulge);
+    cof::feature_plan_destroy(ctx->feat);
     if (ctx->d_genome) (void)hipFree(ctx->d_genome);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

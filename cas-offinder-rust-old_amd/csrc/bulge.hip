@@ -796,11 +796,11 @@ void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uin
 void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
                const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats,
                const BulgeTune& tune) {
-    const std::vector<uint32_t>* order = tune.order;
     const uint32_t flags = tune.flags;
     out.clear();
     ws.tile_j0.clear();
     ws.tile_n.clear();
+    ws.tile_chunk.clear();
     if (!bp || bp->pats.empty()) return;
     ws.ensure_events();
     std::vector<BTile> tiles;
@@ -813,13 +813,31 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     }
     if (tiles.empty()) return;
     const uint32_t n_tiles = (uint32_t)tiles.size();
-    std::vector<uint32_t> ord(n_tiles);
-    if (order && order->size() == n_tiles) ord = *order;
-    else for (uint32_t t = 0; t < n_tiles; t++) ord[t] = t;
     ws.last_tiles = n_tiles;
     ws.tile_j0.resize(n_tiles);
     ws.tile_n.resize(n_tiles);
-    for (uint32_t t = 0; t < n_tiles; t++) { ws.tile_j0[t] = tiles[t].j0; ws.tile_n[t] = tiles[t].n; }
+    ws.tile_chunk.resize(n_tiles);
+    for (uint32_t t = 0; t < n_tiles; t++) {
+        ws.tile_j0[t] = tiles[t].j0;
+        ws.tile_n[t] = tiles[t].n;
+        ws.tile_chunk[t] = (uint32_t)g.chunk_of(tiles[t].j0);
+    }
+    // Work weight per tile: its end positions, or with a cost model the
+    // chunk's predicted traceback work shared out by end positions (+1 per
+    // end for the filter).  With a model, tiles run heavy-first.
+    std::vector<double> wt(n_tiles);
+    const std::vector<float>* cc = tune.chunk_cost;
+    const bool model = cc && cc->size() == g.chunks.size();
+    for (uint32_t t = 0; t < n_tiles; t++) {
+        wt[t] = tiles[t].n;
+        if (model) {
+            const uint32_t c = ws.tile_chunk[t];
+            wt[t] += (double)std::max(0.f, (*cc)[c]) * tiles[t].n / (double)std::max<uint64_t>(1, g.chunks[c].size());
+        }
+    }
+    std::vector<uint32_t> ord(n_tiles);
+    for (uint32_t t = 0; t < n_tiles; t++) ord[t] = t;
+    if (model) std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return wt[a] > wt[b]; });
     ws_grow(&ws.tiles, &ws.tiles_cap, n_tiles, sizeof(BTile));
     ws_grow((void**)&ws.tile_order, &ws.tile_order_cap, n_tiles, sizeof(uint32_t));
     ws_grow((void**)&ws.tile_cand, &ws.tile_cand_cap, n_tiles, sizeof(uint32_t));
@@ -926,12 +944,13 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     if (first < n_tiles) {
         // 2. the rest in rounds expected to fill half a queue, at least ~4
         //    rounds (so filter and traceback overlap) unless rounds would fall
-        //    below 4 Mi candidates
-        const size_t remaining = n_tiles - first;
-        const double per_tile = std::max(1.0, (double)cand_sync / (double)first);
-        const double target = std::min(0.5 * cap_max, std::max(per_tile * remaining / 4.0, (double)(1u << 22)));
-        size_t round_tiles = tune.round_tiles ? tune.round_tiles
-                                              : (size_t)std::max(1.0, std::min(target / per_tile, (double)remaining));
+        //    below 4 Mi candidates; the first round calibrates candidates per
+        //    unit of weight
+        double w_first = 0, w_rest = 0;
+        for (size_t k = 0; k < first; k++) w_first += wt[ord[k]];
+        for (size_t k = first; k < n_tiles; k++) w_rest += wt[ord[k]];
+        const double per_w = std::max(1.0, (double)cand_sync) / std::max(1.0, w_first);
+        const double target = std::min(0.5 * cap_max, std::max(per_w * w_rest / 4.0, (double)(1u << 22)));
         uint32_t q_cap = cap_max;
         if (!tune.round_tiles) {
             uint64_t want = 1;
@@ -943,8 +962,21 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         ensure_steps(0, q_cap);
         ensure_steps(1, q_cap);
         std::vector<std::pair<size_t, size_t>> rounds;
-        for (size_t t0 = first; t0 < n_tiles; t0 += round_tiles)
-            rounds.push_back({t0, std::min<size_t>(n_tiles, t0 + round_tiles)});
+        if (tune.round_tiles) {
+            for (size_t t0 = first; t0 < n_tiles; t0 += tune.round_tiles)
+                rounds.push_back({t0, std::min<size_t>(n_tiles, t0 + tune.round_tiles)});
+        } else {
+            size_t t0 = first;
+            double acc = 0;
+            for (size_t k = first; k < n_tiles; k++) {
+                acc += wt[ord[k]] * per_w;
+                if (acc >= target || k + 1 == n_tiles) {
+                    rounds.push_back({t0, k + 1});
+                    t0 = k + 1;
+                    acc = 0;
+                }
+            }
+        }
         const uint32_t R = (uint32_t)rounds.size();
         if (flags & COF_FLAG_SYNC_ROUNDS) {
             for (auto& r : rounds) run_sync(r.first, r.second, q_cap);

@@ -542,7 +542,7 @@ void Mlp::loss_grad(const float* x, const float* t, uint64_t n, float* loss, flo
     if (loss) *loss = g[kMlpParams];
 }
 
-void Mlp::train(const float* x, const float* t, uint64_t n, uint32_t steps, float lr, float* loss) {
+void Mlp::train(const float* x, const float* t, uint64_t n, uint32_t steps, float lr, bool eager, float* loss) {
     HIPCHECK(hipSetDevice(device));
     if (!n) throw Error(COF_E_INVALID, "no rows");
     DevBuf dx, dt;
@@ -551,27 +551,35 @@ void Mlp::train(const float* x, const float* t, uint64_t n, uint32_t steps, floa
     HIPCHECK(hipMemcpyAsync(dx.p, x, n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipMemcpyAsync(dt.p, t, n * sizeof(float), hipMemcpyHostToDevice, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    // one full-batch Adam step captured once, replayed `steps` times
+    auto step = [&]() {
+        launch_grad((const float*)dx.p, (const float*)dt.p, (uint32_t)n);
+        hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, d_params, d_m, d_v,
+                           d_grad, d_step, lr);
+        hipLaunchKernelGGL(mlp_step_kernel, dim3(1), dim3(1), 0, stream, d_step);
+    };
     if (train_exec) {
         (void)hipGraphExecDestroy(train_exec);
         train_exec = nullptr;
     }
-    hipGraph_t graph = nullptr;
-    HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    launch_grad((const float*)dx.p, (const float*)dt.p, (uint32_t)n);
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, d_params, d_m, d_v,
-                       d_grad, d_step, lr);
-    hipLaunchKernelGGL(mlp_step_kernel, dim3(1), dim3(1), 0, stream, d_step);
-    HIPCHECK(hipStreamEndCapture(stream, &graph));
-    HIPCHECK(hipGraphInstantiate(&train_exec, graph, nullptr, nullptr, 0));
-    HIPCHECK(hipGraphDestroy(graph));
-    for (uint32_t s = 0; s < steps; s++) HIPCHECK(hipGraphLaunch(train_exec, stream));
+    if (eager) {
+        for (uint32_t s = 0; s < steps; s++) step();
+        HIPCHECK(hipGetLastError());
+    } else {
+        // one full-batch Adam step captured once, replayed `steps` times
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        step();
+        HIPCHECK(hipStreamEndCapture(stream, &graph));
+        HIPCHECK(hipGraphInstantiate(&train_exec, graph, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphDestroy(graph));
+        for (uint32_t s = 0; s < steps; s++) HIPCHECK(hipGraphLaunch(train_exec, stream));
+    }
     // loss after training
     launch_grad((const float*)dx.p, (const float*)dt.p, (uint32_t)n);
     float l = 0;
     HIPCHECK(hipMemcpyAsync(&l, d_grad + kMlpParams, sizeof(float), hipMemcpyDeviceToHost, stream));
     HIPCHECK(hipStreamSynchronize(stream));
-    HIPCHECK(hipGraphExecDestroy(train_exec));
+    if (train_exec) HIPCHECK(hipGraphExecDestroy(train_exec));
     train_exec = nullptr;
     if (loss) *loss = l;
 }

@@ -50,7 +50,7 @@ struct Mlp {
     void reset_optimizer();
     void forward(const float* x, uint64_t n, float* y);
     void loss_grad(const float* x, const float* t, uint64_t n, float* loss, float* grad);
-    void train(const float* x, const float* t, uint64_t n, uint32_t steps, float lr, float* loss);
+    void train(const float* x, const float* t, uint64_t n, uint32_t steps, float lr, bool eager, float* loss);
     void launch_grad(const float* d_x, const float* d_t, uint32_t n);
 };
 

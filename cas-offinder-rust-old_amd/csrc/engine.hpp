@@ -41,6 +41,7 @@ struct Workspace {
     uint32_t last_tiles = 0;
     std::vector<uint64_t> tile_j0;             // host copy of the last search's tiles
     std::vector<uint32_t> tile_n;
+    std::vector<uint32_t> tile_chunk;
     uint32_t* rounds = nullptr;                // per round: queue count, cursor, overflow
     size_t rounds_cap = 0;
     uint32_t* round_ctr = nullptr;             // synchronous round: cursor, overflow
@@ -66,7 +67,9 @@ struct BulgeTune {
     uint32_t verify_per_cu = 12;    // persistent traceback workgroups per CU
     uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
     uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
-    const std::vector<uint32_t>* order = nullptr;   // tile processing order (identity if null)
+    // predicted traceback work per genome chunk (cost model); null: static
+    // genome order and rounds of equal tile counts
+    const std::vector<float>* chunk_cost = nullptr;
 };
 
 struct PlainPlan;

@@ -151,6 +151,8 @@ typedef struct {
     uint32_t n_ends;       /* end positions in the tile (<= 4096) */
     uint32_t candidates;   /* Myers survivors sent to the traceback */
     uint64_t dfs_steps;    /* traceback DFS transitions spent on them (COF_FLAG_TILE_STATS, else 0) */
+    uint32_t chunk;        /* genome chunk holding first_end (row of cof_chunk_features) */
+    uint32_t _pad;
 } cof_tile_stat;
 
 /* Replaces search_chunk_cuda{,_myers}'s set-up (search.rs:322-343,1193-1281):
@@ -196,6 +198,37 @@ cof_status cof_set_tuning(cof_ctx* ctx, const cof_tuning* t);
 /* Per-tile work of the last bulge search: writes min(cap, tiles) records,
  * *n = number of tiles.  Build-only, no reference counterpart. */
 cof_status cof_get_tile_stats(const cof_ctx* ctx, cof_tile_stat* out, uint64_t cap, uint64_t* n);
+/* ---- Learned cost model of the bulge scheduler (no reference counterpart) ----
+ * An FP32 MLP 12 -> 64 -> 64 -> 1 (ReLU) maps 12 features of a genome chunk
+ * to a predicted log(1 + traceback work).  With a model set, the bulge path
+ * orders its filter tiles heavy-first and cuts rounds of equal predicted work;
+ * hits never depend on it.  Parameters (COF_MLP_PARAMS floats): W1[64][12],
+ * b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define COF_MLP_IN 12
+#define COF_MLP_PARAMS 5057
+typedef struct cof_mlp cof_mlp;
+/* New model on a HIP device, He-uniform weights from `seed`, zero biases. */
+cof_status cof_mlp_create(int hip_device, uint64_t seed, cof_mlp** out);
+void cof_mlp_destroy(cof_mlp* m);
+cof_status cof_mlp_get_params(const cof_mlp* m, float* out);
+cof_status cof_mlp_set_params(cof_mlp* m, const float* params);   /* also resets Adam */
+/* y[r] = model(x[r * 12 .. r * 12 + 11]) */
+cof_status cof_mlp_forward(cof_mlp* m, const float* x, uint64_t n_rows, float* y);
+/* loss = sum_r (y[r] - t[r])^2 / n_rows and its gradient (COF_MLP_PARAMS floats) */
+cof_status cof_mlp_loss_grad(cof_mlp* m, const float* x, const float* t, uint64_t n_rows, float* loss, float* grad);
+/* `steps` full-batch Adam steps at learning rate `lr`; *loss = loss afterwards.
+ * flags: COF_MLP_EAGER launches each step's kernels directly instead of
+ * replaying one captured hipGraph (same kernels, same results). */
+#define COF_MLP_EAGER 1u
+cof_status cof_mlp_train(cof_mlp* m, const float* x, const float* t, uint64_t n_rows, uint32_t steps, float lr,
+                         uint32_t flags, float* loss);
+/* Features of every genome chunk for the context's query (row c = chunk c,
+ * COF_MLP_IN floats per row), computed on the device. */
+cof_status cof_chunk_features(cof_ctx* ctx, float* out, uint64_t cap_rows, uint64_t* n_rows);
+/* Uses a copy of the model's current parameters for later searches (NULL:
+ * static genome order). */
+cof_status cof_set_cost_model(cof_ctx* ctx, const cof_mlp* m);
+
 /* Effective launch shapes, defaults filled in. */
 cof_status cof_get_tuning(const cof_ctx* ctx, cof_tuning* out);
 /* Times candidate shapes on the first `sample_nucl` nucleotides of the
