@@ -102,8 +102,7 @@ def bulge_workload(args):
     s = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * n, [4] * n, [2] * n, [2] * n, count_only=True,
                      flags=schedule_flags(args))
     s.load_synth(sg)
-    if args.autotune:
-        log(f"autotune: {s.autotune()}")
+    apply_tuning(s, args)
     for w in range(args.warmup):
         s.search()
         log(f"warmup {w + 1} done")
@@ -135,6 +134,90 @@ def bulge_workload(args):
            "kernel_ms_filter": round(filt / args.steps, 1), "kernel_ms_verify": round(ver / args.steps, 1)}
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def costmodel_workload(args):
+    """Cost-model work order for the bulge search: train on seeded genomes, compare wall time of orders.
+
+    Training rows: every 64 kbp chunk of 20 repeat-rich genomes (seeds 1001-1020,
+    --train-bp each) searched with the bulge query below; target log(1 + DFS
+    transitions of the chunk).  Evaluation: the bulge workload's genome and query
+    under static genome order, an untrained (random) model and the trained model.
+    """
+    import numpy as np
+    import costmodel
+    rng = random.Random(args.seed + 5)
+    guides = ["".join(rng.choice("ACGT") for _ in range(20)) for _ in range(args.bulge_guides)]
+    plants = [(g + "NRG", 0, 20, 4, 100) for g in guides]
+    pats = [g + "NNN" for g in guides]
+    n = len(pats)
+    filt = ["NNNNNNNNNNNNNNNNNNNNNRG"] * n
+    s = cofpy.Search(pats, filt, [4] * n, [2] * n, [2] * n, count_only=True, flags=schedule_flags(args))
+    if args.tune:
+        s.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
+    xs, ts = [], []
+    t0 = time.time()
+    train_bp = int(args.train_bp)
+    for k, seed in enumerate(range(1001, 1001 + args.train_genomes)):
+        lens = chrom_lengths(train_bp, 24, seed)
+        sg = cofpy.SynthGenome(seed, lens, n_frac=0.05, gap=(10_000, 1_000_000), repeats=(0.10, 20, 300, 0.10),
+                               plants=plants)
+        s.load_synth(sg)
+        x, t = costmodel.collect(s)
+        xs.append(x)
+        ts.append(t)
+        log(f"training genome {k + 1}/{args.train_genomes} (seed {seed}): {len(x)} chunks, "
+            f"{time.time() - t0:.1f} s so far")
+    X, T = np.concatenate(xs), np.concatenate(ts)
+    t1 = time.time()
+    trained, loss = costmodel.fit(X, T, steps=args.train_steps)
+    pred = trained.forward(X)
+    r2 = 1 - float(np.mean((pred - T) ** 2) / np.var(T))
+    log(f"trained on {len(X)} rows in {time.time() - t1:.2f} s: standardised loss {loss:.4f}, R^2 {r2:.3f}")
+    total = int(args.genome_bp)
+    lens = chrom_lengths(total, 24, args.seed)
+    sg = cofpy.SynthGenome(args.seed + 3, lens, n_frac=0.05, gap=(10_000, 1_000_000), repeats=(0.10, 20, 300, 0.10),
+                           plants=plants)
+    s.load_synth(sg)
+    xe, te = costmodel.collect(s)
+    pe = trained.forward(xe)
+    r2e = 1 - float(np.mean((pe - te) ** 2) / np.var(te))
+    log(f"evaluation genome {total / 1e9:.2f} Gbp: held-out R^2 {r2e:.3f}")
+    untrained = cofpy.Mlp(seed=7)
+    res = {}
+    for name, m in (("static", None), ("untrained", untrained), ("trained", trained)):
+        s.set_cost_model(m)
+        times = []
+        for k in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            s.search()
+            dt = time.perf_counter() - t0
+            st = s.stats()
+            if k >= args.warmup:
+                times.append(dt)
+            log(f"{name} {'warmup' if k < args.warmup else 'step'}: {dt:.3f} s, hits {st.bulge_hits}, "
+                f"rounds {st.n_pipeline_rounds}, redone {st.n_redo_rounds}")
+        res[name] = {"seconds": round(sum(times) / len(times), 3), "bulge_hits": st.bulge_hits,
+                     "checksum": st.bulge_checksum}
+    out = {"metric": "bulge search wall time by work order (cost model)", "value": res["trained"]["seconds"],
+           "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(res["trained"]["seconds"] * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32 (MLP), u32/u64 (search)", "data": "synthetic",
+           "config": {"workload": f"bulge search {total / 1e9:.2f} Gbp repeat-rich synthetic, {n} guides NRG M4 D2 R2; "
+                                  f"model trained on {args.train_genomes} x {train_bp / 1e6:.0f} Mbp genomes",
+                      "genome_bp": st.genome_nucl},
+           "orders": res, "speedup_vs_static": round(res["static"]["seconds"] / res["trained"]["seconds"], 3),
+           "identical": len({(v["bulge_hits"], v["checksum"]) for v in res.values()}) == 1,
+           "train_rows": len(X), "train_r2": round(r2, 3), "heldout_r2": round(r2e, 3)}
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+def apply_tuning(s, args):
+    if args.tune:
+        s.set_tuning(**{k: int(v) for k, v in (t.split("=") for t in args.tune)})
+    if args.autotune:
+        log(f"autotune: {s.autotune()}")
 
 
 def schedule_flags(args):
@@ -172,8 +255,7 @@ def batched_workload(args):
     s = cofpy.Search(list(pats), list(filts), list(mms), list(ds), list(rs), count_only=True,
                      flags=schedule_flags(args))
     s.load_synth(sg)
-    if args.autotune:
-        log(f"autotune: {s.autotune()}")
+    apply_tuning(s, args)
     res = {}
     # batched: one query
     times = []
@@ -236,13 +318,18 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-bp", type=float, default=32e6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "bulge", "batched"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel"], default="plain")
+    ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
+    ap.add_argument("--train-genomes", type=int, default=20)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--bulge-guides", type=int, default=100)
     ap.add_argument("--batched-guides", type=int, default=10_000)
     ap.add_argument("--compare-plain-schedule", action="store_true")
     ap.add_argument("--schedule", choices=["graph", "eager", "sync"], default="graph",
                     help="bulge rounds: hipGraph pipeline, eager pipeline, or synchronous rounds")
     ap.add_argument("--autotune", action="store_true", help="run the autotuner on a 64 Mnt prefix first")
+    ap.add_argument("--tune", action="append", default=[], metavar="FIELD=VALUE",
+                    help="cof_tuning field for the bulge/batched/costmodel contexts (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the search is single-GPU by design "
@@ -252,6 +339,8 @@ def main():
         return bulge_workload(args)
     if args.workload == "batched":
         return batched_workload(args)
+    if args.workload == "costmodel":
+        return costmodel_workload(args)
 
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)

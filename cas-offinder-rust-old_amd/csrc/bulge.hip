@@ -838,6 +838,23 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     std::vector<uint32_t> ord(n_tiles);
     for (uint32_t t = 0; t < n_tiles; t++) ord[t] = t;
     if (model) std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return wt[a] > wt[b]; });
+    // The first (synchronous) round measures candidates per unit of weight:
+    // it takes tiles evenly spaced over the order, so the sample spans the
+    // genome (and, with a model, the whole cost range); the rest keep order.
+    const size_t first = std::min<size_t>(n_tiles, std::max<uint32_t>(1, tune.first_tiles));
+    if (first < n_tiles) {
+        std::vector<uint8_t> picked(n_tiles, 0);
+        std::vector<uint32_t> o2;
+        o2.reserve(n_tiles);
+        for (size_t i = 0; i < first; i++) {
+            const size_t k = i * (size_t)n_tiles / first;
+            picked[k] = 1;
+            o2.push_back(ord[k]);
+        }
+        for (size_t k = 0; k < n_tiles; k++)
+            if (!picked[k]) o2.push_back(ord[k]);
+        ord.swap(o2);
+    }
     ws_grow(&ws.tiles, &ws.tiles_cap, n_tiles, sizeof(BTile));
     ws_grow((void**)&ws.tile_order, &ws.tile_order_cap, n_tiles, sizeof(uint32_t));
     ws_grow((void**)&ws.tile_cand, &ws.tile_cand_cap, n_tiles, sizeof(uint32_t));
@@ -935,13 +952,44 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         }
     };
 
-    // 1. first round, synchronous: measures the candidate density
-    const size_t first = std::min<size_t>(n_tiles, std::max<uint32_t>(1, tune.first_tiles));
+    // 1. first round: its filter runs synchronously and measures the
+    //    candidate density; its traceback becomes round 0 of the pipeline
+    //    (own queue, so the pipeline's queues can be sized afterwards)
     const uint32_t cap0 = std::min<uint32_t>(cap_max, 1u << 24);
-    ensure_queue(0, cap0);
-    ensure_steps(0, cap0);
-    run_sync(0, first, cap0);
-    if (first < n_tiles) {
+    const bool sync_rounds = (flags & COF_FLAG_SYNC_ROUNDS) != 0;
+    bool have_r0 = false;
+    if (sync_rounds) {
+        ensure_queue(0, cap0);
+        ensure_steps(0, cap0);
+        run_sync(0, first, cap0);
+    } else {
+        ws_grow(&ws.q_first, &ws.q_first_cap, cap0, sizeof(BCand));
+        if (want_steps) ws_grow((void**)&ws.q_first_steps, &ws.q_first_steps_cap, cap0, sizeof(uint32_t));
+        uint32_t* cnt = ws.counters + 2;
+        HIPCHECK(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st));
+        HIPCHECK(hipEventRecord(ws.ev[0], st));
+        launch_filter(rc, d_order, 0, first, (BCand*)ws.q_first, cnt, cap0, st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ws.ev[1], st));
+        uint32_t nq = 0;
+        HIPCHECK(hipMemcpyAsync(&nq, cnt, sizeof nq, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
+        if (stats) stats->kernel_ms_filter += ms;
+        if (nq > cap0) {
+            // dense start: synchronous rounds in halves
+            if (stats) stats->n_redo_rounds++;
+            ensure_queue(0, cap0);
+            ensure_steps(0, cap0);
+            run_sync(0, first, cap0);
+        } else {
+            cand_sync = nq;
+            have_r0 = nq > 0;
+            if (stats) stats->n_candidates += nq;
+        }
+    }
+    if (first < n_tiles || have_r0) {
         // 2. the rest in rounds expected to fill half a queue, at least ~4
         //    rounds (so filter and traceback overlap) unless rounds would fall
         //    below 4 Mi candidates; the first round calibrates candidates per
@@ -957,11 +1005,8 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             while (want < (uint64_t)(2.0 * target)) want <<= 1;
             q_cap = (uint32_t)std::min<uint64_t>(cap_max, want);
         }
-        ensure_queue(0, q_cap);
-        ensure_queue(1, q_cap);
-        ensure_steps(0, q_cap);
-        ensure_steps(1, q_cap);
         std::vector<std::pair<size_t, size_t>> rounds;
+        if (have_r0) rounds.push_back({0, first});      // filtered already, queue q_first
         if (tune.round_tiles) {
             for (size_t t0 = first; t0 < n_tiles; t0 += tune.round_tiles)
                 rounds.push_back({t0, std::min<size_t>(n_tiles, t0 + tune.round_tiles)});
@@ -978,29 +1023,64 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             }
         }
         const uint32_t R = (uint32_t)rounds.size();
-        if (flags & COF_FLAG_SYNC_ROUNDS) {
+        if (R > (have_r0 ? 1u : 0u) || sync_rounds) {
+            ensure_queue(0, q_cap);
+            ensure_steps(0, q_cap);
+            if (!sync_rounds) {
+                ensure_queue(1, q_cap);
+                ensure_steps(1, q_cap);
+            }
+        }
+        if (sync_rounds) {
             for (auto& r : rounds) run_sync(r.first, r.second, q_cap);
         } else {
             // 3. pipeline: filter of round r on `st`, its traceback on
             //    st2/st3 alternately (consecutive tracebacks overlap, so one
-            //    round's tail does not idle the GPU); queue r%2 is refilled by
-            //    round r+2 only after round r's traceback ended
+            //    round's tail does not idle the GPU).  Round 0 may be the
+            //    first round (queue q_first, count in counters[2]); the other
+            //    rounds alternate between the two queues, a queue being
+            //    refilled only after the traceback that read it has ended.
             ws_grow((void**)&ws.rounds, &ws.rounds_cap, 3 * (size_t)R, sizeof(uint32_t));
             ws.ensure_round_events(R);
             const bool use_graph = !(flags & COF_FLAG_NO_GRAPH);
             hipGraph_t graph = nullptr;
             if (use_graph) HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
             fill_u32(ws.rounds, 3 * R, 0, st);
+            uint32_t k_buf = 0;                   // pipeline rounds using qs[]
+            std::vector<int> buf_of(R, -1);
             for (uint32_t r = 0; r < R; r++) {
                 uint32_t* rq = ws.rounds + 3 * r;          // count, cursor, overflow
-                const int buf = (int)(r & 1);
-                hipStream_t vs = buf ? ws.st3 : ws.st2;
-                if (r >= 2) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * (r - 2) + 1], 0));
-                launch_filter(rc, d_order, rounds[r].first, rounds[r].second, (BCand*)ws.qs[buf], rq, q_cap, st);
-                HIPCHECK(hipEventRecord(ws.rev[2 * r], st));
+                hipStream_t vs = (r & 1) ? ws.st3 : ws.st2;
+                const BCand* qv;
+                const uint32_t* qcount;
+                uint32_t cap_r;
+                uint32_t* steps;
+                if (have_r0 && r == 0) {
+                    qv = (const BCand*)ws.q_first;
+                    qcount = ws.counters + 2;
+                    cap_r = cap0;
+                    steps = want_steps ? ws.q_first_steps : nullptr;
+                    HIPCHECK(hipEventRecord(ws.rev[0], st));
+                } else {
+                    const int buf = (int)(k_buf & 1);
+                    buf_of[r] = buf;
+                    // the queue's previous user is two pipeline rounds back
+                    for (int q = (int)r - 1; q >= 0; q--) {
+                        if (buf_of[q] == buf) {
+                            HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * q + 1], 0));
+                            break;
+                        }
+                    }
+                    launch_filter(rc, d_order, rounds[r].first, rounds[r].second, (BCand*)ws.qs[buf], rq, q_cap, st);
+                    HIPCHECK(hipEventRecord(ws.rev[2 * r], st));
+                    qv = (const BCand*)ws.qs[buf];
+                    qcount = rq;
+                    cap_r = q_cap;
+                    steps = want_steps ? ws.qsteps[buf] : nullptr;
+                    k_buf++;
+                }
                 HIPCHECK(hipStreamWaitEvent(vs, ws.rev[2 * r], 0));
-                launch_verify(rc, (const BCand*)ws.qs[buf], rq, q_cap, rq + 2, rq + 1, rc.verify_blocks, vs,
-                              want_steps ? ws.qsteps[buf] : nullptr);
+                launch_verify(rc, qv, qcount, cap_r, rq + 2, rq + 1, rc.verify_blocks, vs, steps);
                 HIPCHECK(hipEventRecord(ws.rev[2 * r + 1], vs));
             }
             for (uint32_t r = R >= 2 ? R - 2 : 0; r < R; r++) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * r + 1], 0));
@@ -1017,7 +1097,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             HIPCHECK(hipGetLastError());
             std::vector<uint32_t> rs(3 * (size_t)R);
             HIPCHECK(hipMemcpy(rs.data(), ws.rounds, rs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-            for (uint32_t r = 0; r < R; r++) {
+            for (uint32_t r = (have_r0 ? 1u : 0u); r < R; r++) {
                 if (rs[3 * r] > q_cap) {
                     // the traceback skipped this round; redo it (in halves if needed)
                     if (stats) stats->n_redo_rounds++;

@@ -32,6 +32,10 @@ struct Workspace {
     size_t qs_cap[2] = {0, 0};
     uint32_t* qsteps[2] = {nullptr, nullptr};  // DFS transitions per queue entry (tile stats only)
     size_t qsteps_cap[2] = {0, 0};
+    void* q_first = nullptr;                   // queue of the first round
+    size_t q_first_cap = 0;
+    uint32_t* q_first_steps = nullptr;
+    size_t q_first_steps_cap = 0;
     uint32_t* tile_order = nullptr;            // tile processing order
     size_t tile_order_cap = 0;
     uint32_t* tile_cand = nullptr;             // candidates per tile (last search)
@@ -64,7 +68,7 @@ struct PlainTune {
 struct BulgeTune {
     uint32_t first_tiles = 1024;    // filter tiles in the synchronous first round
     uint32_t round_tiles = 0;       // tiles per pipeline round; 0 = sized from the first round
-    uint32_t verify_per_cu = 12;    // persistent traceback workgroups per CU
+    uint32_t verify_per_cu = 24;    // persistent traceback workgroups per CU
     uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
     uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
     // predicted traceback work per genome chunk (cost model); null: static

@@ -402,7 +402,8 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
 Workspace::~Workspace() {
     for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
-                    qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
+                    qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps,
+                    (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
                     (void*)round_ctr})
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);

@@ -139,7 +139,7 @@ typedef struct {
                                       first round of the bulge path (default 1024) */
     uint32_t bulge_round_tiles;    /* tiles per later round (0: sized from the first round's
                                       candidate density: about 4 rounds, each within half a queue) */
-    uint32_t verify_blocks_per_cu; /* persistent traceback workgroups per CU (default 12) */
+    uint32_t verify_blocks_per_cu; /* persistent traceback workgroups per CU (default 24) */
     uint32_t bulge_queue_cap;      /* candidate queue capacity per round (default 2^27 = 4 GiB of
                                       32-byte records, min 64); a round that overflows it is redone
                                       in halves */

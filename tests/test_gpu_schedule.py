@@ -114,10 +114,11 @@ def test_queue_overflow_redo(tmp_path, flags):
     cand = s.tile_stats()["candidates"].astype(np.int64)
     # a queue that holds the densest tile but not a whole round of them
     cap = max(64, int(cand.max()))
-    # the shortest round length (first round and pipeline rounds alike) at
-    # which some round overflows
-    rt = next(r for r in range(8, len(cand) + 1)
-              if max(cand[i:i + r].sum() for i in range(0, len(cand), r)) > cap)
+    # a round length at which every full round overflows, whichever tiles it
+    # holds: the rt lightest tiles already exceed the queue
+    light = np.cumsum(np.sort(cand))
+    rt = int(np.argmax(light > cap)) + 1
+    assert light[rt - 1] > cap and rt < len(cand)
     s.set_tuning(bulge_first_tiles=rt, bulge_round_tiles=rt, bulge_queue_cap=cap)
     assert _rows(s, tmp_path) == want
     assert s.stats().n_redo_rounds > 0
