@@ -118,7 +118,7 @@ def bulge_workload(args):
             f"bulge path {st.bulge_ms:.1f} ms ({st.n_pipeline_rounds} pipeline rounds, {st.n_redo_rounds} redone; "
             f"synchronous rounds: filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms), "
             f"dfs steps {st.verify_steps} ({st.verify_steps / max(st.n_candidates, 1):.1f}/cand), "
-            f"lane use {st.verify_steps / max(st.verify_slots, 1):.3f}")
+            f"lane use {st.verify_busy / max(st.verify_slots, 1):.3f}")
     st = s.stats()
     step = sum(times) / len(times)
     units = n * st.genome_nucl

@@ -31,7 +31,7 @@ COF_MLP_EAGER = 1
 class CofTuning(C.Structure):
     _fields_ = [("plain_threads", C.c_uint32), ("plain_pat_chunk", C.c_uint32), ("bulge_first_tiles", C.c_uint32),
                 ("bulge_round_tiles", C.c_uint32), ("verify_blocks_per_cu", C.c_uint32),
-                ("bulge_queue_cap", C.c_uint32)]
+                ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -71,7 +71,7 @@ class CofStats(C.Structure):
                 ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64),
                 ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
                 ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
-                ("n_redo_rounds", C.c_uint64)]
+                ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64)]
 
 
 _lib = None

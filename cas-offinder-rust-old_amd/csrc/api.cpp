@@ -98,6 +98,7 @@ cof_tuning get_tuning(const cof_ctx* ctx) {
     t.bulge_round_tiles = ctx->bulge_tune.round_tiles;
     t.verify_blocks_per_cu = ctx->bulge_tune.verify_per_cu;
     t.bulge_queue_cap = ctx->bulge_tune.queue_cap;
+    t.verify_refill = ctx->bulge_tune.refill;
     return t;
 }
 
@@ -120,6 +121,10 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
         if (t.bulge_queue_cap < 64 || t.bulge_queue_cap > (1u << 27))
             throw cof::Error(COF_E_INVALID, "bulge_queue_cap must be in [64, 2^27]");
         bt.queue_cap = t.bulge_queue_cap;
+    }
+    if (t.verify_refill) {
+        if (t.verify_refill > 64) throw cof::Error(COF_E_INVALID, "verify_refill must be in [1, 64]");
+        bt.refill = t.verify_refill;
     }
     ctx->plain_tune = pt;
     ctx->bulge_tune = bt;

@@ -329,9 +329,11 @@ struct Sink {
     uint32_t n_chunks;
     unsigned long long* kept;
     unsigned long long* checksum;
-    unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop
+    unsigned long long* diag;       // [0] DFS transitions, [1] lane slots of the DFS loop, [2] busy slots
     uint32_t* error;                // set when a lane reaches an impossible state
     uint32_t* cand_steps;           // DFS transitions per queue entry (null: not recorded)
+    const uint32_t* chunk_lut;      // [k] = chunk holding nibble k << 16 (count-only drop rule)
+    uint32_t refill;                // refill a wave once this many lanes are idle
 };
 
 // tile_steps[queue[i].tile] += steps[i] over the round's queue; a wave whose
@@ -413,13 +415,15 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     // DFS state
     bool busy = false;
     uint32_t i = 0, jj = 0, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
+    uint32_t d_floor = 0;   // depth the lane's subtree starts at (> 0 for donated work)
+    __shared__ uint32_t pairing[64];
     uint64_t ops[kOpsW], trs[kTrsW];
 #pragma unroll
     for (int k = 0; k < kOpsW; k++) ops[k] = 0;
 #pragma unroll
     for (int k = 0; k < kTrsW; k++) trs[k] = 0;
-    uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0;
-    uint32_t cand_steps = 0, cand_idx = 0;
+    uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0, my_busy = 0;
+    uint32_t cand_steps = 0, cand_idx = 0, cand_chunk = 0;
     bool drained = false;
 
     auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
@@ -506,17 +510,21 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             r.start = st; r.pattern = pid;
             r.mm = (uint8_t)mm; r.db = (uint8_t)db; r.rb = (uint8_t)rb; r.n_ops = (uint8_t)d;
             r.ops[0] = o0; r.ops[1] = o1; r.ops[2] = 0;
-            uint32_t idx = atomicAdd(sink.count, 1u);
+            // one reservation for the lanes emitting in this step
+            const uint64_t em = __ballot(1);
+            const uint32_t leader = (uint32_t)__builtin_ctzll(em);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(sink.count, (uint32_t)__popcll(em));
+            base = __shfl(base, (int)leader);
+            const uint32_t idx =
+                base + __builtin_amdgcn_mbcnt_hi((uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0u));
             if (idx < sink.cap) sink.out[idx] = r;
         } else {
             // convert_matches_myers' drop rule (search.rs:1338-1350) on the
-            // chunk of the alignment start
-            uint32_t lo = 0, hi = sink.n_chunks;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (sink.chunks[mid].nib_begin <= st) lo = mid + 1; else hi = mid;
-            }
-            uint32_t c = lo - 1;
+            // chunk of the alignment start: the candidate's first chunk or a
+            // later one
+            uint32_t c = cand_chunk;
+            while (c + 1 < sink.n_chunks && sink.chunks[c + 1].nib_begin <= st) c++;
             bool end_chrom = c + 1 == last_chunk || sink.chunks[c + 1].chrom_start;
             bool past_end = st - sink.chunks[c].nib_begin + (L + db - rb) > sink.chunks[c].size;
             if (!(end_chrom && past_end)) {
@@ -530,7 +538,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         // refill idle lanes from the queue, a wave at a time
         uint64_t idle = __ballot(!busy);
         uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (!drained && n_idle >= 8) {
+        if (!drained && n_idle >= sink.refill) {
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd(q_next, n_idle);
             b0 = __shfl(b0, 0);
@@ -548,6 +556,13 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     fa = (WORD)P.fa; fc = (WORD)P.fc; fg = (WORD)P.fg; ft = (WORD)P.ft;
                     pn = (WORD)P.is_n;
                     last_chunk = C.last_chunk;
+                    if (sink.count_only) {
+                        uint32_t c = sink.chunk_lut[C.j >> 16];
+                        const uint64_t t0 = C.j + 1 >= C.floor + L + D ? C.j + 1 - (L + D) : C.floor;
+                        while (c > 0 && sink.chunks[c].nib_begin > t0) c--;
+                        while (c + 1 < sink.n_chunks && sink.chunks[c + 1].nib_begin <= t0) c++;
+                        cand_chunk = c;
+                    }
                     uint32_t W = L + D;
                     ts = C.j + 1 >= C.floor + W ? C.j + 1 - W : C.floor;
                     uint32_t n = (uint32_t)(C.j - ts + 1);
@@ -574,7 +589,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                             cvn[(ii + 1) * 64 + lane] = VN;
                         }
                     }
-                    i = L; jj = n; d = 0; cost = mm = db = rb = 0;
+                    i = L; jj = n; d = 0; d_floor = 0; cost = mm = db = rb = 0;
                     // root admission: padding at the window end, DP bound
                     busy = !bit(z, n - 1) && lbc(L, VP, VN, n) <= E;
                     if (busy) set4(0, children());
@@ -584,51 +599,177 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             continue;
         }
         if (drained && __ballot(busy) == 0) break;
+        if (drained && !sink.cand_steps) {
+            // Work donation once the queue is empty: a busy lane with an
+            // unexplored sibling above its current node hands the siblings
+            // of its shallowest such depth, with the path to them and the
+            // candidate's context, to an idle lane of the wave.  The DFS of
+            // one long candidate then spreads over the wave instead of
+            // leaving the other lanes idle.
+            const uint64_t idle = __ballot(!busy);
+            if ((uint32_t)__popcll(idle) >= sink.refill) {
+                uint32_t kd = 0xFFFFFFFFu;
+                if (busy) {
+#pragma unroll
+                    for (int w = kTrsW - 1; w >= 0; w--) {
+                        const uint32_t lo = (uint32_t)w * 16u;
+                        uint64_t v = trs[w];
+                        // depths [max(lo, d_floor), min(lo + 16, d)) of this word
+                        if (d <= lo) v = 0;
+                        else if (d < lo + 16) v &= (1ull << (4 * (d - lo))) - 1ull;
+                        if (d_floor > lo) v = d_floor >= lo + 16 ? 0ull : v & ~((1ull << (4 * (d_floor - lo))) - 1ull);
+                        if (v) kd = lo + (uint32_t)__builtin_ctzll(v) / 4u;
+                    }
+                }
+                const bool can = kd != 0xFFFFFFFFu;
+                const uint64_t donors = __ballot(can);
+                const uint32_t n_pairs = min((uint32_t)__popcll(donors), (uint32_t)__popcll(idle));
+                if (n_pairs) {
+                    const uint32_t rd = __builtin_amdgcn_mbcnt_hi((uint32_t)(donors >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)donors, 0u));
+                    const uint32_t ri = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                    const bool give = can && rd < n_pairs;
+                    const bool recv = !busy && ri < n_pairs;
+                    if (give) pairing[rd] = lane;
+                    __syncthreads();
+                    const int src = recv ? (int)pairing[ri] : (int)lane;
+                    __syncthreads();
+                    // donor: node state at depth kd (undo the ops of depths [kd, d))
+                    uint32_t s_i = i, s_jj = jj, s_cost = cost, s_mm = mm, s_db = db, s_rb = rb, s_mask = 0;
+                    if (give) {
+                        uint32_t nS = 0, nD = 0, nR = 0;
+#pragma unroll
+                        for (int w = 0; w < kOpsW; w++) {
+                            const uint32_t lo = (uint32_t)w * 32u;
+                            uint64_t x = ops[w];
+                            uint64_t keep = ~0ull;
+                            if (d <= lo) keep = 0;
+                            else if (d < lo + 32) keep = (1ull << (2 * (d - lo))) - 1ull;
+                            if (kd > lo) keep = kd >= lo + 32 ? 0ull : keep & ~((1ull << (2 * (kd - lo))) - 1ull);
+                            x &= keep;
+                            const uint64_t b0 = x & 0x5555555555555555ull, b1 = (x >> 1) & 0x5555555555555555ull;
+                            nS += (uint32_t)__popcll(b0 & ~b1);
+                            nD += (uint32_t)__popcll(~b0 & b1 & 0x5555555555555555ull);
+                            nR += (uint32_t)__popcll(b0 & b1);
+                        }
+                        const uint32_t nM = d - kd - nS - nD - nR;
+                        s_i = i + nM + nS + nR;
+                        s_jj = jj + nM + nS + nD;
+                        s_cost = cost - nS - nD - nR;
+                        s_mm = mm - nS;
+                        s_db = db - nD;
+                        s_rb = rb - nR;
+                        s_mask = get4(kd);
+                        set4(kd, 0u);
+                    }
+                    // every lane takes part in the shuffles; receivers keep the values
+                    auto xfer32 = [&](uint32_t v) -> uint32_t { return (uint32_t)__shfl((int)v, src); };
+                    auto xfer64 = [&](uint64_t v) -> uint64_t {
+                        const uint32_t lo = xfer32((uint32_t)v), hi = xfer32((uint32_t)(v >> 32));
+                        return ((uint64_t)hi << 32) | lo;
+                    };
+                    auto xferw = [&](WORD v) -> WORD {
+                        if constexpr (sizeof(WORD) == 8) return (WORD)xfer64((uint64_t)v);
+                        else return (WORD)xfer32((uint32_t)v);
+                    };
+                    const uint32_t r_i = xfer32(s_i), r_jj = xfer32(s_jj), r_cost = xfer32(s_cost), r_mm = xfer32(s_mm),
+                                   r_db = xfer32(s_db), r_rb = xfer32(s_rb), r_mask = xfer32(s_mask),
+                                   r_kd = xfer32(kd);
+                    const uint64_t r_ts = xfer64(ts), r_hash = xfer64(hash_id);
+                    const WORD r_z = xferw(z), r_tN = xferw(tN), r_tA = xferw(tA), r_tC = xferw(tC), r_tG = xferw(tG),
+                               r_tT = xferw(tT);
+                    const WORD r_pa = xferw(pa), r_pc = xferw(pc), r_pg = xferw(pg), r_pt = xferw(pt),
+                               r_fa = xferw(fa), r_fc = xferw(fc), r_fg = xferw(fg), r_ft = xferw(ft),
+                               r_pn = xferw(pn);
+                    const uint32_t r_L = xfer32(L), r_M = xfer32(M), r_D = xfer32(D), r_R = xfer32(R),
+                                   r_E = xfer32(E), r_pid = xfer32(pid), r_last = xfer32(last_chunk),
+                                   r_chunk = xfer32(cand_chunk);
+                    uint64_t r_ops[kOpsW];
+#pragma unroll
+                    for (int w = 0; w < kOpsW; w++) r_ops[w] = xfer64(ops[w]);
+                    // the candidate's Myers columns, rows 0..L, into this lane's slots
+                    for (uint32_t r = 0; r <= r_L; r++) {
+                        const WORD vp = cvp[r * 64 + (uint32_t)src], vn = cvn[r * 64 + (uint32_t)src];
+                        if (recv) {
+                            cvp[r * 64 + lane] = vp;
+                            cvn[r * 64 + lane] = vn;
+                        }
+                    }
+                    if (recv) {
+                        i = r_i; jj = r_jj; cost = r_cost; mm = r_mm; db = r_db; rb = r_rb;
+                        d = r_kd; d_floor = r_kd;
+                        ts = r_ts; hash_id = r_hash;
+                        z = r_z; tN = r_tN; tA = r_tA; tC = r_tC; tG = r_tG; tT = r_tT;
+                        pa = r_pa; pc = r_pc; pg = r_pg; pt = r_pt; fa = r_fa; fc = r_fc; fg = r_fg; ft = r_ft;
+                        pn = r_pn;
+                        L = r_L; M = r_M; D = r_D; R = r_R; E = r_E; pid = r_pid; last_chunk = r_last;
+                        cand_chunk = r_chunk;
+#pragma unroll
+                        for (int w = 0; w < kOpsW; w++) ops[w] = r_ops[w];
+#pragma unroll
+                        for (int w = 0; w < kTrsW; w++) trs[w] = 0;
+                        set4(d, r_mask);
+                        busy = true;
+                    }
+                }
+            }
+        }
         my_slots++;
         if (!busy) continue;
-        my_steps++;
-        cand_steps++;
+        my_busy++;
+        // One iteration = pop exhausted nodes (cheap, may repeat) and then one
+        // push with its child expansion (the costly part), so the expansion
+        // runs with every busy lane of the wave at once.
         uint32_t viable = get4(d);
-        if (viable) {
-            // push the next admissible child and expand it
-            uint32_t t = (uint32_t)__builtin_ctz(viable);
-            set4(d, viable & (viable - 1));
-            if (t == 0) {
-                uint32_t mc = cmp_loose(pat_at(i - 1), text_at(jj - 1)) ? 0u : 1u;
-                set2(d, mc);
-                i--; jj--; cost += mc; mm += mc;
-            } else if (t == 1) {
-                set2(d, 3);
-                i--; cost++; rb++;
-            } else {
-                set2(d, 2);
-                jj--; cost++; db++;
-            }
-            d++;
-            if (d > (uint32_t)sizeof(WORD) * 8 || i > L || jj > (uint32_t)sizeof(WORD) * 8) {
-                // impossible state: stop this lane and report instead of looping
-                atomicOr(sink.error, 1u);
-                busy = false;
-                continue;
-            }
-            if (i == 0) emit();
-            set4(d, children());
-        } else if (d == 0) {
-            busy = false;
-            if (sink.cand_steps) sink.cand_steps[cand_idx] = cand_steps;
-        } else {
+        while (!viable && d > d_floor) {
+            my_steps++;
+            cand_steps++;
             d--;
-            uint32_t op = get2(d);
+            const uint32_t op = get2(d);
             if (op <= 1) { i++; jj++; cost -= op; mm -= op; }
             else if (op == 2) { jj++; cost--; db--; }
             else { i++; cost--; rb--; }
+            viable = get4(d);
         }
+        my_steps++;
+        cand_steps++;
+        if (!viable) {
+            // root exhausted: candidate done
+            busy = false;
+            if (sink.cand_steps) sink.cand_steps[cand_idx] = cand_steps;
+            continue;
+        }
+        // push the next admissible child and expand it
+        const uint32_t t = (uint32_t)__builtin_ctz(viable);
+        set4(d, viable & (viable - 1));
+        if (t == 0) {
+            uint32_t mc = cmp_loose(pat_at(i - 1), text_at(jj - 1)) ? 0u : 1u;
+            set2(d, mc);
+            i--; jj--; cost += mc; mm += mc;
+        } else if (t == 1) {
+            set2(d, 3);
+            i--; cost++; rb++;
+        } else {
+            set2(d, 2);
+            jj--; cost++; db++;
+        }
+        d++;
+        if (d > (uint32_t)sizeof(WORD) * 8 || i > L || jj > (uint32_t)sizeof(WORD) * 8) {
+            // impossible state: stop this lane and report instead of looping
+            atomicOr(sink.error, 1u);
+            busy = false;
+            continue;
+        }
+        if (i == 0) emit();
+        set4(d, children());
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_kept += __shfl_down(my_kept, off);
         my_sum += __shfl_down(my_sum, off);
         my_steps += __shfl_down(my_steps, off);
         my_slots += __shfl_down(my_slots, off);
+        my_busy += __shfl_down(my_busy, off);
     }
     if (lane == 0) {
         if (sink.count_only && my_kept) {
@@ -637,6 +778,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         }
         atomicAdd(&sink.diag[0], (unsigned long long)my_steps);
         atomicAdd(&sink.diag[1], (unsigned long long)my_slots);
+        atomicAdd(&sink.diag[2], (unsigned long long)my_busy);
     }
 }
 
@@ -863,7 +1005,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     HIPCHECK(hipMemcpyAsync(ws.tile_order, ord.data(), n_tiles * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemsetAsync(ws.tile_cand, 0, n_tiles * sizeof(uint32_t), st));
     HIPCHECK(hipMemsetAsync(ws.tile_steps, 0, n_tiles * sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(ws.sums, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHECK(hipMemsetAsync(ws.sums, 0, 8 * sizeof(unsigned long long), st));
     HIPCHECK(hipMemsetAsync(ws.counters, 0, 4 * sizeof(uint32_t), st));
     if (bp->count_only) {
         std::vector<DChunk> dc(g.chunks.size());
@@ -871,6 +1013,16 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             dc[c] = DChunk{g.chunks[c].nib_begin(), (uint32_t)g.chunks[c].size(), g.chunks[c].start == 0 ? 1u : 0u};
         ws_grow(&ws.chunks, &ws.chunks_cap, dc.size(), sizeof(DChunk));
         HIPCHECK(hipMemcpyAsync(ws.chunks, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice, st));
+        // chunk holding nibble k << 16, for the device drop rule
+        std::vector<uint32_t> lut((g.n_nibbles() >> 16) + 2);
+        uint32_t c = 0;
+        for (size_t k = 0; k < lut.size(); k++) {
+            while (c + 1 < g.chunks.size() && g.chunks[c + 1].nib_begin() <= ((uint64_t)k << 16)) c++;
+            lut[k] = c;
+        }
+        ws_grow((void**)&ws.chunk_lut, &ws.chunk_lut_cap, lut.size(), sizeof(uint32_t));
+        HIPCHECK(hipMemcpyAsync(ws.chunk_lut, lut.data(), lut.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
     }
     const uint32_t cap_max = std::max<uint32_t>(64, tune.queue_cap);
     auto ensure_queue = [&](int k, uint32_t cap) {
@@ -889,7 +1041,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
                 Sink{}};
     rc.base_sink = Sink{(BulgeRaw*)ws.bulge_out, ws.counters + 1, ws.bulge_cap, bp->count_only ? 1 : 0,
                         (const DChunk*)ws.chunks, (uint32_t)g.chunks.size(), ws.sums, ws.sums + 1, ws.sums + 2,
-                        ws.counters + 3, nullptr};
+                        ws.counters + 3, nullptr, ws.chunk_lut, std::max(1u, std::min(64u, tune.refill))};
     const bool want_steps = (flags & COF_FLAG_TILE_STATS) != 0;
     auto ensure_steps = [&](int k, uint32_t cap) {
         if (!want_steps) return;
@@ -1131,10 +1283,11 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         if (no) HIPCHECK(hipMemcpy(out.data(), ws.bulge_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost));
     }
     if (stats) {
-        unsigned long long sums[4] = {0, 0, 0, 0};
+        unsigned long long sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         HIPCHECK(hipMemcpy(sums, ws.sums, sizeof sums, hipMemcpyDeviceToHost));
         stats->verify_steps = sums[2];
         stats->verify_slots = sums[3];
+        stats->verify_busy = sums[4];
         if (bp->count_only) {
             stats->bulge_hits = sums[0];
             stats->bulge_checksum = sums[1];

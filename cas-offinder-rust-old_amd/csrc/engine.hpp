@@ -24,7 +24,7 @@ struct Workspace {
     void* chunks = nullptr;        // chunk table for the device drop rule
     size_t chunks_cap = 0;
     uint32_t* counters = nullptr;  // [0] queue count, [1] record count, [2] queue cursor
-    unsigned long long* sums = nullptr;  // [0] kept, [1] checksum, [2..3] diagnostics
+    unsigned long long* sums = nullptr;  // [0] kept, [1] checksum, [2..4] diagnostics
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus = 0;
     // bulge pipeline
@@ -32,6 +32,8 @@ struct Workspace {
     size_t qs_cap[2] = {0, 0};
     uint32_t* qsteps[2] = {nullptr, nullptr};  // DFS transitions per queue entry (tile stats only)
     size_t qsteps_cap[2] = {0, 0};
+    uint32_t* chunk_lut = nullptr;             // chunk of nibble k << 16
+    size_t chunk_lut_cap = 0;
     void* q_first = nullptr;                   // queue of the first round
     size_t q_first_cap = 0;
     uint32_t* q_first_steps = nullptr;
@@ -70,6 +72,7 @@ struct BulgeTune {
     uint32_t round_tiles = 0;       // tiles per pipeline round; 0 = sized from the first round
     uint32_t verify_per_cu = 24;    // persistent traceback workgroups per CU
     uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
+    uint32_t refill = 8;            // traceback wave refills once this many lanes are idle
     uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
     // predicted traceback work per genome chunk (cost model); null: static
     // genome order and rounds of equal tile counts

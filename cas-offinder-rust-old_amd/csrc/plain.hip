@@ -402,7 +402,7 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
 Workspace::~Workspace() {
     for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
-                    qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps,
+                    qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
                     (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
                     (void*)round_ctr})
         if (p) (void)hipFree(p);
@@ -424,7 +424,7 @@ void Workspace::ensure_events() {
     if (ev[0]) return;
     for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipMalloc(&counters, 4 * sizeof(uint32_t)));
-    HIPCHECK(hipMalloc(&sums, 4 * sizeof(unsigned long long)));
+    HIPCHECK(hipMalloc(&sums, 8 * sizeof(unsigned long long)));
     HIPCHECK(hipMalloc(&plain_count, sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&round_ctr, 2 * sizeof(uint32_t)));
     int dev = 0;

@@ -121,10 +121,11 @@ typedef struct {
     uint64_t bulge_checksum;   /* sum over final bulge hits of a 64-bit hash of
                                   (start nibble, pattern, mm, db, rb, ops) */
     uint64_t verify_steps;     /* diagnostics: DFS transitions of the traceback kernel */
-    uint64_t verify_slots;     /* diagnostics: lane slots its DFS loop ran (steps / slots = lane use) */
+    uint64_t verify_slots;     /* diagnostics: lane slots of its DFS loop (64 per wave iteration) */
     uint64_t n_pipeline_rounds;/* bulge rounds run by the pipeline */
     double bulge_ms;           /* wall time of the bulge path (filter + traceback, all rounds) */
     uint64_t n_redo_rounds;    /* bulge rounds whose queue overflowed and were redone in halves */
+    uint64_t verify_busy;      /* diagnostics: slots whose lane held a candidate (busy / slots = lane use) */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes
@@ -143,6 +144,8 @@ typedef struct {
     uint32_t bulge_queue_cap;      /* candidate queue capacity per round (default 2^27 = 4 GiB of
                                       32-byte records, min 64); a round that overflows it is redone
                                       in halves */
+    uint32_t verify_refill;        /* a traceback wave takes new candidates once this many of its
+                                      64 lanes are idle (1-64, default 8) */
 } cof_tuning;
 
 /* Per filter tile of the last bulge search (tiles in genome order). */
