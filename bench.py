@@ -364,13 +364,16 @@ def main():
         log(f"warmup {w + 1}/{args.warmup} done")
 
     kernel_ms = [0.0] * len(runs)
+    launches = [0] * len(runs)
     step_s = []
     hits = [0] * len(runs)
     for k in range(args.steps):
         t0 = time.perf_counter()
         for i, s in enumerate(ctxs):
             hits[i] = s.search()
-            kernel_ms[i] += s.stats().kernel_ms_plain
+            st = s.stats()
+            kernel_ms[i] += st.kernel_ms_plain
+            launches[i] = st.n_plain_launches
         step_s.append(time.perf_counter() - t0)
         log(f"step {k + 1}/{args.steps}: {step_s[-1] * 1e3:.1f} ms")
 
@@ -381,6 +384,17 @@ def main():
     kms = [km / args.steps for km in kernel_ms]
     alg_ops = sum(ops_per_guide_bp(r["L"]) * len(r["pats"]) * genome_nucl for r in runs)
     achieved = alg_ops / (sum(kms) / 1e3)
+    # HBM roofline of the plain-scan kernel: algorithmic bytes of one launch =
+    # the packed genome read once (0.5 B per position for all guides of the
+    # launch) + 16 B per hit record; divided by the mean launch duration
+    n_launch = sum(launches)
+    alg_bytes = sum(launches[i] * genome_nucl / 2 + 16 * hits[i] for i in range(len(runs))) / n_launch
+    launch_s = sum(kms) / 1e3 / n_launch
+    hbm_achieved = alg_bytes / launch_s
+    # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+    # this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a 256 MiB
+    # device copy reads on the same counters); only valid at the default size
+    traffic = 1.5750e9 + 2.37e7 if abs(genome_nucl - 3.1e9) < 1e6 else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
                for i, r in enumerate(runs)]
@@ -406,14 +420,20 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 (bit4 AND+popcount)",
         "data": "synthetic",
-        "config": {"workload": "plain GPU mismatch scan: 3.1 Gbp synthetic genome (24 chroms, 41% GC, 5% N), "
-                               "100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both strands, "
-                               "1000 planted near-hits per guide; 12 reference-format runs per step",
+        "config": {"workload": f"plain GPU mismatch scan: {genome_nucl / 1e9:.2f} Gbp synthetic genome (24 chroms, "
+                               "41% GC, 5% N), 100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both "
+                               "strands, 1000 planted near-hits per guide; 12 reference-format runs per step",
                    "genome_bp": genome_nucl, "guides_per_run": 100, "runs": len(runs)},
-        "roofline": {"bound": "int32-valu", "achieved": round(achieved / 1e12, 3), "peak": round(INT32_PEAK / 1e12, 1),
-                     "unit": "TOP/s", "frac": round(achieved / INT32_PEAK, 3), "traffic": None,
-                     "note": "achieved = brute-force int32 ops 2*(3*ceil(L/8)+1) per guide-bp / plain-scan kernel "
-                             "time; the kernel tests the PAM first and so executes fewer ops"},
+        "roofline": {"bound": "hbm", "achieved": round(hbm_achieved / 1e9, 1), "peak": round(HBM_PEAK / 1e9),
+                     "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "kernel": "plain_scan_kernel", "launch_ms": round(launch_s * 1e3, 3),
+                     "bytes_per_launch": round(alg_bytes),
+                     "note": "the scan is integer-ALU bound, not HBM bound: see roofline_int32"},
+        "roofline_int32": {"bound": "int32-valu", "achieved": round(achieved / 1e12, 3),
+                           "peak": round(INT32_PEAK / 1e12, 1), "unit": "TOP/s",
+                           "frac": round(achieved / INT32_PEAK, 3),
+                           "note": "achieved = brute-force int32 ops 2*(3*ceil(L/8)+1) per guide-bp / plain-scan "
+                                   "kernel time; the kernel tests the PAM first and so executes fewer ops"},
         "cpu_baseline": baseline,
         "kernel_guide_gbp_s": round(units / (sum(kms) / 1e3) / 1e9, 1),
         "per_run": per_run,

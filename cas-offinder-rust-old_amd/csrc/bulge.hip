@@ -397,10 +397,12 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     }
     if (blockIdx.x * 64u >= n_queue) return;
     // depth d <= L + D = window length <= bits of WORD.  Ops are stored for
-    // depths 0..W-1 (2 bits each); child masks for depths 0..W (4 bits each,
-    // the leaf at depth W included).
+    // depths 0..W-1 (2 bits each).  The unexplored children of the node at
+    // each depth are three bit planes (bit d of cm0/cm1/cm2: diagonal, RNA,
+    // DNA), so no access indexes a register array.  Depth 64 (only with
+    // W = 64) is always a leaf (L + D ops consume the whole pattern), whose
+    // child mask is 0 and needs no storage.
     constexpr int kOpsW = sizeof(WORD) / 4;        // u64 words of ops
-    constexpr int kTrsW = sizeof(WORD) / 2 + 1;    // u64 words of child masks
     extern __shared__ unsigned char verify_lds[];
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
     WORD* cvn = cvp + (size_t)n_cols * 64;
@@ -417,11 +419,10 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     uint32_t i = 0, jj = 0, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
     uint32_t d_floor = 0;   // depth the lane's subtree starts at (> 0 for donated work)
     __shared__ uint32_t pairing[64];
-    uint64_t ops[kOpsW], trs[kTrsW];
+    uint64_t ops[kOpsW];
+    uint64_t cm0 = 0, cm1 = 0, cm2 = 0;
 #pragma unroll
     for (int k = 0; k < kOpsW; k++) ops[k] = 0;
-#pragma unroll
-    for (int k = 0; k < kTrsW; k++) trs[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0, my_busy = 0;
     uint32_t cand_steps = 0, cand_idx = 0, cand_chunk = 0;
     bool drained = false;
@@ -441,17 +442,15 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         }
     };
     auto get4 = [&](uint32_t dd) -> uint32_t {
-        uint32_t w = dd >> 4;
-        uint64_t v = trs[0];
-#pragma unroll
-        for (int k = 1; k < kTrsW; k++) v = w == (uint32_t)k ? trs[k] : v;
-        return (uint32_t)(v >> (4 * (dd & 15))) & 15u;
+        if (dd >= 64) return 0u;
+        return ((uint32_t)(cm0 >> dd) & 1u) | (((uint32_t)(cm1 >> dd) & 1u) << 1) | (((uint32_t)(cm2 >> dd) & 1u) << 2);
     };
     auto set4 = [&](uint32_t dd, uint32_t val) {
-        uint32_t w = dd >> 4;
-        uint64_t m = ~(15ull << (4 * (dd & 15))), b = (uint64_t)val << (4 * (dd & 15));
-#pragma unroll
-        for (int k = 0; k < kTrsW; k++) trs[k] = w == (uint32_t)k ? ((trs[k] & m) | b) : trs[k];
+        if (dd >= 64) return;
+        const uint64_t m = 1ull << dd;
+        cm0 = (val & 1u) ? (cm0 | m) : (cm0 & ~m);
+        cm1 = (val & 2u) ? (cm1 | m) : (cm1 & ~m);
+        cm2 = (val & 4u) ? (cm2 | m) : (cm2 & ~m);
     };
     auto text_at = [&](uint32_t a) -> uint32_t {
         uint32_t t = bit(tT, a) | bit(tC, a) << 1 | bit(tA, a) << 2 | bit(tG, a) << 3;
@@ -610,16 +609,11 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             if ((uint32_t)__popcll(idle) >= sink.refill) {
                 uint32_t kd = 0xFFFFFFFFu;
                 if (busy) {
-#pragma unroll
-                    for (int w = kTrsW - 1; w >= 0; w--) {
-                        const uint32_t lo = (uint32_t)w * 16u;
-                        uint64_t v = trs[w];
-                        // depths [max(lo, d_floor), min(lo + 16, d)) of this word
-                        if (d <= lo) v = 0;
-                        else if (d < lo + 16) v &= (1ull << (4 * (d - lo))) - 1ull;
-                        if (d_floor > lo) v = d_floor >= lo + 16 ? 0ull : v & ~((1ull << (4 * (d_floor - lo))) - 1ull);
-                        if (v) kd = lo + (uint32_t)__builtin_ctzll(v) / 4u;
-                    }
+                    // pending siblings at depths [d_floor, d)
+                    uint64_t v = cm0 | cm1 | cm2;
+                    if (d < 64) v &= (1ull << d) - 1ull;
+                    v &= ~((1ull << d_floor) - 1ull);
+                    if (v) kd = (uint32_t)__builtin_ctzll(v);
                 }
                 const bool can = kd != 0xFFFFFFFFu;
                 const uint64_t donors = __ballot(can);
@@ -707,8 +701,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         cand_chunk = r_chunk;
 #pragma unroll
                         for (int w = 0; w < kOpsW; w++) ops[w] = r_ops[w];
-#pragma unroll
-                        for (int w = 0; w < kTrsW; w++) trs[w] = 0;
+                        cm0 = cm1 = cm2 = 0;
                         set4(d, r_mask);
                         busy = true;
                     }
