@@ -309,9 +309,11 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
                 st.bulge_hits = ctx->hits.size() - first_bulge;
                 for (size_t h = first_bulge; h < ctx->hits.size(); h++) {
                     const cof_hit& x = ctx->hits[h];
+                    uint64_t so[3];
+                    cof::ops_reverse(x.ops, x.n_ops, so);
                     st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, cof::pattern_id_of(ctx->q.pats[x.pattern]),
-                                                             x.mismatches, x.dna_bulge,
-                                                             x.rna_bulge, x.n_ops, x.ops[0], x.ops[1], x.ops[2]);
+                                                             x.mismatches, x.dna_bulge, x.rna_bulge, x.n_ops, so[0],
+                                                             so[1]);
                 }
             }
             st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -371,16 +371,6 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v
     if (i < n) p[i] = v;
 }
 
-// Reverses the order of the 2-bit ops held in NW words (op 0 lowest), so a
-// stack pushed from the alignment's end reads in alignment order.
-__device__ __forceinline__ uint64_t rev2(uint64_t x) {
-    x = ((x >> 2) & 0x3333333333333333ull) | ((x & 0x3333333333333333ull) << 2);
-    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
-    x = ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x & 0x00FF00FF00FF00FFull) << 8);
-    x = ((x >> 16) & 0x0000FFFF0000FFFFull) | ((x & 0x0000FFFF0000FFFFull) << 16);
-    return (x >> 32) | (x << 32);
-}
-
 template <typename WORD>
 __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __restrict__ genome,
                                                           const BCand* __restrict__ queue,
@@ -425,6 +415,9 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     for (int k = 0; k < kOpsW; k++) ops[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0, my_busy = 0;
     uint32_t cand_steps = 0, cand_idx = 0, cand_chunk = 0;
+    uint64_t c0_begin = 0;
+    uint32_t c0_size = 0;
+    bool c0_endc = false;
     bool drained = false;
 
     auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
@@ -490,20 +483,14 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             v |= 4u;
         return v;
     };
-    // leaf (i == 0): emit the alignment
+    // leaf (i == 0): emit the alignment (ops stay in stack order; the host
+    // reverses recorded ones)
     auto emit = [&]() {
-        uint64_t o0, o1 = 0;
-        if constexpr (kOpsW == 1) {
-            o0 = rev2(ops[0]) >> (2 * (32 - d));
-        } else {
-            // 128-bit reverse, then shift right by 2 * (64 - d)
-            uint64_t h = rev2(ops[0]), l = rev2(ops[1]);
-            uint32_t sh = 2 * (64 - d);
-            if (sh >= 64) { o0 = h >> (sh - 64); o1 = 0; }
-            else if (sh == 0) { o0 = l; o1 = h; }
-            else { o0 = (l >> sh) | (h << (64 - sh)); o1 = h >> sh; }
-        }
-        uint64_t st = ts + jj;
+        // ops of depths [0, d) only: deeper slots hold stale ops
+        const uint64_t o0 = d >= 32 ? ops[0] : ops[0] & ((1ull << (2 * d)) - 1ull);
+        uint64_t o1 = 0;
+        if constexpr (kOpsW > 1) o1 = d <= 32 ? 0ull : d >= 64 ? ops[1] : ops[1] & ((1ull << (2 * (d - 32))) - 1ull);
+        const uint64_t st = ts + jj;
         if (!sink.count_only) {
             BulgeRaw r;
             r.start = st; r.pattern = pid;
@@ -520,17 +507,31 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             if (idx < sink.cap) sink.out[idx] = r;
         } else {
             // convert_matches_myers' drop rule (search.rs:1338-1350) on the
-            // chunk of the alignment start: the candidate's first chunk or a
-            // later one
-            uint32_t c = cand_chunk;
-            while (c + 1 < sink.n_chunks && sink.chunks[c + 1].nib_begin <= st) c++;
-            bool end_chrom = c + 1 == last_chunk || sink.chunks[c + 1].chrom_start;
-            bool past_end = st - sink.chunks[c].nib_begin + (L + db - rb) > sink.chunks[c].size;
+            // chunk of the alignment start: the candidate's first chunk
+            // (fields cached at refill) or, past its end, a later one
+            const uint32_t ext = L + db - rb;
+            bool end_chrom, past_end;
+            if (st < c0_begin + ((c0_size + 1u) & ~1u)) {
+                end_chrom = c0_endc;
+                past_end = st - c0_begin + ext > c0_size;
+            } else {
+                uint32_t c = cand_chunk;
+                while (c + 1 < sink.n_chunks && sink.chunks[c + 1].nib_begin <= st) c++;
+                end_chrom = c + 1 == last_chunk || sink.chunks[c + 1].chrom_start;
+                past_end = st - sink.chunks[c].nib_begin + ext > sink.chunks[c].size;
+            }
             if (!(end_chrom && past_end)) {
                 my_kept++;
-                my_sum += bulge_hit_hash(st, hash_id, mm, db, rb, d, o0, o1, 0);
+                my_sum += bulge_hit_hash(st, hash_id, mm, db, rb, d, o0, o1);
             }
         }
+    };
+    // chunk fields of the candidate's first chunk for the drop rule
+    auto load_c0 = [&]() {
+        const DChunk C0 = sink.chunks[cand_chunk];
+        c0_begin = C0.nib_begin;
+        c0_size = C0.size;
+        c0_endc = cand_chunk + 1 == last_chunk || sink.chunks[cand_chunk + 1].chrom_start;
     };
 
     for (;;) {
@@ -561,6 +562,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         while (c > 0 && sink.chunks[c].nib_begin > t0) c--;
                         while (c + 1 < sink.n_chunks && sink.chunks[c + 1].nib_begin <= t0) c++;
                         cand_chunk = c;
+                        load_c0();
                     }
                     uint32_t W = L + D;
                     ts = C.j + 1 >= C.floor + W ? C.j + 1 - W : C.floor;
@@ -699,6 +701,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         pn = r_pn;
                         L = r_L; M = r_M; D = r_D; R = r_R; E = r_E; pid = r_pid; last_chunk = r_last;
                         cand_chunk = r_chunk;
+                        if (sink.count_only) load_c0();
 #pragma unroll
                         for (int w = 0; w < kOpsW; w++) ops[w] = r_ops[w];
                         cm0 = cm1 = cm2 = 0;

@@ -113,8 +113,19 @@ struct BulgeRaw {
     uint64_t start;
     uint32_t pattern;
     uint8_t mm, db, rb, n_ops;
-    uint64_t ops[3];          // forward order, 2 bits per op
+    uint64_t ops[3];          // DFS stack order: bits 2k of op k counted from the alignment's end
 };
+
+// Stack order <-> alignment order of n 2-bit ops (an involution).
+inline void ops_reverse(const uint64_t* in, uint32_t n, uint64_t* out) {
+    uint64_t o[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < n; k++) {
+        uint64_t op = (in[k / 32] >> (2 * (k % 32))) & 3u;
+        uint32_t r = n - 1 - k;
+        o[r / 32] |= op << (2 * (r % 32));
+    }
+    out[0] = o[0]; out[1] = o[1]; out[2] = o[2];
+}
 
 #if defined(__HIPCC__)
 #define COF_HD __host__ __device__
@@ -129,11 +140,12 @@ COF_HD inline uint64_t cof_mix64(uint64_t x) {
 }
 // pattern_id: a hash of the pattern's sequence and strand (pattern_id_of), so
 // the checksum does not depend on how a query numbers its patterns
+// Hash of one bulge hit; ops in stack order (at most 64 ops: two words).
 COF_HD inline uint64_t bulge_hit_hash(uint64_t start, uint64_t pattern_id, uint32_t mm, uint32_t db, uint32_t rb,
-                                      uint32_t n_ops, uint64_t ops0, uint64_t ops1, uint64_t ops2) {
-    uint64_t h = cof_mix64(start * 0x9E3779B97F4A7C15ull ^ pattern_id ^ ((uint64_t)mm << 24) ^
-                           ((uint64_t)db << 16) ^ ((uint64_t)rb << 8) ^ n_ops);
-    return cof_mix64(h ^ ops0) + cof_mix64(h ^ ops1 ^ 0x1234567ull) + cof_mix64(h ^ ops2 ^ 0x89ABCDEFull);
+                                      uint32_t n_ops, uint64_t ops0, uint64_t ops1) {
+    const uint64_t meta = ((uint64_t)mm << 24) | ((uint64_t)db << 16) | ((uint64_t)rb << 8) | n_ops;
+    return cof_mix64((start * 0x9E3779B97F4A7C15ull) ^ pattern_id ^ (meta << 40) ^ (ops0 * 0xD6E8FEB86659FD93ull) ^
+                     (ops1 * 0xA0761D6478BD642Full));
 }
 
 inline uint64_t pattern_id_of(const Pattern& p) {

@@ -157,7 +157,7 @@ void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs
         h.dna_bulge = r.db;
         h.rna_bulge = r.rb;
         h.n_ops = r.n_ops;
-        h.ops[0] = r.ops[0]; h.ops[1] = r.ops[1]; h.ops[2] = r.ops[2];
+        ops_reverse(r.ops, r.n_ops, h.ops);      // device stack order -> alignment order
         out.push_back(h);
         aux.push_back({r.start, bi.end_nib[b]});
     }
