@@ -364,6 +364,8 @@ def main():
         log(f"warmup {w + 1}/{args.warmup} done")
 
     kernel_ms = [0.0] * len(runs)
+    search_ms = [0.0] * len(runs)
+    convert_ms = [0.0] * len(runs)
     launches = [0] * len(runs)
     step_s = []
     hits = [0] * len(runs)
@@ -373,6 +375,8 @@ def main():
             hits[i] = s.search()
             st = s.stats()
             kernel_ms[i] += st.kernel_ms_plain
+            search_ms[i] += st.search_ms
+            convert_ms[i] += st.convert_ms
             launches[i] = st.n_plain_launches
         step_s.append(time.perf_counter() - t0)
         log(f"step {k + 1}/{args.steps}: {step_s[-1] * 1e3:.1f} ms")
@@ -396,6 +400,7 @@ def main():
     # device copy reads on the same counters); only valid at the default size
     traffic = 1.5750e9 + 2.37e7 if abs(genome_nucl - 3.1e9) < 1e6 else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
+                "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
                for i, r in enumerate(runs)]
     for s in ctxs:

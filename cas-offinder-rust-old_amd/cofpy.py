@@ -71,7 +71,8 @@ class CofStats(C.Structure):
                 ("n_raw_hits", C.c_uint64), ("n_hits", C.c_uint64), ("genome_nucl", C.c_uint64),
                 ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
                 ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
-                ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64)]
+                ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64),
+                ("convert_ms", C.c_double)]
 
 
 _lib = None

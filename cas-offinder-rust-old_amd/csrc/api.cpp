@@ -35,8 +35,8 @@ struct cof_ctx {
     cof::BulgeTune bulge_tune;
     bool have_genome = false;
     bool cached = false;
-    std::vector<cof_hit> hits;
-    std::vector<cof::HitAux> aux;
+    cof::HitVec hits;
+    cof::AuxVec aux;
     cof_stats stats{};
     std::string err;
 };
@@ -138,7 +138,7 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
 void autotune(cof_ctx* ctx, uint64_t sample) {
     const uint64_t n = std::min<uint64_t>(sample, ctx->g.n_nibbles());
     if (!cof::plain_plan_empty(ctx->plain)) {
-        std::vector<cof::PlainRaw> raw;
+        cof::RawView<cof::PlainRaw> raw;
         double best = 1e300;
         cof::PlainTune best_t = ctx->plain_tune;
         for (uint32_t th : {128u, 256u, 512u}) {
@@ -161,7 +161,7 @@ void autotune(cof_ctx* ctx, uint64_t sample) {
         size_t nc = 0;
         while (nc < ctx->g.chunks.size() && ctx->g.chunks[nc].nib_begin() < n) nc++;
         std::vector<cof::Batch> sb = cof::make_batches(std::max<size_t>(nc, 1), ctx->q.batch_nucl);
-        std::vector<cof::BulgeRaw> raw;
+        cof::RawView<cof::BulgeRaw> raw;
         double best = 1e300;
         cof::BulgeTune best_t = ctx->bulge_tune;
         for (uint32_t v : {4u, 8u, 12u, 16u, 24u}) {
@@ -290,10 +290,10 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             st.genome_nucl = genome_nucl;
             ctx->hits.clear();
             ctx->aux.clear();
-            std::vector<cof::PlainRaw> praw;
+            cof::RawView<cof::PlainRaw> praw;
             cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw,
                            &st.kernel_ms_plain, &st.n_plain_launches, ctx->plain_tune);
-            std::vector<cof::BulgeRaw> braw;
+            cof::RawView<cof::BulgeRaw> braw;
             cof::BulgeTune bt = ctx->bulge_tune;
             bt.flags = ctx->q.flags;
             if (!ctx->model.empty() && ctx->bulge && cof::bulge_plan_size(ctx->bulge)) {
@@ -302,6 +302,7 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             }
             cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
             st.n_raw_hits = praw.size() + braw.size();
+            auto tc = std::chrono::steady_clock::now();
             cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
             size_t first_bulge = ctx->hits.size();
             cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
@@ -316,7 +317,9 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
                                                              so[1]);
                 }
             }
-            st.search_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            const auto t_end = std::chrono::steady_clock::now();
+            st.convert_ms = std::chrono::duration<double, std::milli>(t_end - tc).count();
+            st.search_ms = std::chrono::duration<double, std::milli>(t_end - t0).count();
             st.n_hits = ctx->hits.size();
             ctx->cached = true;
         }

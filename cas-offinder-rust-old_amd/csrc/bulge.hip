@@ -932,10 +932,10 @@ void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uin
 // synchronously in halves (the reference bisects the same way,
 // search.rs:1140-1152).  `order` (optional) is the tile processing order.
 void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats,
+               const uint32_t* d_genome, hipStream_t st, RawView<BulgeRaw>& out, cof_stats* stats,
                const BulgeTune& tune) {
     const uint32_t flags = tune.flags;
-    out.clear();
+    out = RawView<BulgeRaw>{};
     ws.tile_j0.clear();
     ws.tile_n.clear();
     ws.tile_chunk.clear();
@@ -1275,8 +1275,9 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
             bulge_run(bp, ws, g, batches, d_genome, st, out, stats, tune);
             return;
         }
-        out.resize(no);
-        if (no) HIPCHECK(hipMemcpy(out.data(), ws.bulge_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost));
+        ws.ensure_host(&ws.h_bulge, &ws.h_bulge_cap, (size_t)no * sizeof(BulgeRaw));
+        if (no) HIPCHECK(hipMemcpy(ws.h_bulge, ws.bulge_out, (size_t)no * sizeof(BulgeRaw), hipMemcpyDeviceToHost));
+        out = RawView<BulgeRaw>{(const BulgeRaw*)ws.h_bulge, no};
     }
     if (stats) {
         unsigned long long sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};

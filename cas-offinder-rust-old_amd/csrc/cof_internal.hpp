@@ -6,6 +6,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "cof.h"
@@ -163,11 +165,36 @@ struct HitAux {
     uint64_t nib;
     uint64_t view_end;
 };
+// Vector that leaves new elements uninitialised (hit buffers are filled by
+// the converters; zero-filling tens of MB per search would only cost time).
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <typename U>
+    void construct(U* p) { ::new ((void*)p) U; }
+    template <typename U, typename... A>
+    void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+using HitVec = std::vector<cof_hit, NoInitAlloc<cof_hit>>;
+using AuxVec = std::vector<HitAux, NoInitAlloc<HitAux>>;
+
+// Raw hits in (pinned) host memory, as copied from the device.
+template <typename T>
+struct RawView {
+    const T* p = nullptr;
+    size_t n = 0;
+    size_t size() const { return n; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
 void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& batches,
-                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux);
+                   RawView<PlainRaw> raw, HitVec& out, AuxVec& aux);
 void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches,
-                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out, std::vector<HitAux>& aux);
-void write_tsv(const Genome& g, const Query& q, const std::vector<cof_hit>& hits,
-               const std::vector<HitAux>& aux, FILE* f);
+                   RawView<BulgeRaw> raw, HitVec& out, AuxVec& aux);
+void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux, FILE* f);
 
 }  // namespace cof

@@ -56,6 +56,12 @@ struct Workspace {
     ~Workspace();
     void ensure_events();
     void ensure_round_events(uint32_t rounds);
+    // pinned host buffers for raw hits (grown, kept across searches)
+    void* h_plain = nullptr;
+    size_t h_plain_cap = 0;
+    void* h_bulge = nullptr;
+    size_t h_bulge_cap = 0;
+    void ensure_host(void** p, size_t* cap, size_t bytes);
 };
 // grows *ptr to hold `need` elements of `elem` bytes (contents not kept)
 void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem);
@@ -86,7 +92,7 @@ bool plain_plan_empty(const PlainPlan* pp);
 uint64_t plain_tile_nucl();
 uint64_t plain_pad_words();
 void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
-               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
+               RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
 
 struct BulgePlan;
 BulgePlan* bulge_plan_create(const Query& q);
@@ -96,7 +102,7 @@ void bulge_plan_destroy(BulgePlan* bp);
 uint64_t bulge_pad_words();
 size_t bulge_plan_size(const BulgePlan* bp);   // patterns of the plan
 void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-               const uint32_t* d_genome, hipStream_t st, std::vector<BulgeRaw>& out, cof_stats* stats,
+               const uint32_t* d_genome, hipStream_t st, RawView<BulgeRaw>& out, cof_stats* stats,
                const BulgeTune& tune);
 
 }  // namespace cof

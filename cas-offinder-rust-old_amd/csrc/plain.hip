@@ -410,6 +410,17 @@ Workspace::~Workspace() {
     for (auto& e : rev) if (e) (void)hipEventDestroy(e);
     if (st2) (void)hipStreamDestroy(st2);
     if (st3) (void)hipStreamDestroy(st3);
+    if (h_plain) (void)hipHostFree(h_plain);
+    if (h_bulge) (void)hipHostFree(h_bulge);
+}
+
+void Workspace::ensure_host(void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap && *p) return;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    size_t want = std::max<size_t>(bytes + bytes / 4, 1u << 20);
+    HIPCHECK(hipHostMalloc(p, want, hipHostMallocDefault));
+    *cap = want;
 }
 
 void Workspace::ensure_round_events(uint32_t r) {
@@ -441,8 +452,8 @@ void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem) {
 }
 
 void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
-               std::vector<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
-    out.clear();
+               RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
+    out = RawView<PlainRaw>{};
     if (plain_plan_empty(pp) || scan_end == 0) return;
     ws.ensure_events();
     if (!ws.plain_out) {
@@ -494,9 +505,10 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
             ws.plain_cap = (uint32_t)cap;
             continue;
         }
-        out.resize(cnt);
-        if (cnt) HIPCHECK(hipMemcpyAsync(out.data(), d_out, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
+        ws.ensure_host(&ws.h_plain, &ws.h_plain_cap, (size_t)cnt * sizeof(PlainRaw));
+        if (cnt) HIPCHECK(hipMemcpyAsync(ws.h_plain, d_out, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
+        out = RawView<PlainRaw>{(const PlainRaw*)ws.h_plain, cnt};
         return;
     }
 }

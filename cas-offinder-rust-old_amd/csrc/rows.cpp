@@ -12,7 +12,10 @@
 // PAM limit); here the chromosome-end drop rule of convert_matches_myers
 // (search.rs:1316-1374) is applied and rows are formatted (main.rs:145-157).
 #include <algorithm>
+#include <condition_variable>
 #include <cstdio>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 #include "cof_internal.hpp"
@@ -61,29 +64,129 @@ bool plain_predicate(const Genome& g, const Pattern& p, uint64_t pos, uint64_t v
 }  // namespace
 
 namespace {
-// Runs fn(begin, end, out, aux) over up to 16 slices of [0, n) in parallel and
-// concatenates the slices' outputs in order.
-template <typename Fn>
-void parallel_convert(size_t n, std::vector<cof_hit>& out, std::vector<HitAux>& aux, Fn fn) {
-    unsigned nt = n < 65536 ? 1u : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-    std::vector<std::vector<cof_hit>> po(nt);
-    std::vector<std::vector<HitAux>> pa(nt);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; t++)
-        th.emplace_back([&, t]() { fn(n * t / nt, n * (t + 1) / nt, po[t], pa[t]); });
-    for (auto& x : th) x.join();
-    for (unsigned t = 0; t < nt; t++) {
-        out.insert(out.end(), po[t].begin(), po[t].end());
-        aux.insert(aux.end(), pa[t].begin(), pa[t].end());
+
+// Persistent worker pool for the converters (creating 16 threads per search
+// costs about a millisecond).
+class Pool {
+public:
+    static Pool& get() {
+        static Pool pool;
+        return pool;
     }
+    unsigned size() const { return (unsigned)workers_.size() + 1; }
+    // runs f(t) for t in [0, n) on the workers and the calling thread
+    void run(unsigned n, const std::function<void(unsigned)>& f) {
+        std::unique_lock<std::mutex> lk(m_);
+        job_ = &f;
+        n_tasks_ = n;
+        next_ = 0;
+        done_ = 0;
+        gen_++;
+        cv_.notify_all();
+        lk.unlock();
+        work();
+        lk.lock();
+        done_cv_.wait(lk, [&] { return done_ == n_tasks_; });
+        job_ = nullptr;
+    }
+
+private:
+    Pool() {
+        unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        unsigned nw = std::min(16u, hw) - 1;
+        for (unsigned w = 0; w < nw; w++)
+            workers_.emplace_back([this] {
+                uint64_t seen = 0;
+                for (;;) {
+                    std::unique_lock<std::mutex> lk(m_);
+                    cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                    if (stop_) return;
+                    seen = gen_;
+                    lk.unlock();
+                    work();
+                }
+            });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    void work() {
+        for (;;) {
+            unsigned t;
+            const std::function<void(unsigned)>* f;
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                if (!job_ || next_ >= n_tasks_) return;
+                t = next_++;
+                f = job_;
+            }
+            (*f)(t);
+            std::lock_guard<std::mutex> lk(m_);
+            if (++done_ == n_tasks_) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(unsigned)>* job_ = nullptr;
+    unsigned n_tasks_ = 0, next_ = 0, done_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Runs fn(begin, end, out, aux) -> count over slices of [0, n) in parallel:
+// slice t writes its hits at out + base + begin (at most end - begin of
+// them), then the slices are moved together, in order, through a scratch copy.
+template <typename Fn>
+void parallel_convert(size_t n, HitVec& out, AuxVec& aux, Fn fn) {
+    if (!n) return;
+    const size_t base = out.size();
+    Pool& pool = Pool::get();
+    const unsigned nt = n < 4096 ? 1u : pool.size();
+    static thread_local HitVec s_out;
+    static thread_local AuxVec s_aux;
+    s_out.resize(n);
+    s_aux.resize(n);
+    // the calling thread's scratch (a thread_local named inside the pool's
+    // tasks would be the worker's own)
+    cof_hit* const so = s_out.data();
+    HitAux* const sa = s_aux.data();
+    std::vector<size_t> cnt(nt), off(nt + 1, 0);
+    auto slice = [&](unsigned t) { return std::make_pair(n * t / nt, n * (t + 1) / nt); };
+    if (nt == 1) {
+        cnt[0] = fn(0, n, so, sa);
+    } else {
+        pool.run(nt, [&](unsigned t) {
+            auto [b0, b1] = slice(t);
+            cnt[t] = fn(b0, b1, so + b0, sa + b0);
+        });
+    }
+    for (unsigned t = 0; t < nt; t++) off[t + 1] = off[t] + cnt[t];
+    out.resize(base + off[nt]);
+    aux.resize(base + off[nt]);
+    auto move = [&](unsigned t) {
+        auto [b0, b1] = slice(t);
+        (void)b1;
+        if (cnt[t]) {
+            std::memcpy(out.data() + base + off[t], so + b0, cnt[t] * sizeof(cof_hit));
+            std::memcpy(aux.data() + base + off[t], sa + b0, cnt[t] * sizeof(HitAux));
+        }
+    };
+    if (nt == 1) move(0);
+    else pool.run(nt, move);
 }
 }  // namespace
 
-void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs,
-                   const std::vector<PlainRaw>& raw, std::vector<cof_hit>& out_all, std::vector<HitAux>& aux_all) {
+void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<PlainRaw> raw,
+                   HitVec& out_all, AuxVec& aux_all) {
     BatchIndex bi(g, bs);
-    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, std::vector<cof_hit>& out,
-                                                       std::vector<HitAux>& aux) {
+    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, cof_hit* out, HitAux* aux) -> size_t {
+    size_t k = 0;
     for (size_t ri = b0; ri < b1; ri++) {
         const PlainRaw& r = raw[ri];
         const Pattern& p = q.pats[r.pattern];
@@ -122,17 +225,19 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
         h.chrom = ch.chrom;
         h.pattern = r.pattern;
         h.mismatches = (uint16_t)mm;
-        out.push_back(h);
-        aux.push_back({r.pos, view_end});
+        out[k] = h;
+        aux[k] = HitAux{r.pos, view_end};
+        k++;
     }
+    return k;
     });
 }
 
-void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs,
-                   const std::vector<BulgeRaw>& raw, std::vector<cof_hit>& out_all, std::vector<HitAux>& aux_all) {
+void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<BulgeRaw> raw,
+                   HitVec& out_all, AuxVec& aux_all) {
     BatchIndex bi(g, bs);
-    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, std::vector<cof_hit>& out,
-                                                       std::vector<HitAux>& aux) {
+    parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, cof_hit* out, HitAux* aux) -> size_t {
+    size_t k = 0;
     for (size_t ri = b0; ri < b1; ri++) {
         const BulgeRaw& r = raw[ri];
         const Pattern& p = q.pats[r.pattern];
@@ -158,15 +263,16 @@ void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs
         h.rna_bulge = r.rb;
         h.n_ops = r.n_ops;
         ops_reverse(r.ops, r.n_ops, h.ops);      // device stack order -> alignment order
-        out.push_back(h);
-        aux.push_back({r.start, bi.end_nib[b]});
+        out[k] = h;
+        aux[k] = HitAux{r.start, bi.end_nib[b]};
+        k++;
     }
+    return k;
     });
 }
 
 // Rows of main.rs:130-247.
-void write_tsv(const Genome& g, const Query& q, const std::vector<cof_hit>& hits,
-               const std::vector<HitAux>& aux, FILE* f) {
+void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux, FILE* f) {
     fputs("#Bulge type\tcrRNA\tDNA\tChromosome\tPosition\tDirection\tMismatches\tBulge Size\n", f);
     std::string line, rna, dna;
     for (size_t hi = 0; hi < hits.size(); hi++) {

@@ -126,6 +126,7 @@ typedef struct {
     double bulge_ms;           /* wall time of the bulge path (filter + traceback, all rounds) */
     uint64_t n_redo_rounds;    /* bulge rounds whose queue overflowed and were redone in halves */
     uint64_t verify_busy;      /* diagnostics: slots whose lane held a candidate (busy / slots = lane use) */
+    double convert_ms;         /* host conversion of raw device hits into final hits */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes
