@@ -1,0 +1,31 @@
+"""libcof on the GPU reproduces the committed golden vectors (oracle rows for
+seeded synthetic genomes: plain and bulge queries, 3'/5' PAMs, IUPAC letters,
+odd line widths, small virtual batches, 32- and 64-bit bulge windows)."""
+import gzip
+import os
+import sys
+
+import pytest
+
+import cofpy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+sys.path.insert(0, GOLDEN)
+
+import make_golden  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", make_golden.CASES, ids=[c["name"] for c in make_golden.CASES])
+def test_gpu_matches_golden(tmp_path, case):
+    want = gzip.decompress(open(os.path.join(GOLDEN, case["name"] + ".tsv.gz"), "rb").read()).decode().splitlines()
+    n = len(case["patterns"])
+    s = cofpy.Search(case["patterns"], [case["filter"]] * n, [case["M"]] * n, [case["D"]] * n, [case["R"]] * n,
+                     batch_nucl=case["batch_nucl"])
+    s.load_synth(make_golden.make_genome(case["genome"]))
+    s.search()
+    s.write_tsv(tmp_path / "gpu.tsv")
+    s.close()
+    assert cofpy.read_rows(tmp_path / "gpu.tsv") == want
