@@ -355,6 +355,7 @@ def main():
     for r in runs:
         s = cofpy.Search(r["pats"], [r["filter"]] * len(r["pats"]), [r["mm"]] * len(r["pats"]))
         s.load_synth(sg)
+        apply_tuning(s, args)
         ctxs.append(s)
     log(f"{len(ctxs)} contexts with the genome resident")
 

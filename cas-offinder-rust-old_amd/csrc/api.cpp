@@ -106,8 +106,8 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
     cof::PlainTune pt;
     cof::BulgeTune bt;
     if (t.plain_threads) {
-        if (t.plain_threads != 128 && t.plain_threads != 256 && t.plain_threads != 512)
-            throw cof::Error(COF_E_INVALID, "plain_threads must be 128, 256 or 512");
+        if (t.plain_threads != 64 && t.plain_threads != 128 && t.plain_threads != 256 && t.plain_threads != 512)
+            throw cof::Error(COF_E_INVALID, "plain_threads must be 64, 128, 256 or 512");
         pt.threads = t.plain_threads;
     }
     pt.pat_chunk = t.plain_pat_chunk;
@@ -141,7 +141,7 @@ void autotune(cof_ctx* ctx, uint64_t sample) {
         cof::RawView<cof::PlainRaw> raw;
         double best = 1e300;
         cof::PlainTune best_t = ctx->plain_tune;
-        for (uint32_t th : {128u, 256u, 512u}) {
+        for (uint32_t th : {64u, 128u, 256u, 512u}) {
             for (uint32_t pc : {0u, 16u, 64u}) {
                 cof::PlainTune t;
                 t.threads = th;

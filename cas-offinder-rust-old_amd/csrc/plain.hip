@@ -380,6 +380,10 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
                   uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const uint32_t* full,
                   PlainRaw* out, uint32_t* count, uint32_t cap) {
     switch (t.threads) {
+        case 64:
+            hipLaunchKernelGGL((plain_scan_kernel<W, 64>), grid, dim3(64), 0, st, d_genome, scan_end, first, g,
+                               n_groups, pats, full, out, count, cap, t.pat_chunk);
+            break;
         case 128:
             hipLaunchKernelGGL((plain_scan_kernel<W, 128>), grid, dim3(128), 0, st, d_genome, scan_end, first, g,
                                n_groups, pats, full, out, count, cap, t.pat_chunk);
@@ -392,7 +396,7 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
             hipLaunchKernelGGL((plain_scan_kernel<W, 512>), grid, dim3(512), 0, st, d_genome, scan_end, first, g,
                                n_groups, pats, full, out, count, cap, t.pat_chunk);
             break;
-        default: throw Error(COF_E_INVALID, "plain workgroup size must be 128, 256 or 512");
+        default: throw Error(COF_E_INVALID, "plain workgroup size must be 64, 128, 256 or 512");
     }
 }
 
