@@ -68,7 +68,7 @@ void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem);
 
 // Launch shape of the plain scan (results do not depend on it).
 struct PlainTune {
-    uint32_t threads = 128;     // workgroup size; a workgroup scans 64 starts per thread
+    uint32_t threads = 64;      // workgroup size; a workgroup scans 64 starts per thread
     uint32_t pat_chunk = 0;     // patterns per workgroup (grid.y slices a group); 0 = whole group
 };
 

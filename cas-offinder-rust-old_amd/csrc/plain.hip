@@ -54,8 +54,8 @@ struct GroupDev {
 // L - M - (zeroed positions), M and the query pattern index.  64 bytes, so a
 // wave fetches a record with one scalar load.
 struct __attribute__((aligned(64))) PatRec {
+    int32_t thr;          // first, so thr + the W compare words are one scalar load
     uint32_t w[8];
-    int32_t thr;
     uint32_t M;
     uint32_t pid;
     uint32_t pad[5];
@@ -81,15 +81,24 @@ __device__ __forceinline__ uint32_t gather_nibble_flags(uint32_t f) {
 }
 
 // Exact writer recount (main.rs:184-187) of one window: pattern positions
-// that are not N and do not hold a plain base of the pattern's set.
-__device__ __forceinline__ uint32_t recount(const uint32_t* gw, uint32_t i, const uint32_t* __restrict__ fw, uint32_t L) {
+// that are not N and do not hold a plain base of the pattern's set, computed
+// 8 positions per word op: a position counts when the
+// pattern nibble is not N and the genome nibble is not a single base sharing a
+// bit with it (N is already masked to 0 in the tile).
+__device__ __forceinline__ uint32_t recount_swar(const uint32_t* gw, uint32_t i, const uint32_t* __restrict__ fw,
+                                                 uint32_t L) {
+    const uint32_t qw = i >> 3, sh = (i & 7u) * 4u;
     uint32_t amm = 0;
-    for (uint32_t k = 0; k < L; k++) {
-        uint32_t q = i + k;
-        uint32_t g = nib_of(gw[q >> 3], q & 7u);
-        uint32_t pb = nib_of(fw[k >> 3], k & 7u);
-        bool single = g != 0 && (g & (g - 1)) == 0;
-        amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
+    for (uint32_t k = 0; k * 8 < L; k++) {
+        const uint32_t g = __builtin_amdgcn_alignbit(gw[qw + k + 1], gw[qw + k], sh);
+        const uint32_t p = fw[k];
+        const uint32_t pn = (p & (p >> 1) & (p >> 2) & (p >> 3) & 0x11111111u) << 3;   // pattern N
+        const uint32_t single = nz_nibbles(g) & ~multi_nibbles(g);
+        const uint32_t ok = single & nz_nibbles(g & p);
+        uint32_t miss = ~pn & ~ok & 0x88888888u;
+        const uint32_t rem = L - k * 8;
+        if (rem < 8) miss &= (1u << (4 * rem)) - 1u;
+        amm += (uint32_t)__popc(miss);
     }
     return amm;
 }
@@ -125,9 +134,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     }
     iupac = __syncthreads_or(iupac);
 
-    // valid-start mask of this thread's 64 positions (the last tile ends at scan_end)
-    const uint64_t my0 = s0 + 64u * tid;
-    const uint64_t n_valid = scan_end > my0 ? scan_end - my0 : 0;
+    // Thread tid owns tile words w * kThreads + tid (w < 8), 8 starts each:
+    // consecutive lanes read consecutive LDS words (no bank conflicts).
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const GroupDev& G = groups[gi];
@@ -141,23 +149,29 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         // 2. PAM test, 8 positions per word: nibble j of X_k is the genome at
         //    start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble is
         //    in the filter's plain-base set iff it overlaps the filter code.
+        uint32_t ok[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
+        for (uint32_t k = 0; k < G.n_pam; k++) {
+            const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
+            const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                const uint32_t q = (uint32_t)w * kThreads + tid + qo;
+                const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
+                const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
+                ok[w] &= nz_nibbles(X & pmask) & ~bad;
+            }
+        }
         uint32_t m8[8];
         uint32_t cnt = 0;
 #pragma unroll
         for (int w = 0; w < 8; w++) {
-            uint32_t t = tid * 8u + (uint32_t)w;
-            uint32_t ok = 0x88888888u;
-            for (uint32_t k = 0; k < G.n_pam; k++) {
-                uint32_t pk = G.pam_pos[k];
-                uint32_t q = t + (pk >> 3);
-                uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], (pk & 7u) * 4u);
-                uint32_t bad = iupac ? multi_nibbles(X) : 0u;
-                ok &= nz_nibbles(X & G.pam_mask[k]) & ~bad;
-            }
-            uint32_t m = gather_nibble_flags(ok);
-            uint64_t first = (uint64_t)w * 8u;
-            if (n_valid <= first) m = 0;
-            else if (n_valid < first + 8) m &= (1u << (uint32_t)(n_valid - first)) - 1u;
+            uint32_t m = gather_nibble_flags(ok[w]);
+            // the last tile ends at scan_end
+            const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
+            if (scan_end <= first) m = 0;
+            else if (scan_end < first + 8) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
             m8[w] = m;
             cnt += (uint32_t)__popc(m);
         }
@@ -183,7 +197,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             uint32_t m = m8[w];
             while (m) {
                 uint32_t j = (uint32_t)__builtin_ctz(m);
-                cand[pos++] = (uint16_t)(tid * 64u + (uint32_t)w * 8u + j);
+                cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
                 m &= m - 1u;
             }
         }
@@ -199,24 +213,25 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 uint32_t win[W];
 #pragma unroll
                 for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                uint32_t p = pb;
                 // four patterns per iteration: their records are fetched with
-                // one batch of scalar loads, and one branch covers the four hits
+                // one batch of scalar loads (pointer offsets, no per-pattern
+                // address arithmetic), and one branch covers the four hits
+                uint32_t p = pb;
                 for (; p + 4 <= pe; p += 4) {
+                    const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
                     uint32_t hit = 0;
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        const PatRec& r = pats[p + u];
                         uint32_t sum = 0;
 #pragma unroll
-                        for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
-                        hit |= ((int32_t)sum >= r.thr ? 1u : 0u) << u;
+                        for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
+                        hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
                     }
                     if (active && hit) {
                         for (int u = 0; u < 4; u++) {
                             if (!((hit >> u) & 1u)) continue;
                             const PatRec& r = pats[p + u];
-                            uint32_t amm = recount(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
                             uint32_t idx = atomicAdd(out_count, 1u);
                             if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                         }
@@ -228,7 +243,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
 #pragma unroll
                     for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
                     if (active && (int32_t)sum >= r.thr) {
-                        uint32_t amm = recount(gw, i, full_words + (size_t)p * 8, G.L);
+                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
                         uint32_t idx = atomicAdd(out_count, 1u);
                         if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                     }

@@ -133,7 +133,7 @@ typedef struct {
  * them at kernel-compile time (`-D` constants, search.rs:322-343,1193-1281).
  * Results never depend on them; 0 selects the default of each field. */
 typedef struct {
-    uint32_t plain_threads;        /* plain-scan workgroup: 64, 128 (default), 256 or 512 threads,
+    uint32_t plain_threads;        /* plain-scan workgroup: 64 (default), 128, 256 or 512 threads,
                                       64 start positions per thread */
     uint32_t plain_pat_chunk;      /* patterns per plain-scan workgroup; a larger group is split
                                       over grid rows (0: whole group) */
