@@ -399,7 +399,7 @@ def main():
     # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     # this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a 256 MiB
     # device copy reads on the same counters); only valid at the default size
-    traffic = 1.5750e9 + 2.37e7 if abs(genome_nucl - 3.1e9) < 1e6 else None
+    traffic = 1.648e9 + 2.40e7 if abs(genome_nucl - 3.1e9) < 1e6 else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
