@@ -220,6 +220,41 @@ def apply_tuning(s, args):
         log(f"autotune: {s.autotune()}")
 
 
+def ingest_workload(args):
+    """Genome file -> resident genome: host reader vs device FASTA encoder on the headline genome."""
+    total = int(args.genome_bp)
+    lens = chrom_lengths(total, 24, args.seed)
+    sg = cofpy.SynthGenome(args.seed, lens, n_frac=0.05, gap=(10_000, 1_000_000))
+    with tempfile.TemporaryDirectory() as d:
+        fa = os.path.join(d, "genome.fa")
+        t = time.time()
+        sg.write_fasta(fa)
+        size = os.path.getsize(fa)
+        log(f"wrote {size / 1e9:.2f} GB FASTA in {time.time() - t:.1f} s")
+        res = {}
+        for name, flags in (("host", cofpy.COF_LOAD_HOST), ("device", cofpy.COF_LOAD_DEVICE)):
+            times = []
+            for k in range(args.warmup + args.steps):
+                s = cofpy.Search(["A" * 20 + "NGG"], ["N" * 20 + "NGG"], [0])
+                t0 = time.perf_counter()
+                s.load_file(fa, flags)
+                dt = time.perf_counter() - t0
+                st = s.stats()
+                s.close()
+                if k >= args.warmup:
+                    times.append(dt)
+                log(f"{name} load: {dt:.3f} s (load_ms {st.load_ms:.1f}, device {st.load_device})")
+            res[name] = round(sum(times) / len(times), 3)
+    out = {"metric": "genome load (FASTA -> resident bit4 genome)", "value": res["device"], "unit": "s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["device"] * 1e3, 1),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u8 (bit4 encode)",
+           "data": "synthetic",
+           "config": {"workload": f"{total / 1e9:.2f} Gbp FASTA, 60-nt lines, 24 chromosomes, 5% N", "bytes": size},
+           "host_reader_s": res["host"], "device_s": res["device"],
+           "speedup": round(res["host"] / res["device"], 2)}
+    print(json.dumps(out), flush=True)
+
+
 def schedule_flags(args):
     return {"graph": 0, "eager": cofpy.COF_FLAG_NO_GRAPH, "sync": cofpy.COF_FLAG_SYNC_ROUNDS}[args.schedule]
 
@@ -318,7 +353,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-bp", type=float, default=32e6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest"], default="plain")
     ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
     ap.add_argument("--train-genomes", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -341,6 +376,8 @@ def main():
         return batched_workload(args)
     if args.workload == "costmodel":
         return costmodel_workload(args)
+    if args.workload == "ingest":
+        return ingest_workload(args)
 
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)

@@ -22,7 +22,10 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_write_tsv", "cof_get_stats", "cof_close", "cof_strerror", "cof_last_error",
                "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
-               "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model"]
+               "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
+               "cof_load_genome_file_ex"]
+COF_LOAD_HOST = 1
+COF_LOAD_DEVICE = 2
 MLP_IN = 12
 MLP_PARAMS = 5057
 COF_MLP_EAGER = 1
@@ -72,7 +75,7 @@ class CofStats(C.Structure):
                 ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
                 ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
                 ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64),
-                ("convert_ms", C.c_double)]
+                ("convert_ms", C.c_double), ("load_ms", C.c_double), ("load_device", C.c_uint64)]
 
 
 _lib = None
@@ -90,6 +93,8 @@ def lib():
         L.cof_open.argtypes = [C.c_int, C.POINTER(CofQuery), C.POINTER(C.c_void_p)]
         L.cof_set_query.argtypes = [C.c_void_p, C.POINTER(CofQuery)]
         L.cof_load_genome_file.argtypes = [C.c_void_p, C.c_char_p]
+        L.cof_load_genome_file_ex.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
+        L.cof_load_genome_file_ex.restype = C.c_int
         L.cof_load_genome.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(CofChunk), C.c_uint64,
                                       C.POINTER(C.c_char_p), C.c_uint32]
         L.cof_search.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
@@ -247,8 +252,11 @@ class Search:
         if st != COF_OK:
             raise CofError(st, f"{what}: {lib().cof_last_error(self.h).decode()}")
 
-    def load_file(self, path):
-        self._check(lib().cof_load_genome_file(self.h, str(path).encode()), "load_genome_file")
+    def load_file(self, path, flags=None):
+        if flags is None:
+            self._check(lib().cof_load_genome_file(self.h, str(path).encode()), "load_genome_file")
+        else:
+            self._check(lib().cof_load_genome_file_ex(self.h, str(path).encode(), flags), "load_genome_file_ex")
 
     def load_synth(self, sg):
         ptr, nb, chunks, n, names = sg.layout()

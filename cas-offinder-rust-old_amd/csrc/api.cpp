@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <sys/stat.h>
 
 #include "cof_internal.hpp"
 #include "costmodel.hpp"
@@ -38,6 +39,8 @@ struct cof_ctx {
     cof::HitVec hits;
     cof::AuxVec aux;
     cof_stats stats{};
+    double load_ms = 0;
+    bool load_device = false;
     std::string err;
 };
 
@@ -48,23 +51,72 @@ cof_status fail(cof_ctx* ctx, const cof::Error& e) {
     return e.code;
 }
 
-void upload_genome(cof_ctx* ctx) {
+// Device buffer for a packed genome of n_bytes, zero-padded past the end so
+// kernels need no bounds checks (one plain tile + halo, bulge window).
+uint32_t* alloc_genome(cof_ctx* ctx, uint64_t n_bytes) {
     if (ctx->d_genome) { (void)hipFree(ctx->d_genome); ctx->d_genome = nullptr; }
-    uint64_t words = (ctx->g.n_bytes + 3) / 4;
+    uint64_t words = (n_bytes + 3) / 4;
     uint64_t tile = cof::plain_tile_nucl();
-    uint64_t n_tiles = (ctx->g.n_nibbles() + tile - 1) / tile + 1;
+    uint64_t n_tiles = (n_bytes * 2 + tile - 1) / tile + 1;
     uint64_t alloc_words = std::max<uint64_t>(words, n_tiles * tile / 8) + cof::plain_pad_words() + cof::bulge_pad_words();
     HIPCHECK(hipMalloc(&ctx->d_genome, alloc_words * 4));
     HIPCHECK(hipMemsetAsync(ctx->d_genome, 0, alloc_words * 4, ctx->stream));
-    if (ctx->g.n_bytes)
-        HIPCHECK(hipMemcpyAsync(ctx->d_genome, ctx->g.packed.data(), ctx->g.n_bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->d_genome_words = alloc_words;
+    return ctx->d_genome;
+}
+
+void finish_genome(cof_ctx* ctx) {
     ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
     ctx->have_genome = true;
     ctx->cached = false;
     ctx->stats.genome_nucl = 0;
     for (auto& c : ctx->g.chunks) ctx->stats.genome_nucl += c.size();
+}
+
+void upload_genome(cof_ctx* ctx) {
+    alloc_genome(ctx, ctx->g.n_bytes);
+    if (ctx->g.n_bytes)
+        HIPCHECK(hipMemcpyAsync(ctx->d_genome, ctx->g.packed.data(), ctx->g.n_bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    finish_genome(ctx);
+}
+
+bool is_fasta_file(const char* path) {
+    struct stat sb;
+    if (stat(path, &sb) != 0 || !S_ISREG(sb.st_mode)) return false;
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    int c = fgetc(f);
+    fclose(f);
+    return c == '>';
+}
+
+// genome file -> resident genome: device ingest for regular FASTA files unless
+// the host reader is asked for; the host reader for 2bit, folders and the rest
+cof_status load_file(cof_ctx* ctx, const char* path, uint32_t flags) {
+    if (!ctx || !path) return COF_E_INVALID;
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        auto t0 = std::chrono::steady_clock::now();
+        bool device = false;
+        if (!(flags & COF_LOAD_HOST) && is_fasta_file(path)) {
+            auto alloc = [](uint64_t n, void* u) { return alloc_genome((cof_ctx*)u, n); };
+            device = cof::read_fasta_device(path, ctx->g, alloc, ctx, ctx->stream);
+            if (device) finish_genome(ctx);
+        }
+        if (!device) {
+            if (flags & COF_LOAD_DEVICE)
+                throw cof::Error(COF_E_UNSUPPORTED, "device ingest needs a FASTA file with regular line widths");
+            cof::read_genome_file(path, ctx->g);
+            upload_genome(ctx);
+        }
+        ctx->load_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ctx->load_device = device;
+    } catch (const cof::Error& e) {
+        ctx->have_genome = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
 }
 
 // chunk features of the resident genome for the query, on the device
@@ -233,17 +285,10 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
     return COF_OK;
 }
 
-cof_status cof_load_genome_file(cof_ctx* ctx, const char* path) {
-    if (!ctx || !path) return COF_E_INVALID;
-    try {
-        HIPCHECK(hipSetDevice(ctx->device));
-        cof::read_genome_file(path, ctx->g);
-        upload_genome(ctx);
-    } catch (const cof::Error& e) {
-        ctx->have_genome = false;
-        return fail(ctx, e);
-    }
-    return COF_OK;
+cof_status cof_load_genome_file(cof_ctx* ctx, const char* path) { return load_file(ctx, path, 0); }
+
+cof_status cof_load_genome_file_ex(cof_ctx* ctx, const char* path, uint32_t flags) {
+    return load_file(ctx, path, flags);
 }
 
 cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
@@ -288,6 +333,8 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             uint64_t genome_nucl = st.genome_nucl;
             st = cof_stats{};
             st.genome_nucl = genome_nucl;
+            st.load_ms = ctx->load_ms;
+            st.load_device = ctx->load_device ? 1 : 0;
             ctx->hits.clear();
             ctx->aux.clear();
             cof::RawView<cof::PlainRaw> praw;
@@ -350,6 +397,8 @@ cof_status cof_write_tsv(cof_ctx* ctx, const char* path) {
 cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out) {
     if (!ctx || !out) return COF_E_INVALID;
     *out = ctx->stats;
+    out->load_ms = ctx->load_ms;
+    out->load_device = ctx->load_device ? 1 : 0;
     return COF_OK;
 }
 

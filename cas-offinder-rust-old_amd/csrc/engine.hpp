@@ -85,6 +85,12 @@ struct BulgeTune {
     const std::vector<float>* chunk_cost = nullptr;
 };
 
+// FASTA ingest on the device (csrc/ingest.hip): fills g (chunks, names, host
+// copy of the packed genome) and encodes into alloc(n_bytes, user); returns
+// false, with g empty, when the file's line layout is irregular.
+bool read_fasta_device(const std::string& path, Genome& g, uint32_t* alloc(uint64_t n_bytes, void* user), void* user,
+                       hipStream_t st);
+
 struct PlainPlan;
 PlainPlan* plain_plan_create(const Query& q);
 void plain_plan_destroy(PlainPlan* pp);

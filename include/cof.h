@@ -127,6 +127,8 @@ typedef struct {
     uint64_t n_redo_rounds;    /* bulge rounds whose queue overflowed and were redone in halves */
     uint64_t verify_busy;      /* diagnostics: slots whose lane held a candidate (busy / slots = lane use) */
     double convert_ms;         /* host conversion of raw device hits into final hits */
+    double load_ms;            /* wall time of the last genome file load */
+    uint64_t load_device;      /* 1 if that load encoded the FASTA on the device */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes
@@ -172,6 +174,16 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q);
  * *.fa files, with the reference readers' chunking (read_fasta.rs:9-71,
  * read_2bit.rs:33-118, read_fasta_folder.rs:8-23, main.rs:106-121). */
 cof_status cof_load_genome_file(cof_ctx* ctx, const char* path);
+
+/* cof_load_genome_file with a choice of reader: COF_LOAD_HOST forces the
+ * host reader; COF_LOAD_DEVICE requires the device FASTA encoder
+ * (COF_E_UNSUPPORTED for 2bit files, folders or FASTA files whose lines in a
+ * chunk do not share one width); 0 = device for regular FASTA, else host.
+ * The result is the same genome either way.  No reference counterpart (the
+ * reference reads on one host thread, read_fasta.rs:9-71). */
+#define COF_LOAD_HOST 1u
+#define COF_LOAD_DEVICE 2u
+cof_status cof_load_genome_file_ex(cof_ctx* ctx, const char* path, uint32_t flags);
 
 /* Makes a caller-built genome resident: `packed` is the concatenation of the
  * chunks in bit4 (search.rs:1376-1427 layout: ceil(size/2) bytes per chunk,

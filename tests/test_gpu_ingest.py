@@ -1,0 +1,96 @@
+"""Device FASTA ingest (cof_load_genome_file_ex) gives the same genome as the host reader.
+
+The host reader restates read_fasta.rs:9-71 (pinned by the reference's
+upstream1000 fixtures in test_oracle.py).  Here FASTA files with several line
+shapes are loaded both ways and searched with a dense all-N query whose rows
+print the genome text around every hit, plus the reference query and
+fixtures; irregular files fall back to the host reader (or are refused when
+the device path is required).
+"""
+import os
+import random
+
+import pytest
+
+import cofpy
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def _rows(path, flags, guides, filt, mm, tmp_path, tag):
+    s = cofpy.Search(guides, [filt] * len(guides), [mm] * len(guides), batch_nucl=1 << 22)
+    s.load_file(path, flags)
+    s.search()
+    st = s.stats()
+    out = tmp_path / f"{tag}.tsv"
+    s.write_tsv(out)
+    s.close()
+    return cofpy.read_rows(out), st
+
+
+def _dense_query(seed):
+    rng = random.Random(seed)
+    return ["".join(rng.choice("ACGT") for _ in range(12)) for _ in range(3)], "N" * 12, 5
+
+
+def _compare(path, tmp_path, seed=1, expect_device=True):
+    guides, filt, mm = _dense_query(seed)
+    host, st_h = _rows(path, cofpy.COF_LOAD_HOST, guides, filt, mm, tmp_path, "h")
+    dev, st_d = _rows(path, 0, guides, filt, mm, tmp_path, "d")
+    assert st_h.load_device == 0
+    assert st_d.load_device == (1 if expect_device else 0)
+    assert len(host) > 1000
+    assert dev == host
+
+
+@pytest.mark.parametrize("width", [60, 61, 80, 1000])
+def test_device_ingest_equals_host(tmp_path, width):
+    sg = cofpy.SynthGenome(300 + width, [400_000, 65_536 * 2 + 7, 1_003], line_width=width, n_frac=0.03,
+                           gap=(20, 900), iupac_frac=0.001)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    _compare(fa, tmp_path, width)
+
+
+def test_crlf_lowercase_and_empty_chromosome(tmp_path):
+    sg = cofpy.SynthGenome(77, [200_000, 50_001], line_width=70, n_frac=0.02, gap=(10, 500))
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    text = open(fa).read().splitlines()
+    rng = random.Random(3)
+    out = []
+    for ln in text:
+        if not ln.startswith(">") and rng.random() < 0.3:
+            ln = ln.lower()
+        out.append(ln)
+    out.insert(0, ">empty_chrom")
+    with open(tmp_path / "crlf.fa", "w", newline="") as f:
+        f.write("\r\n".join(out) + "\r\n")
+    _compare(tmp_path / "crlf.fa", tmp_path, 5)
+
+
+def test_irregular_lines_fall_back(tmp_path):
+    rng = random.Random(9)
+    lines = [">chrA"]
+    for _ in range(3000):
+        lines.append("".join(rng.choice("ACGTN") for _ in range(rng.randint(40, 90))))
+    fa = tmp_path / "irr.fa"
+    with open(fa, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    _compare(fa, tmp_path, 7, expect_device=False)
+    guides, filt, mm = _dense_query(7)
+    s = cofpy.Search(guides, [filt] * 3, [mm] * 3)
+    with pytest.raises(cofpy.CofError) as e:
+        s.load_file(fa, cofpy.COF_LOAD_DEVICE)
+    assert e.value.status == cofpy.COF_E_UNSUPPORTED
+    s.close()
+
+
+def test_reference_fixture_both_ways(tmp_path):
+    # integration_bulge.rs:159-175 query on upstream1000.fa: 117 rows either way
+    fa = os.path.join(GOLDEN, "upstream1000.fa")
+    guides = ["CCGTGGTTCAACATTTGCTTAGCA", "GATGTTGGTAAGTGGGATATGGCA"]
+    h, st_h = _rows(fa, cofpy.COF_LOAD_HOST, guides, "N" * 24, 11, tmp_path, "h")
+    d, st_d = _rows(fa, cofpy.COF_LOAD_DEVICE, guides, "N" * 24, 11, tmp_path, "d")
+    assert len(h) == 117 and d == h and st_d.load_device == 1
