@@ -98,6 +98,17 @@ __device__ __forceinline__ bool pam_cell_ok(uint32_t g, uint32_t f) {
     return (g & f) != 0;
 }
 
+// The same two tests as 0/1 values without branches (the traceback's hot
+// path evaluates them for 64 lanes at once).
+__device__ __forceinline__ uint32_t pam_cell_ok01(uint32_t g, uint32_t f) {
+    const uint32_t any = g != 0u;
+    const uint32_t spec = (g != 0u) & (g != 0xFu) & ((g & f) != 0u);
+    return f == 0xFu ? any : spec;
+}
+__device__ __forceinline__ uint32_t cmp_loose01(uint32_t p, uint32_t t) {
+    return (t != 0u) & ((p == 0xFu) | ((t != 0xFu) & ((p & t) != 0u)));
+}
+
 // cmp_loose_bit4 (traceback.rs:129-134)
 __device__ __forceinline__ bool cmp_loose(uint32_t p, uint32_t t) {
     if (t == 0) return false;
@@ -410,7 +421,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     uint32_t d_floor = 0;   // depth the lane's subtree starts at (> 0 for donated work)
     __shared__ uint32_t pairing[64];
     uint64_t ops[kOpsW];
-    uint64_t cm0 = 0, cm1 = 0, cm2 = 0;
+    uint64_t cm0 = 0, cm1 = 0, cm2 = 0, cm3 = 0;   // cm3: the diagonal child is a mismatch
 #pragma unroll
     for (int k = 0; k < kOpsW; k++) ops[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0, my_busy = 0;
@@ -434,9 +445,11 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             ops[0] = (ops[0] & m) | b;
         }
     };
+    // bits 0-2: unexplored children (diagonal, RNA, DNA); bit 3: diagonal is a mismatch
     auto get4 = [&](uint32_t dd) -> uint32_t {
         if (dd >= 64) return 0u;
-        return ((uint32_t)(cm0 >> dd) & 1u) | (((uint32_t)(cm1 >> dd) & 1u) << 1) | (((uint32_t)(cm2 >> dd) & 1u) << 2);
+        return ((uint32_t)(cm0 >> dd) & 1u) | (((uint32_t)(cm1 >> dd) & 1u) << 1) | (((uint32_t)(cm2 >> dd) & 1u) << 2) |
+               (((uint32_t)(cm3 >> dd) & 1u) << 3);
     };
     auto set4 = [&](uint32_t dd, uint32_t val) {
         if (dd >= 64) return;
@@ -444,6 +457,25 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         cm0 = (val & 1u) ? (cm0 | m) : (cm0 & ~m);
         cm1 = (val & 2u) ? (cm1 | m) : (cm1 & ~m);
         cm2 = (val & 4u) ? (cm2 | m) : (cm2 & ~m);
+        cm3 = (val & 8u) ? (cm3 | m) : (cm3 & ~m);
+    };
+    // counts of the mismatch, DNA and RNA ops at depths [k, dd) (2-bit op
+    // codes 0 match, 1 mismatch, 2 DNA, 3 RNA, depth k at bits 2k)
+    auto op_counts = [&](uint32_t k, uint32_t dd, uint32_t& nS, uint32_t& nD, uint32_t& nR) {
+        nS = nD = nR = 0;
+#pragma unroll
+        for (int w = 0; w < kOpsW; w++) {
+            const uint32_t lo = (uint32_t)w * 32u;
+            uint64_t keep = ~0ull;
+            if (dd <= lo) keep = 0;
+            else if (dd < lo + 32) keep = (1ull << (2 * (dd - lo))) - 1ull;
+            if (k > lo) keep = k >= lo + 32 ? 0ull : keep & ~((1ull << (2 * (k - lo))) - 1ull);
+            const uint64_t x = ops[w] & keep;
+            const uint64_t b0 = x & 0x5555555555555555ull, b1 = (x >> 1) & 0x5555555555555555ull;
+            nS += (uint32_t)__popcll(b0 & ~b1);
+            nD += (uint32_t)__popcll(b1 & ~b0);
+            nR += (uint32_t)__popcll(b0 & b1);
+        }
     };
     auto text_at = [&](uint32_t a) -> uint32_t {
         uint32_t t = bit(tT, a) | bit(tC, a) << 1 | bit(tA, a) << 2 | bit(tG, a) << 3;
@@ -463,25 +495,32 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     // of classify_myers_candidate (search.rs:638-671) would reject the leaf.
     auto children = [&]() -> uint32_t {
         if (i == 0) return 0u;
-        WORD vp1 = cvp[(i - 1) * 64 + lane], vn1 = cvn[(i - 1) * 64 + lane];
-        WORD vp0 = cvp[i * 64 + lane], vn0 = cvn[i * 64 + lane];
-        uint32_t v = 0;
-        const uint32_t n_im1 = bit(pn, i - 1), n_i = i < L ? bit(pn, i) : 0u, n_im2 = i >= 2 ? bit(pn, i - 2) : 0u;
-        if (jj > 0 && !bit(z, jj - 1)) {
-            uint32_t tv = text_at(jj - 1);
-            uint32_t mc = cmp_loose(pat_at(i - 1), tv) ? 0u : 1u;
-            bool pad_next = i - 1 >= 1 && jj - 1 >= 1 && bit(z, jj - 2);
-            if (!(mc && n_im1) && mm + mc <= M && !pad_next && pam_cell_ok(tv, filt_at(i - 1)) &&
-                cost + mc + lbc(i - 1, vp1, vn1, jj - 1) <= E)
-                v |= 1u;
-        }
-        if (rb < R && !n_im1 && !n_i && !n_im2 && !(i - 1 >= 1 && jj >= 1 && bit(z, jj - 1)) &&
-            cost + 1 + lbc(i - 1, vp1, vn1, jj) <= E)
-            v |= 2u;
-        if (db < D && jj > 0 && i < L && !(n_im1 && n_i) && !(jj - 1 >= 1 && bit(z, jj - 2)) && !bit(z, jj - 1) &&
-            cost + 1 + lbc(i, vp0, vn0, jj - 1) <= E)
-            v |= 4u;
-        return v;
+        const WORD vp1 = cvp[(i - 1) * 64 + lane], vn1 = cvn[(i - 1) * 64 + lane];
+        const WORD vp0 = cvp[i * 64 + lane], vn0 = cvn[i * 64 + lane];
+        // every test below is a 0/1 value (straight-line code, no lane branches);
+        // indices are clamped where a test does not apply and masked by it
+        const uint32_t has_j = jj > 0u;
+        const uint32_t j1 = jj - has_j;                     // jj - 1 (or 0)
+        const uint32_t has_j2 = jj >= 2u;
+        const uint32_t j2 = has_j2 ? jj - 2u : 0u;
+        const uint32_t z1 = has_j & bit(z, j1);             // padding at jj - 1
+        const uint32_t z2 = has_j2 & bit(z, j2);            // padding at jj - 2
+        const uint32_t n_im1 = bit(pn, i - 1);
+        const uint32_t n_i = i < L ? bit(pn, i < L ? i : 0u) : 0u;
+        const uint32_t n_im2 = i >= 2u ? bit(pn, i >= 2u ? i - 2u : 0u) : 0u;
+        const uint32_t i2 = i >= 2u;
+        // diagonal (match or substitution)
+        const uint32_t tv = text_at(j1);
+        const uint32_t mc = 1u - cmp_loose01(pat_at(i - 1), tv);
+        const uint32_t diag = has_j & (1u - z1) & (1u - (mc & n_im1)) & (mm + mc <= M) & (1u - (i2 & z2)) &
+                              pam_cell_ok01(tv, filt_at(i - 1)) & (cost + mc + lbc(i - 1, vp1, vn1, j1) <= E);
+        // RNA bulge (skip a pattern base)
+        const uint32_t rna = (rb < R) & (1u - n_im1) & (1u - n_i) & (1u - n_im2) & (1u - (i2 & z1)) &
+                             (cost + 1u + lbc(i - 1, vp1, vn1, jj) <= E);
+        // DNA bulge (skip a text base)
+        const uint32_t dna = (db < D) & has_j & (i < L) & (1u - (n_im1 & n_i)) & (1u - z2) & (1u - z1) &
+                             (cost + 1u + lbc(i, vp0, vn0, j1) <= E);
+        return diag | (rna << 1) | (dna << 2) | (mc << 3);
     };
     // leaf (i == 0): emit the alignment (ops stay in stack order; the host
     // reverses recorded ones)
@@ -634,21 +673,8 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     // donor: node state at depth kd (undo the ops of depths [kd, d))
                     uint32_t s_i = i, s_jj = jj, s_cost = cost, s_mm = mm, s_db = db, s_rb = rb, s_mask = 0;
                     if (give) {
-                        uint32_t nS = 0, nD = 0, nR = 0;
-#pragma unroll
-                        for (int w = 0; w < kOpsW; w++) {
-                            const uint32_t lo = (uint32_t)w * 32u;
-                            uint64_t x = ops[w];
-                            uint64_t keep = ~0ull;
-                            if (d <= lo) keep = 0;
-                            else if (d < lo + 32) keep = (1ull << (2 * (d - lo))) - 1ull;
-                            if (kd > lo) keep = kd >= lo + 32 ? 0ull : keep & ~((1ull << (2 * (kd - lo))) - 1ull);
-                            x &= keep;
-                            const uint64_t b0 = x & 0x5555555555555555ull, b1 = (x >> 1) & 0x5555555555555555ull;
-                            nS += (uint32_t)__popcll(b0 & ~b1);
-                            nD += (uint32_t)__popcll(~b0 & b1 & 0x5555555555555555ull);
-                            nR += (uint32_t)__popcll(b0 & b1);
-                        }
+                        uint32_t nS, nD, nR;
+                        op_counts(kd, d, nS, nD, nR);
                         const uint32_t nM = d - kd - nS - nD - nR;
                         s_i = i + nM + nS + nR;
                         s_jj = jj + nM + nS + nD;
@@ -656,7 +682,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         s_mm = mm - nS;
                         s_db = db - nD;
                         s_rb = rb - nR;
-                        s_mask = get4(kd);
+                        s_mask = get4(kd);          // children left at kd, with the diagonal's mismatch bit
                         set4(kd, 0u);
                     }
                     // every lane takes part in the shuffles; receivers keep the values
@@ -704,7 +730,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         if (sink.count_only) load_c0();
 #pragma unroll
                         for (int w = 0; w < kOpsW; w++) ops[w] = r_ops[w];
-                        cm0 = cm1 = cm2 = 0;
+                        cm0 = cm1 = cm2 = cm3 = 0;
                         set4(d, r_mask);
                         busy = true;
                     }
@@ -717,39 +743,49 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         // One iteration = pop exhausted nodes (cheap, may repeat) and then one
         // push with its child expansion (the costly part), so the expansion
         // runs with every busy lane of the wave at once.
-        uint32_t viable = get4(d);
-        while (!viable && d > d_floor) {
-            my_steps++;
-            cand_steps++;
-            d--;
-            const uint32_t op = get2(d);
-            if (op <= 1) { i++; jj++; cost -= op; mm -= op; }
-            else if (op == 2) { jj++; cost--; db--; }
-            else { i++; cost--; rb--; }
-            viable = get4(d);
-        }
-        my_steps++;
-        cand_steps++;
-        if (!viable) {
-            // root exhausted: candidate done
+        // Backtrack in one step: the deepest depth in [d_floor, d] that still
+        // has unexplored children is the highest set bit of the child planes;
+        // the state there follows from the counts of the ops above it.
+        uint64_t pend = cm0 | cm1 | cm2;
+        if (d < 63) pend &= (2ull << d) - 1ull;
+        pend &= ~((1ull << d_floor) - 1ull);
+        if (!pend) {
+            // subtree exhausted: candidate (or donated part) done
+            my_steps += d - d_floor + 1u;
+            cand_steps += d - d_floor + 1u;
             busy = false;
             if (sink.cand_steps) sink.cand_steps[cand_idx] = cand_steps;
             continue;
         }
-        // push the next admissible child and expand it
-        const uint32_t t = (uint32_t)__builtin_ctz(viable);
-        set4(d, viable & (viable - 1));
-        if (t == 0) {
-            uint32_t mc = cmp_loose(pat_at(i - 1), text_at(jj - 1)) ? 0u : 1u;
-            set2(d, mc);
-            i--; jj--; cost += mc; mm += mc;
-        } else if (t == 1) {
-            set2(d, 3);
-            i--; cost++; rb++;
-        } else {
-            set2(d, 2);
-            jj--; cost++; db++;
+        const uint32_t k = 63u - (uint32_t)__builtin_clzll(pend);
+        my_steps += d - k + 1u;
+        cand_steps += d - k + 1u;
+        if (k < d) {
+            uint32_t nS, nD, nR;
+            op_counts(k, d, nS, nD, nR);
+            const uint32_t nM = d - k - nS - nD - nR;
+            i += nM + nS + nR;
+            jj += nM + nS + nD;
+            cost -= nS + nD + nR;
+            mm -= nS;
+            db -= nD;
+            rb -= nR;
+            d = k;
         }
+        const uint32_t node = get4(d);
+        const uint32_t viable = node & 7u;
+        // push the next admissible child (diagonal, then RNA, then DNA) and expand it
+        const uint32_t t = (uint32_t)__builtin_ctz(viable);
+        const uint32_t mcb = node >> 3;
+        set4(d, (viable & (viable - 1)) | (node & 8u));
+        const uint32_t op = t == 0u ? mcb : (t == 1u ? 3u : 2u);
+        set2(d, op);
+        i -= (uint32_t)(t != 2u);
+        jj -= (uint32_t)(t != 1u);
+        cost += t == 0u ? mcb : 1u;
+        mm += (uint32_t)(t == 0u) & mcb;
+        rb += (uint32_t)(t == 1u);
+        db += (uint32_t)(t == 2u);
         d++;
         if (d > (uint32_t)sizeof(WORD) * 8 || i > L || jj > (uint32_t)sizeof(WORD) * 8) {
             // impossible state: stop this lane and report instead of looping
