@@ -87,7 +87,7 @@ def cpu_baseline(run, sample_bp, seed, threads):
 
 
 def bulge_workload(args):
-    """Plain-schedule bulge search: 3.1 Gbp layout with 10% repeat copies, 100 guides, NRG, M4 D2 R2."""
+    """Bulge search: 3.1 Gbp layout with 10% repeat copies, 100 guides, NRG, M4 D2 R2."""
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)
     rng = random.Random(args.seed + 5)
@@ -133,6 +133,9 @@ def bulge_workload(args):
            "schedule": args.schedule, "tuning": s.tuning(), "pipeline_rounds": st.n_pipeline_rounds,
            "kernel_ms_filter": round(filt / args.steps, 1), "kernel_ms_verify": round(ver / args.steps, 1)}
     s.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = bulge_cpu_baseline(args, guides, plants)
+        log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 
@@ -259,6 +262,25 @@ def schedule_flags(args):
     return {"graph": 0, "eager": cofpy.COF_FLAG_NO_GRAPH, "sync": cofpy.COF_FLAG_SYNC_ROUNDS}[args.schedule]
 
 
+def bulge_cpu_baseline(args, guides, plants):
+    """CPU oracle on a bounded sample of the bulge workload (same genome model and query)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    sample = int(args.bulge_cpu_sample_bp)
+    threads = min(16, os.cpu_count() or 1)
+    sg = cofpy.SynthGenome(args.seed + 23, [sample], n_frac=0.05, gap=(10_000, 1_000_000),
+                           repeats=(0.10, 20, 300, 0.10), plants=[(p[0], p[1], p[2], p[3], 1) for p in plants])
+    with tempfile.TemporaryDirectory() as d:
+        fa = os.path.join(d, "s.fa")
+        sg.write_fasta(fa)
+        in_text = f"{fa}\nNNNNNNNNNNNNNNNNNNNNNRG 2 2\n" + "".join(f"{g}NNN 4\n" for g in guides)
+        n, st = oracle.run(in_text, os.path.join(d, "o.tsv"), batch_nucl=1 << 22, threads=threads)
+    secs = st["search_seconds"]
+    return {"value": round(len(guides) * sample / secs / 1e9, 4), "unit": "guide-Gbp/s", "cores": threads,
+            "kind": "port", "sample": f"{len(guides)} guides x {sample / 1e6:.1f} Mbp repeat-rich synthetic, M4 D2 R2, "
+                                      f"4 Mnt batches, search-only time {secs:.2f} s, {n} rows"}
+
+
 def batched_guides(seed, n):
     """10,000-guide mix: len U{17..30}, PAM in {NRG, NGG, NNGRRT, TTTN}, M U{0..6}, 20% with D, R in U{1,2}."""
     rng = random.Random(seed)
@@ -352,6 +374,7 @@ def main():
     ap.add_argument("--genome-bp", type=float, default=3.1e9)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-bp", type=float, default=32e6)
+    ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest"], default="plain")
     ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
