@@ -23,7 +23,7 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
                "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
-               "cof_load_genome_file_ex"]
+               "cof_load_genome_file_ex", "cof_device_count", "cof_device_name"]
 COF_LOAD_HOST = 1
 COF_LOAD_DEVICE = 2
 MLP_IN = 12

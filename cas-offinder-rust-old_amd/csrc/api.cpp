@@ -572,6 +572,25 @@ void cof_close(cof_ctx* ctx) {
     delete ctx;
 }
 
+cof_status cof_device_count(int* n) {
+    if (!n) return COF_E_INVALID;
+    *n = 0;
+    if (hipGetDeviceCount(n) != hipSuccess) {
+        *n = 0;
+        return COF_E_HIP;
+    }
+    return COF_OK;
+}
+
+cof_status cof_device_name(int device, char* buf, size_t cap) {
+    if (!buf || cap == 0) return COF_E_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return COF_E_INVALID;
+    snprintf(buf, cap, "%s (%s, %d CUs, %.0f GiB)", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+             prop.totalGlobalMem / 1073741824.0);
+    return COF_OK;
+}
+
 const char* cof_strerror(cof_status s) {
     switch (s) {
         case COF_OK: return "ok";

@@ -2,11 +2,15 @@
 // on one MI355X.  Usage: cof-cli {input|-} {C|G|A} {output|-}
 // The device letter is accepted for compatibility; the search always runs on
 // HIP device 0 (COF_DEVICE overrides).  Input parsing follows
-// cas-offinder-cli/src/cli_utils.rs:37-148.
+// cas-offinder-cli/src/cli_utils.rs:37-148.  A run that writes to a file also
+// writes the run summary <output>.log (log_writer.rs:27-53, main.rs:277-304).
+#include <sys/stat.h>
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -44,19 +48,58 @@ int die(const char* msg) {
     return 1;
 }
 
+// Usage text of main.rs:17-46; the device list names the HIP devices the
+// library sees.
+void usage() {
+    fprintf(stderr,
+            "\nCas-OFFinder 2 - cof-cli (MI355X)\n\n"
+            "Usage: cof-cli [options] {input_filename|-} {C|G|A}[device_id(s)] {output_filename|-}\n"
+            "(C: using CPUs, G: using GPUs, A: using accelerators)\n\n"
+            "Example input file:\n/var/chromosomes/human_hg19\nNNNNNNNNNNNNNNNNNNNNNRG\n"
+            "GGCCGACCTGTCGCTGACGCNNN 5\nCGCCAGCGTCAGCGACAGGTNNN 5\n"
+            "ACGGCGCCAGCGTCAGCGACNNN 5\nGTCGCTGACGCTGGCGCCGTNNN 5\n\n"
+            "Available device list:\n");
+    int n = 0;
+    if (cof_device_count(&n) == COF_OK)
+        for (int d = 0; d < n; d++) {
+            char name[256];
+            if (cof_device_name(d, name, sizeof name) == COF_OK) fprintf(stderr, "Type: GPU, ID: %d, <%s>\n", d, name);
+        }
+}
+
+// Run summary with the fields and layout of log_writer.rs:27-53.
+struct RunLog {
+    std::string genome_path, search_filter, device_label, algorithm;
+    uint64_t genome_size = 0, n_matches = 0;
+    size_t n_patterns = 0;
+    uint32_t max_mismatches = 0, max_dna_bulges = 0, max_rna_bulges = 0;
+    double total_elapsed_secs = 0;
+};
+
+bool write_log(const std::string& path, const RunLog& l) {
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    fprintf(f, "=== Cas-OFFinder Rust ===\nRun date (unix timestamp): %llu\n\n", (unsigned long long)time(nullptr));
+    fprintf(f, "Algorithm: %s\nDevice: %s\n\nInput:\n", l.algorithm.c_str(), l.device_label.c_str());
+    fprintf(f, "  Genome: %s\n  Genome size: %llu bytes\n  Patterns: %zu\n  Search filter: %s\n",
+            l.genome_path.c_str(), (unsigned long long)l.genome_size, l.n_patterns, l.search_filter.c_str());
+    fprintf(f, "  Max mismatches: %u\n  Max DNA bulges: %u\n  Max RNA bulges: %u\n\n", l.max_mismatches,
+            l.max_dna_bulges, l.max_rna_bulges);
+    fprintf(f, "Results:\n  Matches found: %llu\n  Total elapsed: %.3fs\n", (unsigned long long)l.n_matches,
+            l.total_elapsed_secs);
+    return fclose(f) == 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 4) {
-        fprintf(stderr,
-                "Usage: cof-cli {input_filename|-} {C|G|A} {output_filename|-}\n"
-                "Example input file:\n/var/chromosomes/human_hg19\nNNNNNNNNNNNNNNNNNNNNNRG\n"
-                "GGCCGACCTGTCGCTGACGCNNN 5\nCGCCAGCGTCAGCGACAGGTNNN 5\n");
+        usage();
         return argc < 2 ? 0 : 1;
     }
     auto t0 = std::chrono::steady_clock::now();
     std::string dev = argv[2];
-    if (dev.size() != 1 || (dev[0] != 'C' && dev[0] != 'G' && dev[0] != 'A'))
+    if (dev.empty() || (dev[0] != 'C' && dev[0] != 'G' && dev[0] != 'A'))
         return die("2nd argument must be one of {C|G|A}");
     std::string text;
     {
@@ -142,5 +185,22 @@ int main(int argc, char** argv) {
     if (st != COF_OK) return die(cof_strerror(st));
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     fprintf(stderr, "Completed in %gs (search %g ms, %llu rows)\n", secs, stats.search_ms, (unsigned long long)n);
+    if (strcmp(argv[3], "-") != 0) {
+        RunLog l;
+        l.genome_path = lines[0];
+        struct stat sb;
+        l.genome_size = stat(lines[0].c_str(), &sb) == 0 ? (uint64_t)sb.st_size : 0;
+        l.n_patterns = guides.size();
+        l.search_filter = filter;
+        l.max_mismatches = M;
+        l.max_dna_bulges = D;
+        l.max_rna_bulges = R;
+        l.device_label = dev[0] == 'C' ? "CPU" : dev[0] == 'G' ? "GPU" : "Accelerator";
+        l.algorithm = (D > 0 || R > 0) ? "Myers bit-parallel" : "popcount (legacy)";
+        l.n_matches = n;
+        l.total_elapsed_secs = secs;
+        std::string lp = std::string(argv[3]) + ".log";
+        if (!write_log(lp, l)) fprintf(stderr, "warning: failed to write log file '%s'\n", lp.c_str());
+    }
     return 0;
 }

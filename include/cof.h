@@ -253,6 +253,11 @@ cof_status cof_get_tuning(const cof_ctx* ctx, cof_tuning* out);
  * reference counterpart (the reference compiles one shape per run). */
 cof_status cof_autotune(cof_ctx* ctx, uint64_t sample_nucl, cof_tuning* chosen);
 void cof_close(cof_ctx* ctx);
+/* HIP devices visible to the library and their marketing names; replaces the
+ * device discovery and get_device_strs of OclRunConfig (run_config.rs:30-74),
+ * which the CLI's usage text lists.  Neither call needs a context. */
+cof_status cof_device_count(int* n);
+cof_status cof_device_name(int device, char* buf, size_t cap);
 const char* cof_strerror(cof_status s);
 const char* cof_last_error(const cof_ctx* ctx);
 

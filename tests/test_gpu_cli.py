@@ -45,10 +45,24 @@ def test_cli_matches_oracle(tmp_path, filt_line, mm):
     assert got[0] == want[0]                      # header
     assert sorted(got[1:]) == sorted(want[1:])
     assert len(want) > 1
+    # run summary beside the output (log_writer.rs:27-53)
+    log = open(str(tmp_path / "out.tsv") + ".log").read().splitlines()
+    bulge = len(filt_line.split()) == 3
+    assert log[0] == "=== Cas-OFFinder Rust ==="
+    assert log[1].startswith("Run date (unix timestamp): ") and int(log[1].split(": ")[1]) > 1_600_000_000
+    assert log[3] == "Algorithm: " + ("Myers bit-parallel" if bulge else "popcount (legacy)")
+    assert log[4] == "Device: GPU"
+    assert log[6:14] == ["Input:", f"  Genome: {fa}", f"  Genome size: {os.path.getsize(fa)} bytes",
+                         "  Patterns: 2", f"  Search filter: {filt_line.split()[0]}", f"  Max mismatches: {mm}",
+                         f"  Max DNA bulges: {1 if bulge else 0}", f"  Max RNA bulges: {1 if bulge else 0}"]
+    assert log[15:17] == ["Results:", f"  Matches found: {len(want) - 1}"]
+    assert log[17].startswith("  Total elapsed: ") and log[17].endswith("s")
     # stdin input, stdout output
+    os.remove(str(tmp_path / "out.tsv") + ".log")
     r2 = _run_cli(["-", "G", "-"], stdin=text)
     assert r2.returncode == 0, r2.stderr
     assert sorted(r2.stdout.splitlines()[1:]) == sorted(want[1:])
+    assert not any(p.name.endswith(".log") for p in tmp_path.iterdir())     # no summary for stdout output
 
 
 def test_cli_rejects_bad_input(tmp_path):
