@@ -366,6 +366,55 @@ def batched_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def tsv_workload(args):
+    """Reference TSV output of the headline run with the most rows (NRG-20-M6 on the 3.1 Gbp genome):
+    rows formatted on the device vs the host writer, into /dev/null (formatting) and into a file."""
+    import hashlib
+    total = int(args.genome_bp)
+    lens = chrom_lengths(total, 24, args.seed)
+    runs = make_queries(args.seed)
+    plants = [p for r in runs for p in r["plants"]]
+    sg = cofpy.SynthGenome(args.seed, lens, n_frac=0.05, gap=(10_000, 1_000_000), plants=plants)
+    r = next(r for r in runs if r["name"] == "NRG-20-M6")
+    s = cofpy.Search(r["pats"], [r["filter"]] * len(r["pats"]), [r["mm"]] * len(r["pats"]))
+    s.load_synth(sg)
+    rows = s.search()
+    log(f"{r['name']}: {rows} rows")
+    res = {}
+    with tempfile.TemporaryDirectory(dir=args.tsv_dir) as d:
+        digests = {}
+        for mode in ("device", "host"):
+            path = os.path.join(d, mode + ".tsv")
+            s.write_tsv(path, host=mode == "host")
+            digests[mode] = (os.path.getsize(path), hashlib.sha256(open(path, "rb").read()).hexdigest())
+            os.remove(path)
+            for target in ("/dev/null", path):
+                times = []
+                for k in range(args.warmup + args.steps):
+                    s.write_tsv(target, host=mode == "host")
+                    if k >= args.warmup:
+                        times.append(s.stats().write_ms)
+                    if target != "/dev/null":
+                        os.remove(target)
+                key = f"{mode}_{'devnull' if target == '/dev/null' else 'file'}_ms"
+                res[key] = round(sum(times) / len(times), 2)
+                log(f"{mode} -> {target}: {res[key]} ms")
+    s.close()
+    size, _ = digests["device"]
+    out = {"metric": "TSV rows/s (reference row format, device formatter)",
+           "value": round(rows / (res["device_devnull_ms"] / 1e3)), "unit": "rows/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["device_devnull_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8 (text)",
+           "data": "synthetic",
+           "config": {"workload": f"TSV rows of NRG-20-M6 (100 guides, M6) on the {total / 1e9:.2f} Gbp headline genome, "
+                                  "written to /dev/null; file figures beside", "genome_bp": total},
+           "rows": rows, "bytes": size, "identical_bytes": digests["device"] == digests["host"],
+           "host_rows_per_s": round(rows / (res["host_devnull_ms"] / 1e3)),
+           "speedup_vs_host": round(res["host_devnull_ms"] / res["device_devnull_ms"], 2)}
+    out.update(res)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,7 +425,8 @@ def main():
     ap.add_argument("--cpu-sample-bp", type=float, default=3.2e8)
     ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv"], default="plain")
+    ap.add_argument("--tsv-dir", default=None, help="tsv: folder for the file-output timings (default: system temp)")
     ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
     ap.add_argument("--train-genomes", type=int, default=20)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -401,6 +451,8 @@ def main():
         return costmodel_workload(args)
     if args.workload == "ingest":
         return ingest_workload(args)
+    if args.workload == "tsv":
+        return tsv_workload(args)
 
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)

@@ -23,7 +23,8 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
                "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
-               "cof_load_genome_file_ex", "cof_device_count", "cof_device_name"]
+               "cof_load_genome_file_ex", "cof_device_count", "cof_device_name", "cof_write_tsv_ex"]
+COF_WRITE_HOST = 1
 COF_LOAD_HOST = 1
 COF_LOAD_DEVICE = 2
 MLP_IN = 12
@@ -75,7 +76,8 @@ class CofStats(C.Structure):
                 ("bulge_hits", C.c_uint64), ("bulge_checksum", C.c_uint64),
                 ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
                 ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64),
-                ("convert_ms", C.c_double), ("load_ms", C.c_double), ("load_device", C.c_uint64)]
+                ("convert_ms", C.c_double), ("load_ms", C.c_double), ("load_device", C.c_uint64),
+                ("write_ms", C.c_double), ("write_device", C.c_uint64)]
 
 
 _lib = None
@@ -99,6 +101,7 @@ def lib():
                                       C.POINTER(C.c_char_p), C.c_uint32]
         L.cof_search.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
         L.cof_write_tsv.argtypes = [C.c_void_p, C.c_char_p]
+        L.cof_write_tsv_ex.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
         L.cof_get_stats.argtypes = [C.c_void_p, C.POINTER(CofStats)]
         L.cof_close.argtypes = [C.c_void_p]
         L.cof_strerror.argtypes = [C.c_int]
@@ -276,8 +279,9 @@ class Search:
         self._check(lib().cof_search(self.h, buf, n, None), "search")
         return buf[:n]
 
-    def write_tsv(self, path):
-        self._check(lib().cof_write_tsv(self.h, str(path).encode()), "write_tsv")
+    def write_tsv(self, path, host=False):
+        """Reference TSV rows of the last search; formatted on the device unless host=True."""
+        self._check(lib().cof_write_tsv_ex(self.h, str(path).encode(), COF_WRITE_HOST if host else 0), "write_tsv")
 
     def stats(self):
         s = CofStats()

@@ -38,6 +38,7 @@ struct cof_ctx {
     bool cached = false;
     cof::HitVec hits;
     cof::AuxVec aux;
+    cof::TsvFormatter fmt;
     cof_stats stats{};
     double load_ms = 0;
     bool load_device = false;
@@ -382,17 +383,38 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
     return COF_OK;
 }
 
-cof_status cof_write_tsv(cof_ctx* ctx, const char* path) {
+cof_status cof_write_tsv_ex(cof_ctx* ctx, const char* path, uint32_t flags) {
     if (!ctx || !path) return COF_E_INVALID;
     if (!ctx->cached) return COF_E_INVALID;
+    if (flags & ~COF_WRITE_HOST) return COF_E_INVALID;
+    auto t0 = std::chrono::steady_clock::now();
     FILE* f = strcmp(path, "-") == 0 ? stdout : fopen(path, "wb");
     if (!f) { ctx->err = std::string("cannot open ") + path; return COF_E_IO; }
     std::vector<char> buf(1 << 22);
     setvbuf(f, buf.data(), _IOFBF, buf.size());
-    cof::write_tsv(ctx->g, ctx->q, ctx->hits, ctx->aux, f);
-    if (f != stdout) fclose(f); else { fflush(f); setvbuf(f, nullptr, _IOLBF, 0); }
-    return COF_OK;
+    cof_status st = COF_OK;
+    try {
+        if (flags & COF_WRITE_HOST) {
+            cof::write_tsv(ctx->g, ctx->q, ctx->hits, ctx->aux, f);
+        } else {
+            HIPCHECK(hipSetDevice(ctx->device));
+            cof::write_tsv_device(ctx->g, ctx->q, ctx->hits, ctx->aux, ctx->d_genome, f, ctx->stream, ctx->fmt);
+        }
+    } catch (const cof::Error& e) {
+        st = fail(ctx, e);
+    }
+    if (f != stdout) {
+        if (fclose(f) != 0 && st == COF_OK) { ctx->err = std::string("cannot write ") + path; st = COF_E_IO; }
+    } else {
+        fflush(f);
+        setvbuf(f, nullptr, _IOLBF, 0);
+    }
+    ctx->stats.write_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats.write_device = (flags & COF_WRITE_HOST) ? 0 : 1;
+    return st;
 }
+
+cof_status cof_write_tsv(cof_ctx* ctx, const char* path) { return cof_write_tsv_ex(ctx, path, 0); }
 
 cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out) {
     if (!ctx || !out) return COF_E_INVALID;

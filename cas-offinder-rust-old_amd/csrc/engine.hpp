@@ -85,6 +85,41 @@ struct BulgeTune {
     const std::vector<float>* chunk_cost = nullptr;
 };
 
+// Device TSV formatter (csrc/format.hip): batch buffers kept across calls.
+struct TsvFormatter {
+    cof_hit* d_hits = nullptr;
+    size_t hits_cap = 0;
+    HitAux* d_aux = nullptr;
+    size_t aux_cap = 0;
+    unsigned long long* d_bsum = nullptr;   // per-workgroup row bytes
+    size_t bsum_cap = 0;
+    unsigned long long* d_boff = nullptr;   // their exclusive prefix + batch total
+    size_t boff_cap = 0;
+    uint8_t* d_out = nullptr;
+    size_t out_cap = 0;
+    uint8_t* d_pats = nullptr;
+    size_t pats_cap = 0;
+    uint8_t* d_blob = nullptr;
+    size_t blob_cap = 0;
+    uint8_t* d_names = nullptr;
+    size_t names_cap = 0;
+    uint64_t* d_name_off = nullptr;
+    size_t name_off_cap = 0;
+    uint32_t* d_name_len = nullptr;
+    size_t name_len_cap = 0;
+    uint8_t* h_out[2] = {nullptr, nullptr};  // pinned, double-buffered
+    size_t h_cap = 0;
+    unsigned long long* h_total = nullptr;   // pinned batch totals
+    hipEvent_t ev_out[2] = {nullptr, nullptr};
+    hipEvent_t ev_total = nullptr;
+    ~TsvFormatter();
+};
+// Writes the header and one reference row per hit (main.rs:130-247), rows
+// formatted on the device from the hits and the resident genome; returns the
+// number of rows.
+uint64_t write_tsv_device(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux,
+                          const uint32_t* d_genome, FILE* f, hipStream_t st, TsvFormatter& fm);
+
 // FASTA ingest on the device (csrc/ingest.hip): fills g (chunks, names, host
 // copy of the packed genome) and encodes into alloc(n_bytes, user); returns
 // false, with g empty, when the file's line layout is irregular.

@@ -129,6 +129,8 @@ typedef struct {
     double convert_ms;         /* host conversion of raw device hits into final hits */
     double load_ms;            /* wall time of the last genome file load */
     uint64_t load_device;      /* 1 if that load encoded the FASTA on the device */
+    double write_ms;           /* wall time of the last cof_write_tsv{,_ex} */
+    uint64_t write_device;     /* 1 if those rows were formatted on the device */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes
@@ -203,8 +205,13 @@ cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes
 cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
 
 /* Writes the cached hits of the last search as reference TSV rows, header
- * included (main.rs:130-247).  path "-" = stdout. */
+ * included (main.rs:130-247).  path "-" = stdout.  The rows are formatted on
+ * the device from the hits and the resident genome and streamed to the file. */
 cof_status cof_write_tsv(cof_ctx* ctx, const char* path);
+/* The same with flags: COF_WRITE_HOST formats the rows on the host (one
+ * thread, the reference writer's shape).  Both produce identical bytes. */
+#define COF_WRITE_HOST 1u
+cof_status cof_write_tsv_ex(cof_ctx* ctx, const char* path, uint32_t flags);
 
 cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out);
 

@@ -27,5 +27,7 @@ def test_gpu_matches_golden(tmp_path, case):
     s.load_synth(make_golden.make_genome(case["genome"]))
     s.search()
     s.write_tsv(tmp_path / "gpu.tsv")
+    s.write_tsv(tmp_path / "host.tsv", host=True)
     s.close()
     assert cofpy.read_rows(tmp_path / "gpu.tsv") == want
+    assert open(tmp_path / "gpu.tsv", "rb").read() == open(tmp_path / "host.tsv", "rb").read()
