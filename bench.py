@@ -224,23 +224,29 @@ def apply_tuning(s, args):
 
 
 def ingest_workload(args):
-    """Genome file -> resident genome: host reader vs device FASTA encoder on the headline genome."""
+    """Genome file -> resident genome: host readers vs device FASTA encoder and 2bit decoder on the headline genome."""
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)
     sg = cofpy.SynthGenome(args.seed, lens, n_frac=0.05, gap=(10_000, 1_000_000))
     with tempfile.TemporaryDirectory() as d:
         fa = os.path.join(d, "genome.fa")
+        tb = os.path.join(d, "genome.2bit")
         t = time.time()
         sg.write_fasta(fa)
         size = os.path.getsize(fa)
         log(f"wrote {size / 1e9:.2f} GB FASTA in {time.time() - t:.1f} s")
+        t = time.time()
+        sg.write_2bit(tb)
+        size2 = os.path.getsize(tb)
+        log(f"wrote {size2 / 1e9:.2f} GB 2bit in {time.time() - t:.1f} s")
         res = {}
-        for name, flags in (("host", cofpy.COF_LOAD_HOST), ("device", cofpy.COF_LOAD_DEVICE)):
+        for name, path, flags in (("host", fa, cofpy.COF_LOAD_HOST), ("device", fa, cofpy.COF_LOAD_DEVICE),
+                                  ("2bit_host", tb, cofpy.COF_LOAD_HOST), ("2bit_device", tb, cofpy.COF_LOAD_DEVICE)):
             times = []
             for k in range(args.warmup + args.steps):
                 s = cofpy.Search(["A" * 20 + "NGG"], ["N" * 20 + "NGG"], [0])
                 t0 = time.perf_counter()
-                s.load_file(fa, flags)
+                s.load_file(path, flags)
                 dt = time.perf_counter() - t0
                 st = s.stats()
                 s.close()
@@ -254,7 +260,9 @@ def ingest_workload(args):
            "data": "synthetic",
            "config": {"workload": f"{total / 1e9:.2f} Gbp FASTA, 60-nt lines, 24 chromosomes, 5% N", "bytes": size},
            "host_reader_s": res["host"], "device_s": res["device"],
-           "speedup": round(res["host"] / res["device"], 2)}
+           "speedup": round(res["host"] / res["device"], 2),
+           "twobit": {"bytes": size2, "host_reader_s": res["2bit_host"], "device_s": res["2bit_device"],
+                      "speedup": round(res["2bit_host"] / res["2bit_device"], 2)}}
     print(json.dumps(out), flush=True)
 
 

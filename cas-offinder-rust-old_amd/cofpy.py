@@ -159,6 +159,7 @@ def synth_lib():
         S.cofsynth_chrom_name.argtypes = [C.c_void_p, C.c_uint32]
         S.cofsynth_chrom_name.restype = C.c_char_p
         S.cofsynth_write_fasta.argtypes = [C.c_void_p, C.c_char_p]
+        S.cofsynth_write_2bit.argtypes = [C.c_void_p, C.c_char_p]
         S.cofsynth_free.argtypes = [C.c_void_p]
         _synth = S
     return _synth
@@ -187,6 +188,11 @@ class SynthGenome:
 
     def write_fasta(self, path):
         if synth_lib().cofsynth_write_fasta(self.h, str(path).encode()) != 0:
+            raise IOError(path)
+
+    def write_2bit(self, path):
+        """.2bit file of the genome (N runs as N blocks; needs a genome without other IUPAC letters)."""
+        if synth_lib().cofsynth_write_2bit(self.h, str(path).encode()) != 0:
             raise IOError(path)
 
     def layout(self):

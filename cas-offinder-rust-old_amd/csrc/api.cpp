@@ -82,32 +82,38 @@ void upload_genome(cof_ctx* ctx) {
     finish_genome(ctx);
 }
 
-bool is_fasta_file(const char* path) {
+// First byte of a regular file (-1: not a regular file or empty): '>' is
+// FASTA, anything else is read as 2bit (main.rs:106-121).
+int first_byte(const char* path) {
     struct stat sb;
-    if (stat(path, &sb) != 0 || !S_ISREG(sb.st_mode)) return false;
+    if (stat(path, &sb) != 0 || !S_ISREG(sb.st_mode)) return -1;
     FILE* f = fopen(path, "rb");
-    if (!f) return false;
+    if (!f) return -1;
     int c = fgetc(f);
     fclose(f);
-    return c == '>';
+    return c;
 }
 
-// genome file -> resident genome: device ingest for regular FASTA files unless
-// the host reader is asked for; the host reader for 2bit, folders and the rest
+// genome file -> resident genome: device ingest for FASTA files with regular
+// line widths and for 2bit files unless the host reader is asked for; the
+// host reader for folders and irregular FASTA files
 cof_status load_file(cof_ctx* ctx, const char* path, uint32_t flags) {
     if (!ctx || !path) return COF_E_INVALID;
     try {
         HIPCHECK(hipSetDevice(ctx->device));
         auto t0 = std::chrono::steady_clock::now();
         bool device = false;
-        if (!(flags & COF_LOAD_HOST) && is_fasta_file(path)) {
+        const int c = (flags & COF_LOAD_HOST) ? -1 : first_byte(path);
+        if (c >= 0) {
             auto alloc = [](uint64_t n, void* u) { return alloc_genome((cof_ctx*)u, n); };
-            device = cof::read_fasta_device(path, ctx->g, alloc, ctx, ctx->stream);
+            device = c == '>' ? cof::read_fasta_device(path, ctx->g, alloc, ctx, ctx->stream)
+                              : cof::read_2bit_device(path, ctx->g, alloc, ctx, ctx->stream);
             if (device) finish_genome(ctx);
         }
         if (!device) {
             if (flags & COF_LOAD_DEVICE)
-                throw cof::Error(COF_E_UNSUPPORTED, "device ingest needs a FASTA file with regular line widths");
+                throw cof::Error(COF_E_UNSUPPORTED,
+                                 "device ingest needs a 2bit file or a FASTA file with regular line widths");
             cof::read_genome_file(path, ctx->g);
             upload_genome(ctx);
         }

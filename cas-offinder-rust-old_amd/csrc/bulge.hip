@@ -432,11 +432,6 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     bool drained = false;
 
     auto bit = [](WORD x, uint32_t k) -> uint32_t { return (uint32_t)(x >> k) & 1u; };
-    auto get2 = [&](uint32_t dd) -> uint32_t {
-        uint64_t w = ops[0];
-        if constexpr (kOpsW > 1) w = dd < 32 ? ops[0] : ops[1];
-        return (uint32_t)(w >> (2 * (dd & 31))) & 3u;
-    };
     auto set2 = [&](uint32_t dd, uint32_t v) {
         uint64_t m = ~(3ull << (2 * (dd & 31))), b = (uint64_t)v << (2 * (dd & 31));
         if constexpr (kOpsW > 1) {

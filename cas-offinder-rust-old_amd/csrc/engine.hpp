@@ -125,6 +125,11 @@ uint64_t write_tsv_device(const Genome& g, const Query& q, const HitVec& hits, c
 // false, with g empty, when the file's line layout is irregular.
 bool read_fasta_device(const std::string& path, Genome& g, uint32_t* alloc(uint64_t n_bytes, void* user), void* user,
                        hipStream_t st);
+// 2bit ingest on the device (csrc/ingest.hip): the host walks the record
+// headers and N blocks, the file streams to the device and one kernel decodes
+// every 64 Knt piece into alloc(n_bytes, user).  Always returns true.
+bool read_2bit_device(const std::string& path, Genome& g, uint32_t* alloc(uint64_t n_bytes, void* user), void* user,
+                      hipStream_t st);
 
 struct PlainPlan;
 PlainPlan* plain_plan_create(const Query& q);

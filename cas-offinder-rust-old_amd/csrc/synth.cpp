@@ -339,6 +339,62 @@ int cofsynth_write_fasta(const cofsynth* s, const char* path) {
     return 0;
 }
 
+// Writes the genome as a version-0 .2bit file: runs of N become N blocks,
+// A/C/G/T pack at 2 bits per base (T C A G = 0 1 2 3, first base in the top
+// bits); other IUPAC letters pack as T, as in UCSC's converter, and so do
+// not read back identically (use a genome without them).
+int cofsynth_write_2bit(const cofsynth* s, const char* path) {
+    FILE* f = fopen(path, "wb");
+    if (!f) return -1;
+    std::vector<char> buf(1 << 22);
+    setvbuf(f, buf.data(), _IOFBF, buf.size());
+    auto put32 = [&](uint32_t v) { fwrite(&v, 4, 1, f); };
+    const uint32_t n = (uint32_t)s->lens.size();
+    put32(0x1A412743u);
+    put32(0);
+    put32(n);
+    put32(0);
+    // N blocks first: the index needs every record's size
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> blocks(n);
+    uint64_t off = 16;
+    for (uint32_t c = 0; c < n; c++) off += 1 + s->names[c].size() + 4;
+    std::vector<uint32_t> rec_off(n);
+    for (uint32_t c = 0; c < n; c++) {
+        for (uint64_t i = 0; i < s->lens[c];) {
+            if (s->get(c, i) != 0xF) { i++; continue; }
+            uint64_t j = i;
+            while (j < s->lens[c] && s->get(c, j) == 0xF) j++;
+            blocks[c].push_back({(uint32_t)i, (uint32_t)(j - i)});
+            i = j;
+        }
+        rec_off[c] = (uint32_t)off;
+        off += 4 + 4 + 8 * blocks[c].size() + 4 + 4 + (s->lens[c] + 3) / 4;
+    }
+    for (uint32_t c = 0; c < n; c++) {
+        const uint8_t len = (uint8_t)s->names[c].size();
+        fwrite(&len, 1, 1, f);
+        fwrite(s->names[c].data(), 1, len, f);
+        put32(rec_off[c]);
+    }
+    std::vector<uint8_t> dna;
+    for (uint32_t c = 0; c < n; c++) {
+        put32((uint32_t)s->lens[c]);
+        put32((uint32_t)blocks[c].size());
+        for (auto& b : blocks[c]) put32(b.first);
+        for (auto& b : blocks[c]) put32(b.second);
+        put32(0);   // mask blocks
+        put32(0);   // reserved
+        dna.assign((s->lens[c] + 3) / 4, 0);
+        for (uint64_t i = 0; i < s->lens[c]; i++) {
+            const uint8_t v = s->get(c, i);
+            const uint8_t code = v == 2 ? 1 : v == 4 ? 2 : v == 8 ? 3 : 0;
+            dna[i >> 2] |= (uint8_t)(code << (6 - 2 * (i & 3)));
+        }
+        fwrite(dna.data(), 1, dna.size(), f);
+    }
+    return fclose(f) == 0 ? 0 : -1;
+}
+
 void cofsynth_free(cofsynth* s) { delete s; }
 
 }  // extern "C"

@@ -178,11 +178,12 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q);
 cof_status cof_load_genome_file(cof_ctx* ctx, const char* path);
 
 /* cof_load_genome_file with a choice of reader: COF_LOAD_HOST forces the
- * host reader; COF_LOAD_DEVICE requires the device FASTA encoder
- * (COF_E_UNSUPPORTED for 2bit files, folders or FASTA files whose lines in a
- * chunk do not share one width); 0 = device for regular FASTA, else host.
- * The result is the same genome either way.  No reference counterpart (the
- * reference reads on one host thread, read_fasta.rs:9-71). */
+ * host reader; COF_LOAD_DEVICE requires device ingest (the FASTA encoder or
+ * the 2bit decoder; COF_E_UNSUPPORTED for folders or FASTA files whose lines
+ * in a chunk do not share one width); 0 = device for 2bit and regular FASTA,
+ * else host.  The result is the same genome either way.  No reference
+ * counterpart (the reference reads on one host thread, read_fasta.rs:9-71,
+ * read_2bit.rs:33-118). */
 #define COF_LOAD_HOST 1u
 #define COF_LOAD_DEVICE 2u
 cof_status cof_load_genome_file_ex(cof_ctx* ctx, const char* path, uint32_t flags);

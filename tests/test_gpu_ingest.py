@@ -1,11 +1,13 @@
-"""Device FASTA ingest (cof_load_genome_file_ex) gives the same genome as the host reader.
+"""Device FASTA and 2bit ingest (cof_load_genome_file_ex) give the same genome as the host readers.
 
 The host reader restates read_fasta.rs:9-71 (pinned by the reference's
 upstream1000 fixtures in test_oracle.py).  Here FASTA files with several line
 shapes are loaded both ways and searched with a dense all-N query whose rows
 print the genome text around every hit, plus the reference query and
 fixtures; irregular files fall back to the host reader (or are refused when
-the device path is required).
+the device path is required).  2bit files (the reference fixture and
+synthetic files with N-block edge cases, read_2bit.rs:33-118) are decoded on
+the device and compared with the host reader and the oracle.
 """
 import os
 import random
@@ -94,3 +96,46 @@ def test_reference_fixture_both_ways(tmp_path):
     h, st_h = _rows(fa, cofpy.COF_LOAD_HOST, guides, "N" * 24, 11, tmp_path, "h")
     d, st_d = _rows(fa, cofpy.COF_LOAD_DEVICE, guides, "N" * 24, 11, tmp_path, "d")
     assert len(h) == 117 and d == h and st_d.load_device == 1
+
+
+def test_reference_2bit_fixture_both_ways(tmp_path):
+    # the same query on upstream1000.2bit: device 2bit decode == host reader, 117 rows
+    tb = os.path.join(GOLDEN, "upstream1000.2bit")
+    guides = ["CCGTGGTTCAACATTTGCTTAGCA", "GATGTTGGTAAGTGGGATATGGCA"]
+    h, st_h = _rows(tb, cofpy.COF_LOAD_HOST, guides, "N" * 24, 11, tmp_path, "h")
+    d, st_d = _rows(tb, cofpy.COF_LOAD_DEVICE, guides, "N" * 24, 11, tmp_path, "d")
+    assert len(h) == 117 and d == h and st_d.load_device == 1 and st_h.load_device == 0
+
+
+def test_2bit_device_decode_edge_cases(tmp_path):
+    # pieces of 64 Knt with N blocks across, on and at the ends of piece
+    # boundaries, unsorted and overlapping blocks (the reader's one-block-back
+    # walk), mask blocks, an empty and a 3-nt chromosome
+    import oracle
+    import twobit
+    from test_twobit_cpu import P, _records
+    rng = random.Random(17)
+    recs = _records(21) + [("chrF", "".join(rng.choice("ACGT") for _ in range(2 * P + 77)),
+                            [(P + 100, 50), (P - 10, 200), (5, 10), (2 * P + 70, 7)], [(3, 4)])]
+    path = tmp_path / "e.2bit"
+    twobit.write_2bit(path, recs)
+    _compare(path, tmp_path, 11)
+    guides, filt, mm = _dense_query(11)
+    got, _ = _rows(path, 0, guides, filt, mm, tmp_path, "o")
+    oracle.run(f"{path}\n{filt}\n" + "".join(f"{g} {mm}\n" for g in guides), tmp_path / "ref.tsv", batch_nucl=1 << 22)
+    assert got == cofpy.read_rows(tmp_path / "ref.tsv")
+
+
+def test_2bit_truncated_file_is_an_error(tmp_path):
+    import twobit
+    path = tmp_path / "t.2bit"
+    twobit.write_2bit(path, [("c1", "ACGT" * 1000, [], [])])
+    data = open(path, "rb").read()
+    open(path, "wb").write(data[:-10])
+    guides, filt, mm = _dense_query(3)
+    for flags in (0, cofpy.COF_LOAD_HOST):
+        s = cofpy.Search(guides, [filt] * 3, [mm] * 3)
+        with pytest.raises(cofpy.CofError) as e:
+            s.load_file(path, flags)
+        assert e.value.status == cofpy.COF_E_IO
+        s.close()
