@@ -19,7 +19,13 @@
 //      passing starts are compacted into an LDS list by a block-wide scan;
 //   3. each lane takes one compacted start, extracts its compare window with
 //      v_alignbit funnel shifts, and loops over the group's patterns (uniform
-//      scalar loads) doing AND + v_bcnt per 32-bit word.
+//      scalar loads).  Groups whose patterns are plain bases on every position
+//      that is not N in all of them compare 2-bit planes when the tile holds
+//      only plain bases (no masked N, no padding, no IUPAC letter): the tile is
+//      also kept as two bit planes (base bit 1, base bit 0), and a window of up
+//      to 32 positions against one pattern is xor + bitop3 + v_bcnt + one
+//      compare.  Other tiles and groups AND the bit4 words and v_bcnt them (W
+//      words per pattern).
 // Without IUPAC letters in the window, genome single bases make the writer's
 // recount equal kmm minus the genome-N cells under pattern N, so the single
 // popcount threshold is exact; IUPAC tiles take an exact per-nibble path.
@@ -45,8 +51,19 @@ constexpr int kMaxPam = 64;
 struct GroupDev {
     uint32_t pat_begin, pat_count;
     uint32_t L, k_lo, n_pam, W;
+    // 2-bit plane compare: window from pk_lo, wmask = positions that are not
+    // N in every pattern
+    uint32_t planes, pk_lo, wmask, pad;
     uint32_t pam_pos[kMaxPam];
     uint32_t pam_mask[kMaxPam];   // filter code in every nibble
+};
+
+// One pattern as two bit planes over the group's plane window (base T C A G =
+// planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and its M.
+struct __attribute__((aligned(16))) PlaneRec {
+    uint32_t ph, pl;
+    uint32_t M;
+    uint32_t pid;
 };
 
 // One pattern of a group: its compare window (the pattern's nibbles from
@@ -72,13 +89,14 @@ __device__ __forceinline__ uint32_t multi_nibbles(uint32_t w) {
     uint32_t o = (w | (w >> 1)) & 0x55555555u;
     return ((x | (x >> 2) | (o & (o >> 2))) & 0x11111111u) << 3;
 }
-// bit 3 of nibble j -> bit j
-__device__ __forceinline__ uint32_t gather_nibble_flags(uint32_t f) {
-    uint32_t m = f >> 3;
+// bit 0 of nibble j -> bit j (other bits zero)
+__device__ __forceinline__ uint32_t gather_nibble_bit0(uint32_t m) {
     m = (m | (m >> 3)) & 0x03030303u;
     m = (m | (m >> 6)) & 0x000F000Fu;
     return (m | (m >> 12)) & 0xFFu;
 }
+// bit 3 of nibble j -> bit j
+__device__ __forceinline__ uint32_t gather_nibble_flags(uint32_t f) { return gather_nibble_bit0(f >> 3); }
 
 // Exact writer recount (main.rs:184-187) of one window: pattern positions
 // that are not N and do not hold a plain base of the pattern's set, computed
@@ -108,13 +126,17 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
     const GroupDev* __restrict__ groups, uint32_t n_groups,
     const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
+    const PlaneRec* __restrict__ prec,        // [pattern] 2-bit planes (groups with planes)
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
     PlainRaw* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
-    uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
+    uint32_t pat_chunk,                       // patterns per grid row (0: whole group)
+    uint32_t use_planes) {                    // some group of the launch compares planes
     constexpr int kTile = THREADS * 64;
     constexpr int kTileWords = kTile / 8 + kHaloWords;
+    constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
     constexpr int kThreads = THREADS;
     __shared__ uint32_t gw[kTileWords];
+    __shared__ uint32_t ghi[kPlaneWords], glo[kPlaneWords];
     __shared__ uint16_t cand[kTile];
     __shared__ uint32_t wave_sum[kThreads / 64];
 
@@ -124,15 +146,34 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint64_t s0 = first_tile_nib + (uint64_t)blockIdx.x * kTile;
     const uint64_t w0 = s0 >> 3;
 
-    // 1. genome tile -> LDS (N-masked), IUPAC detection
-    int iupac = 0;
+    // 1. genome tile -> LDS (N-masked), IUPAC and zero-nibble detection
+    int flags = 0;   // bit 0: an IUPAC letter, bit 1: a zero nibble (masked N, padding)
     for (uint32_t i = tid; i < (uint32_t)kTileWords; i += kThreads) {
         uint32_t w = genome[w0 + i];
         uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
-        iupac |= (multi_nibbles(w) & ~(f << 3)) != 0;
-        gw[i] = w & ~(f * 0xFu);
+        w &= ~(f * 0xFu);
+        flags |= (multi_nibbles(w) != 0 ? 1 : 0) | (nz_nibbles(w) != 0x88888888u ? 2 : 0);
+        gw[i] = w;
     }
-    iupac = __syncthreads_or(iupac);
+    flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    const int iupac = flags & 1;
+    const bool plain_tile = use_planes && flags == 0;
+    if (plain_tile) {
+        // 32 positions per plane word: base bit 1 (A, G) and base bit 0 (C, G)
+        for (uint32_t j = tid; j < (uint32_t)kPlaneWords; j += kThreads) {
+            uint32_t h = 0, l = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; b++) {
+                const uint32_t idx = 4 * j + b;
+                const uint32_t w = idx < (uint32_t)kTileWords ? gw[idx] : 0u;
+                h |= gather_nibble_bit0(((w >> 2) | (w >> 3)) & 0x11111111u) << (8 * b);
+                l |= gather_nibble_bit0(((w >> 1) | (w >> 3)) & 0x11111111u) << (8 * b);
+            }
+            ghi[j] = h;
+            glo[j] = l;
+        }
+        __syncthreads();
+    }
 
     // Thread tid owns tile words w * kThreads + tid (w < 8), 8 starts each:
     // consecutive lanes read consecutive LDS words (no bank conflicts).
@@ -203,11 +244,13 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         }
         __syncthreads();
         // 3. compare windows against the group's patterns
+        const bool planes = plain_tile && G.planes;   // uniform
         for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
             uint32_t c = c0 + lane;
             bool active = c < nc;
             uint32_t i = active ? cand[c] : 0;
-            if (!iupac) {
+            // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
+            auto bit4_compare = [&](bool act) {
                 uint32_t q = i + G.k_lo;
                 uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
                 uint32_t win[W];
@@ -227,7 +270,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                         for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
                         hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
                     }
-                    if (active && hit) {
+                    if (act && hit) {
                         for (int u = 0; u < 4; u++) {
                             if (!((hit >> u) & 1u)) continue;
                             const PatRec& r = pats[p + u];
@@ -242,12 +285,53 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                     uint32_t sum = 0;
 #pragma unroll
                     for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
-                    if (active && (int32_t)sum >= r.thr) {
+                    if (act && (int32_t)sum >= r.thr) {
                         uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
                         uint32_t idx = atomicAdd(out_count, 1u);
                         if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                     }
                 }
+            };
+            if (planes) {
+                // plain bases only: kmm = window positions whose planes differ
+                // (positions that are N in every pattern match any base)
+                const uint32_t q = i + G.pk_lo;
+                const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
+                const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                const bool act = active;
+                const uint64_t act_mask = __ballot(act);
+                uint32_t p = pb;
+                for (; p + 8 <= pe; p += 8) {
+                    const PlaneRec* const rp = prec + p;   // uniform base
+                    // per pattern: xor, bitop3, v_bcnt, v_cmp into a wave mask;
+                    // the masks are ORed on the scalar unit
+                    uint64_t any = 0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
+                        any |= __ballot((uint32_t)__popc(d) <= rp[u].M);
+                    }
+                    if (any & act_mask) {
+                        for (int u = 0; u < 8; u++) {
+                            const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
+                            if (!(act && (uint32_t)__popc(d) <= rp[u].M)) continue;
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                            uint32_t idx = atomicAdd(out_count, 1u);
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
+                        }
+                    }
+                }
+                for (; p < pe; p++) {
+                    const PlaneRec& r = prec[p];
+                    const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
+                    if (act && (uint32_t)__popc(d) <= r.M) {
+                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
+                        uint32_t idx = atomicAdd(out_count, 1u);
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                    }
+                }
+            } else if (!iupac) {
+                bit4_compare(active);
             } else {
                 for (uint32_t p = pb; p < pe; p++) {
                     const uint32_t* fw = full_words + (size_t)p * 8;
@@ -280,11 +364,13 @@ struct PlainPlan {
         int W;
         uint32_t group_begin, n_groups;
         uint32_t max_pats;         // largest group of the launch
+        uint32_t planes;           // some group compares 2-bit planes
     };
     std::vector<GroupDev> groups;
     std::vector<Launch> launches;
     GroupDev* d_groups = nullptr;
     PatRec* d_pats = nullptr;
+    PlaneRec* d_prec = nullptr;
     uint32_t* d_full = nullptr;
     uint32_t max_L = 0;
 };
@@ -302,6 +388,7 @@ PlainPlan* plain_plan_create(const Query& q) {
         by_key[{p.L, f}].push_back(i);
     }
     std::vector<PatRec> recs;
+    std::vector<PlaneRec> precs;
     std::vector<uint32_t> full;
     std::map<int, std::vector<GroupDev>> by_w;
     uint32_t pat_index = 0;
@@ -335,6 +422,25 @@ PlainPlan* plain_plan_create(const Query& q) {
         G.W = (k_hi - k_lo + 7) / 8;
         G.pat_begin = pat_index;
         G.pat_count = (uint32_t)kv.second.size();
+        // 2-bit planes: positions that are N in every pattern of the group
+        // leave the window; the rest must be plain bases in every pattern and
+        // fit one 32-bit word
+        std::vector<bool> all_n(L, true);
+        for (uint32_t pi : kv.second)
+            for (uint32_t k = 0; k < L; k++) all_n[k] = all_n[k] && q.pats[pi].b4[k] == 0xF;
+        uint32_t pk_lo = L, pk_hi = 0;
+        for (uint32_t k = 0; k < L; k++)
+            if (!all_n[k]) { pk_lo = std::min(pk_lo, k); pk_hi = std::max(pk_hi, k + 1); }
+        if (pk_lo >= pk_hi) pk_lo = pk_hi = 0;
+        bool planes = pk_hi - pk_lo <= 32;
+        for (uint32_t pi : kv.second)
+            for (uint32_t k = pk_lo; planes && k < pk_hi; k++)
+                if (!all_n[k] && !single_base(q.pats[pi].b4[k])) planes = false;
+        G.planes = planes ? 1u : 0u;
+        G.pk_lo = pk_lo;
+        G.wmask = 0;
+        for (uint32_t k = pk_lo; k < pk_hi; k++)
+            if (!all_n[k]) G.wmask |= 1u << (k - pk_lo);
         for (uint32_t pi : kv.second) {
             const Pattern& p = q.pats[pi];
             uint32_t cw[8] = {0}, fw[8] = {0};
@@ -350,15 +456,27 @@ PlainPlan* plain_plan_create(const Query& q) {
             r.M = p.M;
             r.pid = pi;
             recs.push_back(r);
+            PlaneRec pr{0, 0, p.M, pi};
+            if (planes)
+                for (uint32_t k = pk_lo; k < pk_hi; k++) {
+                    if (all_n[k]) continue;
+                    const uint32_t v = p.b4[k];   // T C A G = 1 2 4 8 -> (0,0) (0,1) (1,0) (1,1)
+                    pr.ph |= (uint32_t)((v & 0xC) != 0) << (k - pk_lo);
+                    pr.pl |= (uint32_t)((v & 0xA) != 0) << (k - pk_lo);
+                }
+            precs.push_back(pr);
             full.insert(full.end(), fw, fw + 8);
             pat_index++;
         }
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
-        uint32_t mp = 0;
-        for (const GroupDev& G : kv.second) mp = std::max(mp, G.pat_count);
-        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp};
+        uint32_t mp = 0, any_planes = 0;
+        for (const GroupDev& G : kv.second) {
+            mp = std::max(mp, G.pat_count);
+            any_planes |= G.planes;
+        }
+        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp, any_planes};
         pp->groups.insert(pp->groups.end(), kv.second.begin(), kv.second.end());
         pp->launches.push_back(l);
     }
@@ -370,6 +488,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             HIPCHECK(hipMemcpy(dptr, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
         };
         up(pp->d_pats, recs);
+        up(pp->d_prec, precs);
         up(pp->d_full, full);
     }
     return pp;
@@ -379,6 +498,7 @@ void plain_plan_destroy(PlainPlan* pp) {
     if (!pp) return;
     (void)hipFree(pp->d_groups);
     (void)hipFree(pp->d_pats);
+    (void)hipFree(pp->d_prec);
     (void)hipFree(pp->d_full);
     delete pp;
 }
@@ -392,27 +512,18 @@ namespace {
 
 template <int W>
 void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
-                  uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const uint32_t* full,
-                  PlainRaw* out, uint32_t* count, uint32_t cap) {
+                  uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const PlaneRec* prec,
+                  const uint32_t* full, PlainRaw* out, uint32_t* count, uint32_t cap, uint32_t planes) {
+#define COF_PLAIN_SHAPE(T)                                                                                  \
+    case T:                                                                                                 \
+        hipLaunchKernelGGL((plain_scan_kernel<W, T>), grid, dim3(T), 0, st, d_genome, scan_end, first, g, n_groups, \
+                           pats, prec, full, out, count, cap, t.pat_chunk, planes);                         \
+        break;
     switch (t.threads) {
-        case 64:
-            hipLaunchKernelGGL((plain_scan_kernel<W, 64>), grid, dim3(64), 0, st, d_genome, scan_end, first, g,
-                               n_groups, pats, full, out, count, cap, t.pat_chunk);
-            break;
-        case 128:
-            hipLaunchKernelGGL((plain_scan_kernel<W, 128>), grid, dim3(128), 0, st, d_genome, scan_end, first, g,
-                               n_groups, pats, full, out, count, cap, t.pat_chunk);
-            break;
-        case 256:
-            hipLaunchKernelGGL((plain_scan_kernel<W, 256>), grid, dim3(256), 0, st, d_genome, scan_end, first, g,
-                               n_groups, pats, full, out, count, cap, t.pat_chunk);
-            break;
-        case 512:
-            hipLaunchKernelGGL((plain_scan_kernel<W, 512>), grid, dim3(512), 0, st, d_genome, scan_end, first, g,
-                               n_groups, pats, full, out, count, cap, t.pat_chunk);
-            break;
+        COF_PLAIN_SHAPE(64) COF_PLAIN_SHAPE(128) COF_PLAIN_SHAPE(256) COF_PLAIN_SHAPE(512)
         default: throw Error(COF_E_INVALID, "plain workgroup size must be 64, 128, 256 or 512");
     }
+#undef COF_PLAIN_SHAPE
 }
 
 }  // namespace
@@ -496,8 +607,8 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
                 dim3 grid(nb, std::max(1u, ny));
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
-        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_full, d_out, \
-                         ws.plain_count, ws.plain_cap);                                                       \
+        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
+                         pp->d_full, d_out, ws.plain_count, ws.plain_cap, l.planes);                          \
         break;
                 switch (l.W) {
                     COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)

@@ -121,3 +121,44 @@ def test_overflow_retry_and_api(tmp_path):
     st = cofpy.lib().cof_search(s.h, buf, 2, C.byref(need))
     assert st == cofpy.COF_E_OVERFLOW and need.value == n
     s.close()
+
+
+@pytest.mark.parametrize("case", ["shared_n", "iupac_guide", "window32", "window33", "n_padded"])
+def test_plane_window_shapes(tmp_path, case):
+    # The scan compares 2-bit planes when a group's patterns are plain bases on
+    # a window of at most 32 positions (positions that are N in every pattern
+    # left out) and the tile holds only plain bases; other groups and tiles use
+    # the bit4 compare.  Each case puts one query on either side of that line;
+    # the genome mixes clean tiles with N gaps and odd chunks (zero padding).
+    import random
+    rng = random.Random(len(case) * 31)
+
+    def g(n):
+        return "".join(rng.choice("ACGT") for _ in range(n))
+
+    if case == "shared_n":          # an N at guide position 5 in every pattern: left out of the window
+        guides = [g(5) + "N" + g(14) for _ in range(9)]
+        filt, pats, mm = "N" * 21 + "RG", [x + "NNN" for x in guides], 3
+    elif case == "iupac_guide":     # one pattern with R: the whole group compares bit4
+        guides = [g(20) for _ in range(8)] + [g(7) + "R" + g(12)]
+        filt, pats, mm = "N" * 21 + "RG", [x + "NNN" for x in guides], 4
+    elif case == "window32":        # 32-position window, L = 35
+        guides = [g(32) for _ in range(5)]
+        filt, pats, mm = "N" * 33 + "RG", [x + "NNN" for x in guides], 12
+    elif case == "window33":        # 33 positions: bit4
+        guides = [g(33) for _ in range(5)]
+        filt, pats, mm = "N" * 34 + "RG", [x + "NNN" for x in guides], 12
+    else:                           # 5' PAM, guides padded with N on both sides
+        guides = [g(18) for _ in range(11)]
+        filt, pats, mm = "TTTN" + "N" * 20, ["NNNN" + x + "NN" for x in guides], 4
+    if case == "n_padded":
+        plants = [("TTTA" + x + "AC", 4, 18, 2, 20) for x in guides]
+    else:                           # guide letters sampled from their IUPAC sets, PAM AGG
+        plants = [(x + "AGG", 0, len(x), 2, 20) for x in guides]
+    sg = cofpy.SynthGenome(77 + len(case), [260_001, 131_071, 9_999], line_width=61, n_frac=0.02, gap=(30, 3000),
+                           plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    got, want, st = run_both(tmp_path, fa, filt, pats, mm)
+    assert len(want) > 0
+    assert got == want
