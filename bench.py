@@ -430,7 +430,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--genome-bp", type=float, default=3.1e9)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample-bp", type=float, default=3.2e8)
+    ap.add_argument("--cpu-sample-bp", type=float, default=8e8)
     ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv"], default="plain")

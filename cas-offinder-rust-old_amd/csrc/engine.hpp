@@ -12,9 +12,11 @@ namespace cof {
 // Device scratch shared by all searches of one context: grown on demand,
 // kept across searches and re-queries, freed with the context.
 struct Workspace {
-    void* plain_out = nullptr;     // PlainRaw records
+    void* plain_out = nullptr;     // PlainRaw records, 64 segments of plain_cap / 64
     uint32_t plain_cap = 0;
-    uint32_t* plain_count = nullptr;
+    uint32_t* plain_count = nullptr;   // per-segment counters, 128 bytes apart
+    void* plain_out2 = nullptr;    // the segments packed together
+    uint32_t plain_cap2 = 0;
     void* queue = nullptr;         // bulge candidates
     uint32_t q_cap = 0;
     void* bulge_out = nullptr;     // BulgeRaw records

@@ -47,6 +47,11 @@ constexpr int kMaxL = 64;                    // plain pattern length supported
 constexpr int kHaloWords = (kMaxL + 8) / 8 + 2;
 constexpr int kMaxTileWords = kMaxThreads * 64 / 8 + kHaloWords;
 constexpr int kMaxPam = 64;
+// Hit records go to 64 segments, one per workgroup index mod 64, each with
+// its own counter on its own 128-byte line: a single counter serialises the
+// atomics of hit-dense queries (about one per 7 ns for the whole GPU).
+constexpr uint32_t kHitSegs = 64;
+constexpr uint32_t kSegStride = 32;          // u32 between segment counters
 
 struct GroupDev {
     uint32_t pat_begin, pat_count;
@@ -59,10 +64,11 @@ struct GroupDev {
 };
 
 // One pattern as two bit planes over the group's plane window (base T C A G =
-// planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and its M.
+// planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and
+// K = 64 - M: a window hits when popcount(mismatch) + K <= 64.
 struct __attribute__((aligned(16))) PlaneRec {
     uint32_t ph, pl;
-    uint32_t M;
+    uint32_t K;
     uint32_t pid;
 };
 
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
     const PlaneRec* __restrict__ prec,        // [pattern] 2-bit planes (groups with planes)
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
-    PlainRaw* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
+    PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
     uint32_t pat_chunk,                       // patterns per grid row (0: whole group)
     uint32_t use_planes) {                    // some group of the launch compares planes
     constexpr int kTile = THREADS * 64;
@@ -145,6 +151,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t wave = tid >> 6;
     const uint64_t s0 = first_tile_nib + (uint64_t)blockIdx.x * kTile;
     const uint64_t w0 = s0 >> 3;
+    // this workgroup's hit segment
+    uint32_t* const out_count = counts + (blockIdx.x & (kHitSegs - 1)) * kSegStride;
+    PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
+    const uint32_t out_cap = seg_cap;
 
     // 1. genome tile -> LDS (N-masked), IUPAC and zero-nibble detection
     int flags = 0;   // bit 0: an IUPAC letter, bit 1: a zero nibble (masked N, padding)
@@ -303,18 +313,19 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 uint32_t p = pb;
                 for (; p + 8 <= pe; p += 8) {
                     const PlaneRec* const rp = prec + p;   // uniform base
-                    // per pattern: xor, bitop3, v_bcnt, v_cmp into a wave mask;
-                    // the masks are ORed on the scalar unit
-                    uint64_t any = 0;
+                    // per pattern: xor, bitop3, v_bcnt accumulating K, and
+                    // half a v_min3; one compare for the eight
+                    uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
-                        any |= __ballot((uint32_t)__popc(d) <= rp[u].M);
+                    for (int u = 0; u < 8; u += 2) {
+                        const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
+                        const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
+                        mn = min(mn, min(t0, t1));
                     }
-                    if (any & act_mask) {
+                    if (__ballot(mn <= 64u) & act_mask) {
                         for (int u = 0; u < 8; u++) {
                             const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
-                            if (!(act && (uint32_t)__popc(d) <= rp[u].M)) continue;
+                            if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
                             uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
                             uint32_t idx = atomicAdd(out_count, 1u);
                             if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
@@ -324,7 +335,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 for (; p < pe; p++) {
                     const PlaneRec& r = prec[p];
                     const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
-                    if (act && (uint32_t)__popc(d) <= r.M) {
+                    if (act && (uint32_t)__popc(d) + r.K <= 64u) {
                         uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
                         uint32_t idx = atomicAdd(out_count, 1u);
                         if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
@@ -355,6 +366,24 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         }
         __syncthreads();
     }
+}
+
+// Packs the hit segments into one array: segment s goes after segments
+// 0..s-1 (counts already checked against the segment capacity).
+__global__ __launch_bounds__(256) void plain_compact_kernel(const PlainRaw* __restrict__ in,
+                                                            const uint32_t* __restrict__ counts, uint32_t seg_cap,
+                                                            PlainRaw* __restrict__ out) {
+    __shared__ uint32_t off;
+    const uint32_t seg = blockIdx.x;
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (uint32_t t = 0; t < seg; t++) o += counts[t * kSegStride];
+        off = o;
+    }
+    __syncthreads();
+    const uint32_t n = min(counts[seg * kSegStride], seg_cap);
+    const PlainRaw* src = in + (size_t)seg * seg_cap;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[off + i] = src[i];
 }
 
 }  // namespace
@@ -456,7 +485,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             r.M = p.M;
             r.pid = pi;
             recs.push_back(r);
-            PlaneRec pr{0, 0, p.M, pi};
+            PlaneRec pr{0, 0, 64u - std::min<uint32_t>(p.M, 64u), pi};
             if (planes)
                 for (uint32_t k = pk_lo; k < pk_hi; k++) {
                     if (all_n[k]) continue;
@@ -531,7 +560,7 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
 // Scans start positions [0, scan_end) of the resident genome.  Device words
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
 Workspace::~Workspace() {
-    for (void* p : {plain_out, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
+    for (void* p : {plain_out, plain_out2, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
                     qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
                     (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
                     (void*)round_ctr})
@@ -566,7 +595,7 @@ void Workspace::ensure_events() {
     for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipMalloc(&counters, 4 * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&sums, 8 * sizeof(unsigned long long)));
-    HIPCHECK(hipMalloc(&plain_count, sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&plain_count, kHitSegs * kSegStride * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&round_ctr, 2 * sizeof(uint32_t)));
     int dev = 0;
     HIPCHECK(hipGetDevice(&dev));
@@ -593,9 +622,11 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
     }
     const uint64_t kTile = (uint64_t)tune.threads * 64;
     uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
+    uint32_t counts[kHitSegs * kSegStride];
     for (;;) {
         PlainRaw* d_out = (PlainRaw*)ws.plain_out;
-        HIPCHECK(hipMemsetAsync(ws.plain_count, 0, sizeof(uint32_t), st));
+        const uint32_t seg_cap = ws.plain_cap / kHitSegs;
+        HIPCHECK(hipMemsetAsync(ws.plain_count, 0, sizeof counts, st));
         HIPCHECK(hipEventRecord(ws.ev[0], st));
         for (const auto& l : pp->launches) {
             // grids of at most 2^20 tiles keep the tile index in 32 bits
@@ -608,7 +639,7 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
         launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, d_out, ws.plain_count, ws.plain_cap, l.planes);                          \
+                         pp->d_full, d_out, ws.plain_count, seg_cap, l.planes);                               \
         break;
                 switch (l.W) {
                     COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
@@ -621,22 +652,38 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
             }
         }
         HIPCHECK(hipEventRecord(ws.ev[1], st));
-        uint32_t cnt = 0;
-        HIPCHECK(hipMemcpyAsync(&cnt, ws.plain_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(counts, ws.plain_count, sizeof counts, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
         if (kernel_ms) *kernel_ms += ms;
-        if (cnt > ws.plain_cap) {
-            // too many hits for the record buffer: grow it and scan again
+        uint64_t total = 0;
+        uint32_t most = 0;
+        for (uint32_t k = 0; k < kHitSegs; k++) {
+            total += counts[k * kSegStride];
+            most = std::max(most, counts[k * kSegStride]);
+        }
+        if (most > seg_cap) {
+            // a segment overflowed: grow every segment past the fullest one and scan again
+            const uint64_t need = ((uint64_t)most + most / 4 + 1024) * kHitSegs;
+            if (need > 0xFFFFFFF0ull) throw Error(COF_E_NOMEM, "plain search: too many hits");
             size_t cap = ws.plain_cap;
-            ws_grow(&ws.plain_out, &cap, std::min<uint64_t>((uint64_t)cnt + cnt / 4 + 1024, 0xFFFFFFF0ull),
-                    sizeof(PlainRaw));
+            ws_grow(&ws.plain_out, &cap, need, sizeof(PlainRaw));
             ws.plain_cap = (uint32_t)cap;
             continue;
         }
+        const uint32_t cnt = (uint32_t)total;
         ws.ensure_host(&ws.h_plain, &ws.h_plain_cap, (size_t)cnt * sizeof(PlainRaw));
-        if (cnt) HIPCHECK(hipMemcpyAsync(ws.h_plain, d_out, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
+        if (cnt) {
+            size_t cap2 = ws.plain_cap2;
+            ws_grow(&ws.plain_out2, &cap2, cnt, sizeof(PlainRaw));
+            ws.plain_cap2 = (uint32_t)cap2;
+            hipLaunchKernelGGL(plain_compact_kernel, dim3(kHitSegs), dim3(256), 0, st, d_out, ws.plain_count, seg_cap,
+                               (PlainRaw*)ws.plain_out2);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(ws.h_plain, ws.plain_out2, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost,
+                                    st));
+        }
         HIPCHECK(hipStreamSynchronize(st));
         out = RawView<PlainRaw>{(const PlainRaw*)ws.h_plain, cnt};
         return;
