@@ -67,6 +67,7 @@ uint32_t* alloc_genome(cof_ctx* ctx, uint64_t n_bytes) {
 }
 
 void finish_genome(cof_ctx* ctx) {
+    ctx->g.index();
     ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
     ctx->have_genome = true;
     ctx->cached = false;

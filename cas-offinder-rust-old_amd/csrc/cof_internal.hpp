@@ -64,7 +64,11 @@ struct Genome {
     std::vector<uint8_t> packed;   // host copy, + zero padding
     uint64_t n_bytes = 0;          // valid bytes
     uint64_t n_nibbles() const { return n_bytes * 2; }
-    // chunk containing global nibble i (upper_bound over byte offsets)
+    // lut[k] = chunk holding byte k << 15 (chunks hold at most 32 KB, so a
+    // bucket has few chunk starts); built by index() once the table is final
+    std::vector<uint32_t> lut;
+    void index();
+    // chunk containing global nibble i (the last chunk starting at or before it)
     size_t chunk_of(uint64_t nibble) const;
 };
 

@@ -36,9 +36,24 @@ std::string revcomp(const std::string& s) {
     return r;
 }
 
+void Genome::index() {
+    lut.assign((size_t)(n_bytes >> 15) + 2, 0);
+    size_t c = 0;
+    for (size_t k = 0; k < lut.size(); k++) {
+        const uint64_t byte = (uint64_t)k << 15;
+        while (c + 1 < chunks.size() && chunks[c + 1].byte_off <= byte) c++;
+        lut[k] = (uint32_t)c;
+    }
+}
+
 size_t Genome::chunk_of(uint64_t nibble) const {
-    uint64_t byte = nibble >> 1;
+    const uint64_t byte = nibble >> 1;
     size_t lo = 0, hi = chunks.size();
+    const uint64_t k = byte >> 15;
+    if (k + 1 < lut.size()) {                  // the chunks starting in this bucket
+        lo = lut[k];
+        hi = std::min<size_t>(chunks.size(), (size_t)lut[k + 1] + 1);
+    }
     while (lo < hi) {                          // first chunk with byte_off > byte
         size_t mid = (lo + hi) / 2;
         if (chunks[mid].byte_off <= byte) lo = mid + 1; else hi = mid;

@@ -15,16 +15,17 @@
 //      N-mask (0xF -> 0, search.rs:512-531) applied on the fly; a SWAR test
 //      flags tiles holding IUPAC genome letters (R, Y, ...);
 //   2. per pattern group (patterns sharing one effective filter), every thread
-//      tests the PAM of its 64 starts, 8 per word (SWAR over nibbles), and the
-//      passing starts are compacted into an LDS list by a block-wide scan;
+//      tests the PAM of its 64 starts and the passing starts are compacted
+//      into an LDS list by a block-wide scan.  Tiles of plain bases only (no
+//      masked N, padding or IUPAC letter) are also kept as two bit planes
+//      (base bit 1, base bit 0) and test a PAM cell on 32 starts with one
+//      v_bitop3; other tiles test 8 starts per word (SWAR over nibbles);
 //   3. each lane takes one compacted start, extracts its compare window with
 //      v_alignbit funnel shifts, and loops over the group's patterns (uniform
 //      scalar loads).  Groups whose patterns are plain bases on every position
-//      that is not N in all of them compare 2-bit planes when the tile holds
-//      only plain bases (no masked N, no padding, no IUPAC letter): the tile is
-//      also kept as two bit planes (base bit 1, base bit 0), and a window of up
-//      to 32 positions against one pattern is xor + bitop3 + v_bcnt + one
-//      compare.  Other tiles and groups AND the bit4 words and v_bcnt them (W
+//      that is not N in all of them compare the planes on plain-base tiles: a
+//      window of up to 32 positions against one pattern is xor + bitop3 +
+//      v_bcnt.  Other tiles and groups AND the bit4 words and v_bcnt them (W
 //      words per pattern).
 // Without IUPAC letters in the window, genome single bases make the writer's
 // recount equal kmm minus the genome-N cells under pattern N, so the single
@@ -95,6 +96,34 @@ __device__ __forceinline__ uint32_t multi_nibbles(uint32_t w) {
     uint32_t o = (w | (w >> 1)) & 0x55555555u;
     return ((x | (x >> 2) | (o & (o >> 2))) & 0x11111111u) << 3;
 }
+// acc & (the base with planes (h, l) is in the filter set `code`), 32
+// positions per operation: one v_bitop3 whose truth table is code << 4
+// (bit 4 + 2h + l set iff base (h, l) = T C A G -> code bit 0 1 2 3 is in
+// the set); `code` is uniform, so the switch is a scalar branch.
+template <uint32_t C>
+__device__ __forceinline__ uint32_t pam_cell_c(uint32_t acc, uint32_t h, uint32_t l) {
+    return __builtin_amdgcn_bitop3_b32(acc, h, l, (C << 4) & 0xF0u);
+}
+__device__ __forceinline__ uint32_t pam_cell(uint32_t acc, uint32_t h, uint32_t l, uint32_t code) {
+    switch (code) {
+        case 1: return pam_cell_c<1>(acc, h, l);
+        case 2: return pam_cell_c<2>(acc, h, l);
+        case 3: return pam_cell_c<3>(acc, h, l);
+        case 4: return pam_cell_c<4>(acc, h, l);
+        case 5: return pam_cell_c<5>(acc, h, l);
+        case 6: return pam_cell_c<6>(acc, h, l);
+        case 7: return pam_cell_c<7>(acc, h, l);
+        case 8: return pam_cell_c<8>(acc, h, l);
+        case 9: return pam_cell_c<9>(acc, h, l);
+        case 10: return pam_cell_c<10>(acc, h, l);
+        case 11: return pam_cell_c<11>(acc, h, l);
+        case 12: return pam_cell_c<12>(acc, h, l);
+        case 13: return pam_cell_c<13>(acc, h, l);
+        case 14: return pam_cell_c<14>(acc, h, l);
+        default: return acc;      // 15: any base
+    }
+}
+
 // bit 0 of nibble j -> bit j (other bits zero)
 __device__ __forceinline__ uint32_t gather_nibble_bit0(uint32_t m) {
     m = (m | (m >> 3)) & 0x03030303u;
@@ -135,8 +164,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const PlaneRec* __restrict__ prec,        // [pattern] 2-bit planes (groups with planes)
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
     PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
-    uint32_t pat_chunk,                       // patterns per grid row (0: whole group)
-    uint32_t use_planes) {                    // some group of the launch compares planes
+    uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
     constexpr int kTile = THREADS * 64;
     constexpr int kTileWords = kTile / 8 + kHaloWords;
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
@@ -167,7 +195,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     }
     flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
     const int iupac = flags & 1;
-    const bool plain_tile = use_planes && flags == 0;
+    const bool plain_tile = flags == 0;
     if (plain_tile) {
         // 32 positions per plane word: base bit 1 (A, G) and base bit 0 (C, G)
         for (uint32_t j = tid; j < (uint32_t)kPlaneWords; j += kThreads) {
@@ -185,8 +213,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         __syncthreads();
     }
 
-    // Thread tid owns tile words w * kThreads + tid (w < 8), 8 starts each:
-    // consecutive lanes read consecutive LDS words (no bank conflicts).
+    // Bit4 PAM test: thread tid owns tile words w * kThreads + tid (w < 8), 8
+    // starts each; plane PAM test: plane words w * kThreads + tid (w < 2), 32
+    // starts each.  Consecutive lanes read consecutive LDS words (no bank
+    // conflicts).
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const GroupDev& G = groups[gi];
@@ -197,34 +227,60 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             pb += lo;
             pe = min(pe, pb + pat_chunk);
         }
-        // 2. PAM test, 8 positions per word: nibble j of X_k is the genome at
-        //    start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble is
-        //    in the filter's plain-base set iff it overlaps the filter code.
-        uint32_t ok[8];
-#pragma unroll
-        for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
-        for (uint32_t k = 0; k < G.n_pam; k++) {
-            const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
-            const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                const uint32_t q = (uint32_t)w * kThreads + tid + qo;
-                const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
-                const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
-                ok[w] &= nz_nibbles(X & pmask) & ~bad;
-            }
-        }
+        // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
         uint32_t m8[8];
         uint32_t cnt = 0;
+        if (plain_tile) {
+            // planes: one v_bitop3 per PAM cell per 32 starts
+            uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+            for (uint32_t k = 0; k < G.n_pam; k++) {
+                const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
+                const uint32_t qo = pk >> 5, sh = pk & 31u;
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-            uint32_t m = gather_nibble_flags(ok[w]);
-            // the last tile ends at scan_end
-            const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
-            if (scan_end <= first) m = 0;
-            else if (scan_end < first + 8) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
-            m8[w] = m;
-            cnt += (uint32_t)__popc(m);
+                for (int w = 0; w < 2; w++) {
+                    const uint32_t q = (uint32_t)w * kThreads + tid + qo;
+                    const uint32_t h = __builtin_amdgcn_alignbit(ghi[q + 1], ghi[q], sh);
+                    const uint32_t l = __builtin_amdgcn_alignbit(glo[q + 1], glo[q], sh);
+                    ok[w] = pam_cell(ok[w], h, l, code);
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < 2; w++) {
+                uint32_t m = ok[w];
+                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
+                if (scan_end <= first) m = 0;
+                else if (scan_end < first + 32) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
+                m8[w] = m;
+                cnt += (uint32_t)__popc(m);
+            }
+        } else {
+            // bit4, 8 positions per word: nibble j of X_k is the genome at
+            // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
+            // is in the filter's plain-base set iff it overlaps the filter code.
+            uint32_t ok[8];
+#pragma unroll
+            for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
+            for (uint32_t k = 0; k < G.n_pam; k++) {
+                const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
+                const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    const uint32_t q = (uint32_t)w * kThreads + tid + qo;
+                    const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
+                    const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
+                    ok[w] &= nz_nibbles(X & pmask) & ~bad;
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                uint32_t m = gather_nibble_flags(ok[w]);
+                // the last tile ends at scan_end
+                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
+                if (scan_end <= first) m = 0;
+                else if (scan_end < first + 8) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
+                m8[w] = m;
+                cnt += (uint32_t)__popc(m);
+            }
         }
         // block-wide exclusive scan of the per-thread counts
         uint32_t v = cnt;
@@ -243,13 +299,25 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             nc += ws;
         }
         uint32_t pos = off + v - cnt;
+        if (plain_tile) {
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-            uint32_t m = m8[w];
-            while (m) {
-                uint32_t j = (uint32_t)__builtin_ctz(m);
-                cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
-                m &= m - 1u;
+            for (int w = 0; w < 2; w++) {
+                uint32_t m = m8[w];
+                while (m) {
+                    uint32_t j = (uint32_t)__builtin_ctz(m);
+                    cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 32u + j);
+                    m &= m - 1u;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                uint32_t m = m8[w];
+                while (m) {
+                    uint32_t j = (uint32_t)__builtin_ctz(m);
+                    cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
+                    m &= m - 1u;
+                }
             }
         }
         __syncthreads();
@@ -393,7 +461,6 @@ struct PlainPlan {
         int W;
         uint32_t group_begin, n_groups;
         uint32_t max_pats;         // largest group of the launch
-        uint32_t planes;           // some group compares 2-bit planes
     };
     std::vector<GroupDev> groups;
     std::vector<Launch> launches;
@@ -500,12 +567,9 @@ PlainPlan* plain_plan_create(const Query& q) {
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
-        uint32_t mp = 0, any_planes = 0;
-        for (const GroupDev& G : kv.second) {
-            mp = std::max(mp, G.pat_count);
-            any_planes |= G.planes;
-        }
-        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp, any_planes};
+        uint32_t mp = 0;
+        for (const GroupDev& G : kv.second) mp = std::max(mp, G.pat_count);
+        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp};
         pp->groups.insert(pp->groups.end(), kv.second.begin(), kv.second.end());
         pp->launches.push_back(l);
     }
@@ -542,11 +606,11 @@ namespace {
 template <int W>
 void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
                   uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const PlaneRec* prec,
-                  const uint32_t* full, PlainRaw* out, uint32_t* count, uint32_t cap, uint32_t planes) {
+                  const uint32_t* full, PlainRaw* out, uint32_t* count, uint32_t cap) {
 #define COF_PLAIN_SHAPE(T)                                                                                  \
     case T:                                                                                                 \
         hipLaunchKernelGGL((plain_scan_kernel<W, T>), grid, dim3(T), 0, st, d_genome, scan_end, first, g, n_groups, \
-                           pats, prec, full, out, count, cap, t.pat_chunk, planes);                         \
+                           pats, prec, full, out, count, cap, t.pat_chunk);                                 \
         break;
     switch (t.threads) {
         COF_PLAIN_SHAPE(64) COF_PLAIN_SHAPE(128) COF_PLAIN_SHAPE(256) COF_PLAIN_SHAPE(512)
@@ -639,7 +703,7 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
         launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, d_out, ws.plain_count, seg_cap, l.planes);                               \
+                         pp->d_full, d_out, ws.plain_count, seg_cap);                                         \
         break;
                 switch (l.W) {
                     COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)

@@ -139,46 +139,37 @@ private:
     bool stop_ = false;
 };
 
-// Runs fn(begin, end, out, aux) -> count over slices of [0, n) in parallel:
-// slice t writes its hits at out + base + begin (at most end - begin of
-// them), then the slices are moved together, in order, through a scratch copy.
+// Runs fn(begin, end, out, aux) -> count over slices of [0, n) in parallel,
+// twice: first with out == nullptr to count each slice's hits, then with
+// out/aux pointing at the slice's place in the final arrays (hits stay in
+// input order).  The converters' work per raw hit is a bucket lookup and a
+// few compares, so evaluating it twice costs less than staging 64-byte
+// records through a scratch copy.
 template <typename Fn>
 void parallel_convert(size_t n, HitVec& out, AuxVec& aux, Fn fn) {
     if (!n) return;
     const size_t base = out.size();
     Pool& pool = Pool::get();
-    const unsigned nt = n < 4096 ? 1u : pool.size();
-    static thread_local HitVec s_out;
-    static thread_local AuxVec s_aux;
-    s_out.resize(n);
-    s_aux.resize(n);
-    // the calling thread's scratch (a thread_local named inside the pool's
-    // tasks would be the worker's own)
-    cof_hit* const so = s_out.data();
-    HitAux* const sa = s_aux.data();
+    const unsigned nt = n < 4096 ? 1u : pool.size() * 4;
     std::vector<size_t> cnt(nt), off(nt + 1, 0);
     auto slice = [&](unsigned t) { return std::make_pair(n * t / nt, n * (t + 1) / nt); };
-    if (nt == 1) {
-        cnt[0] = fn(0, n, so, sa);
-    } else {
-        pool.run(nt, [&](unsigned t) {
-            auto [b0, b1] = slice(t);
-            cnt[t] = fn(b0, b1, so + b0, sa + b0);
-        });
-    }
+    auto each = [&](const std::function<void(unsigned)>& f) {
+        if (nt == 1) f(0);
+        else pool.run(nt, f);
+    };
+    each([&](unsigned t) {
+        auto [b0, b1] = slice(t);
+        cnt[t] = fn(b0, b1, nullptr, nullptr);
+    });
     for (unsigned t = 0; t < nt; t++) off[t + 1] = off[t] + cnt[t];
     out.resize(base + off[nt]);
     aux.resize(base + off[nt]);
-    auto move = [&](unsigned t) {
+    cof_hit* const o = out.data() + base;
+    HitAux* const a = aux.data() + base;
+    each([&](unsigned t) {
         auto [b0, b1] = slice(t);
-        (void)b1;
-        if (cnt[t]) {
-            std::memcpy(out.data() + base + off[t], so + b0, cnt[t] * sizeof(cof_hit));
-            std::memcpy(aux.data() + base + off[t], sa + b0, cnt[t] * sizeof(HitAux));
-        }
-    };
-    if (nt == 1) move(0);
-    else pool.run(nt, move);
+        if (cnt[t]) fn(b0, b1, o + off[t], a + off[t]);
+    });
 }
 }  // namespace
 
@@ -220,13 +211,15 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
             if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)b + 1]);
         }
         if (!emit) continue;
-        cof_hit h{};
-        h.chrom_pos = chrom_pos;
-        h.chrom = ch.chrom;
-        h.pattern = r.pattern;
-        h.mismatches = (uint16_t)mm;
-        out[k] = h;
-        aux[k] = HitAux{r.pos, view_end};
+        if (out) {
+            cof_hit h{};
+            h.chrom_pos = chrom_pos;
+            h.chrom = ch.chrom;
+            h.pattern = r.pattern;
+            h.mismatches = (uint16_t)mm;
+            out[k] = h;
+            aux[k] = HitAux{r.pos, view_end};
+        }
         k++;
     }
     return k;
@@ -254,17 +247,19 @@ void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs
         bool is_end_chrom = c + 1 == bs[b].last || g.chunks[c + 1].start == 0;
         bool past_end = chrom_pos + span > ch.end;
         if (is_end_chrom && past_end) continue;
-        cof_hit h{};
-        h.chrom_pos = chrom_pos;
-        h.chrom = ch.chrom;
-        h.pattern = r.pattern;
-        h.mismatches = r.mm;
-        h.dna_bulge = r.db;
-        h.rna_bulge = r.rb;
-        h.n_ops = r.n_ops;
-        ops_reverse(r.ops, r.n_ops, h.ops);      // device stack order -> alignment order
-        out[k] = h;
-        aux[k] = HitAux{r.start, bi.end_nib[b]};
+        if (out) {
+            cof_hit h{};
+            h.chrom_pos = chrom_pos;
+            h.chrom = ch.chrom;
+            h.pattern = r.pattern;
+            h.mismatches = r.mm;
+            h.dna_bulge = r.db;
+            h.rna_bulge = r.rb;
+            h.n_ops = r.n_ops;
+            ops_reverse(r.ops, r.n_ops, h.ops);      // device stack order -> alignment order
+            out[k] = h;
+            aux[k] = HitAux{r.start, bi.end_nib[b]};
+        }
         k++;
     }
     return k;
