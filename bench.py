@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--cpu-sample-bp", type=float, default=8e8)
     ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="plain workload: finish each run before starting the next")
     ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv"], default="plain")
     ap.add_argument("--tsv-dir", default=None, help="tsv: folder for the file-output timings (default: system temp)")
     ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
@@ -479,26 +481,40 @@ def main():
         ctxs.append(s)
     log(f"{len(ctxs)} contexts with the genome resident")
 
-    for w in range(args.warmup):
-        for s in ctxs:
-            s.search()
-        log(f"warmup {w + 1}/{args.warmup} done")
-
     kernel_ms = [0.0] * len(runs)
     search_ms = [0.0] * len(runs)
     convert_ms = [0.0] * len(runs)
     launches = [0] * len(runs)
-    step_s = []
     hits = [0] * len(runs)
+
+    def one_step(record):
+        # run i+1's kernels are queued before run i's hits are copied back and
+        # converted (cof_search_async / cof_search_wait), unless --no-overlap
+        def finish(i):
+            hits[i] = ctxs[i].wait() if not args.no_overlap else ctxs[i].search()
+            if record:
+                st = ctxs[i].stats()
+                kernel_ms[i] += st.kernel_ms_plain
+                search_ms[i] += st.search_ms
+                convert_ms[i] += st.convert_ms
+                launches[i] = st.n_plain_launches
+        for i, s in enumerate(ctxs):
+            if not args.no_overlap:
+                s.search_async()
+                if i:
+                    finish(i - 1)
+            else:
+                finish(i)
+        if not args.no_overlap:
+            finish(len(ctxs) - 1)
+
+    for w in range(args.warmup):
+        one_step(False)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    step_s = []
     for k in range(args.steps):
         t0 = time.perf_counter()
-        for i, s in enumerate(ctxs):
-            hits[i] = s.search()
-            st = s.stats()
-            kernel_ms[i] += st.kernel_ms_plain
-            search_ms[i] += st.search_ms
-            convert_ms[i] += st.convert_ms
-            launches[i] = st.n_plain_launches
+        one_step(True)
         step_s.append(time.perf_counter() - t0)
         log(f"step {k + 1}/{args.steps}: {step_s[-1] * 1e3:.1f} ms")
 
@@ -553,7 +569,8 @@ def main():
         "config": {"workload": f"plain GPU mismatch scan: {genome_nucl / 1e9:.2f} Gbp synthetic genome (24 chroms, "
                                "41% GC, 5% N), 100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both "
                                "strands, 1000 planted near-hits per guide; 12 reference-format runs per step",
-                   "genome_bp": genome_nucl, "guides_per_run": 100, "runs": len(runs)},
+                   "genome_bp": genome_nucl, "guides_per_run": 100, "runs": len(runs),
+                   "runs_overlapped": not args.no_overlap},
         "roofline": {"bound": "hbm", "achieved": round(hbm_achieved / 1e9, 1), "peak": round(HBM_PEAK / 1e9),
                      "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": "plain_scan_kernel", "launch_ms": round(launch_s * 1e3, 3),

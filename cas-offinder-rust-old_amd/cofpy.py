@@ -23,7 +23,8 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
                "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
-               "cof_load_genome_file_ex", "cof_device_count", "cof_device_name", "cof_write_tsv_ex"]
+               "cof_load_genome_file_ex", "cof_device_count", "cof_device_name", "cof_write_tsv_ex",
+               "cof_search_async", "cof_search_wait"]
 COF_WRITE_HOST = 1
 COF_LOAD_HOST = 1
 COF_LOAD_DEVICE = 2
@@ -102,6 +103,8 @@ def lib():
         L.cof_search.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
         L.cof_write_tsv.argtypes = [C.c_void_p, C.c_char_p]
         L.cof_write_tsv_ex.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
+        L.cof_search_async.argtypes = [C.c_void_p]
+        L.cof_search_wait.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.cof_get_stats.argtypes = [C.c_void_p, C.POINTER(CofStats)]
         L.cof_close.argtypes = [C.c_void_p]
         L.cof_strerror.argtypes = [C.c_int]
@@ -277,6 +280,15 @@ class Search:
         st = lib().cof_search(self.h, None, 0, C.byref(n))
         if st not in (COF_OK, COF_E_OVERFLOW):
             self._check(st, "search")
+        return n.value
+
+    def search_async(self):
+        """Queues a search (count query); wait() completes it."""
+        self._check(lib().cof_search_async(self.h), "search_async")
+
+    def wait(self):
+        n = C.c_uint64()
+        self._check(lib().cof_search_wait(self.h, C.byref(n)), "search_wait")
         return n.value
 
     def hits(self):

@@ -36,6 +36,8 @@ struct cof_ctx {
     cof::BulgeTune bulge_tune;
     bool have_genome = false;
     bool cached = false;
+    bool pending = false;          // between cof_search_async and cof_search_wait
+    std::chrono::steady_clock::time_point search_t0;
     cof::HitVec hits;
     cof::AuxVec aux;
     cof::TsvFormatter fmt;
@@ -50,6 +52,12 @@ namespace {
 cof_status fail(cof_ctx* ctx, const cof::Error& e) {
     if (ctx) ctx->err = e.what();
     return e.code;
+}
+
+// Entry points other than cof_search_wait refuse a context whose search is in flight.
+cof_status refuse_pending(cof_ctx* ctx) {
+    ctx->err = "a search started by cof_search_async is in flight: call cof_search_wait first";
+    return COF_E_INVALID;
 }
 
 // Device buffer for a packed genome of n_bytes, zero-padded past the end so
@@ -100,6 +108,7 @@ int first_byte(const char* path) {
 // host reader for folders and irregular FASTA files
 cof_status load_file(cof_ctx* ctx, const char* path, uint32_t flags) {
     if (!ctx || !path) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
     try {
         HIPCHECK(hipSetDevice(ctx->device));
         auto t0 = std::chrono::steady_clock::now();
@@ -262,6 +271,8 @@ cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out) {
 }
 
 cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx) return COF_E_INVALID;
     try {
         HIPCHECK(hipSetDevice(ctx->device));
@@ -296,12 +307,14 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
 cof_status cof_load_genome_file(cof_ctx* ctx, const char* path) { return load_file(ctx, path, 0); }
 
 cof_status cof_load_genome_file_ex(cof_ctx* ctx, const char* path, uint32_t flags) {
+
     return load_file(ctx, path, flags);
 }
 
 cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
                            uint64_t n_chunks, const char* const* names, uint32_t n_chroms) {
     if (!ctx || (!packed && n_bytes) || (!chunks && n_chunks) || (!names && n_chroms)) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
     try {
         HIPCHECK(hipSetDevice(ctx->device));
         cof::Genome g;
@@ -329,54 +342,75 @@ cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes
     return COF_OK;
 }
 
+namespace {
+
+// First half of a search: statistics reset and the plain scan queued.
+void search_begin(cof_ctx* ctx) {
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->search_t0 = std::chrono::steady_clock::now();
+    cof_stats& st = ctx->stats;
+    uint64_t genome_nucl = st.genome_nucl;
+    st = cof_stats{};
+    st.genome_nucl = genome_nucl;
+    st.load_ms = ctx->load_ms;
+    st.load_device = ctx->load_device ? 1 : 0;
+    ctx->hits.clear();
+    ctx->aux.clear();
+    ctx->cached = false;
+    cof::plain_launch(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, &st.n_plain_launches,
+                      ctx->plain_tune);
+    ctx->pending = true;
+}
+
+// Second half: plain hits to the host, the bulge path, conversion to final hits.
+void search_end(cof_ctx* ctx) {
+    ctx->pending = false;
+    HIPCHECK(hipSetDevice(ctx->device));
+    cof_stats& st = ctx->stats;
+    cof::RawView<cof::PlainRaw> praw;
+    cof::plain_finish(ctx->plain, ctx->ws, ctx->d_genome, ctx->stream, praw, &st.kernel_ms_plain,
+                      &st.n_plain_launches, ctx->plain_tune);
+    cof::RawView<cof::BulgeRaw> braw;
+    cof::BulgeTune bt = ctx->bulge_tune;
+    bt.flags = ctx->q.flags;
+    if (!ctx->model.empty() && ctx->bulge && cof::bulge_plan_size(ctx->bulge)) {
+        predict_chunk_cost(ctx);
+        bt.chunk_cost = &ctx->chunk_cost;
+    }
+    cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
+    st.n_raw_hits = praw.size() + braw.size();
+    auto tc = std::chrono::steady_clock::now();
+    cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
+    size_t first_bulge = ctx->hits.size();
+    cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
+    if (!(ctx->q.flags & COF_FLAG_COUNT_ONLY)) {
+        st.bulge_hits = ctx->hits.size() - first_bulge;
+        for (size_t h = first_bulge; h < ctx->hits.size(); h++) {
+            const cof_hit& x = ctx->hits[h];
+            uint64_t so[3];
+            cof::ops_reverse(x.ops, x.n_ops, so);
+            st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, cof::pattern_id_of(ctx->q.pats[x.pattern]),
+                                                     x.mismatches, x.dna_bulge, x.rna_bulge, x.n_ops, so[0], so[1]);
+        }
+    }
+    const auto t_end = std::chrono::steady_clock::now();
+    st.convert_ms = std::chrono::duration<double, std::milli>(t_end - tc).count();
+    st.search_ms = std::chrono::duration<double, std::milli>(t_end - ctx->search_t0).count();
+    st.n_hits = ctx->hits.size();
+    ctx->cached = true;
+}
+
+}  // namespace
+
 cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits) {
     if (!ctx) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
     if (!ctx->have_genome) return COF_E_NOGENOME;
     try {
         // a count query (no buffer) always searches; a fetch reuses the cache
         if (!ctx->cached || !out) {
-            HIPCHECK(hipSetDevice(ctx->device));
-            auto t0 = std::chrono::steady_clock::now();
-            cof_stats& st = ctx->stats;
-            uint64_t genome_nucl = st.genome_nucl;
-            st = cof_stats{};
-            st.genome_nucl = genome_nucl;
-            st.load_ms = ctx->load_ms;
-            st.load_device = ctx->load_device ? 1 : 0;
-            ctx->hits.clear();
-            ctx->aux.clear();
-            cof::RawView<cof::PlainRaw> praw;
-            cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, praw,
-                           &st.kernel_ms_plain, &st.n_plain_launches, ctx->plain_tune);
-            cof::RawView<cof::BulgeRaw> braw;
-            cof::BulgeTune bt = ctx->bulge_tune;
-            bt.flags = ctx->q.flags;
-            if (!ctx->model.empty() && ctx->bulge && cof::bulge_plan_size(ctx->bulge)) {
-                predict_chunk_cost(ctx);
-                bt.chunk_cost = &ctx->chunk_cost;
-            }
-            cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
-            st.n_raw_hits = praw.size() + braw.size();
-            auto tc = std::chrono::steady_clock::now();
-            cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
-            size_t first_bulge = ctx->hits.size();
-            cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
-            if (!(ctx->q.flags & COF_FLAG_COUNT_ONLY)) {
-                st.bulge_hits = ctx->hits.size() - first_bulge;
-                for (size_t h = first_bulge; h < ctx->hits.size(); h++) {
-                    const cof_hit& x = ctx->hits[h];
-                    uint64_t so[3];
-                    cof::ops_reverse(x.ops, x.n_ops, so);
-                    st.bulge_checksum += cof::bulge_hit_hash(ctx->aux[h].nib, cof::pattern_id_of(ctx->q.pats[x.pattern]),
-                                                             x.mismatches, x.dna_bulge, x.rna_bulge, x.n_ops, so[0],
-                                                             so[1]);
-                }
-            }
-            const auto t_end = std::chrono::steady_clock::now();
-            st.convert_ms = std::chrono::duration<double, std::milli>(t_end - tc).count();
-            st.search_ms = std::chrono::duration<double, std::milli>(t_end - t0).count();
-            st.n_hits = ctx->hits.size();
-            ctx->cached = true;
+            search_begin(ctx);
+            search_end(ctx);
         }
         if (n_hits) *n_hits = ctx->hits.size();
         if (cap < ctx->hits.size()) {
@@ -385,12 +419,44 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
         }
         if (out && !ctx->hits.empty()) memcpy(out, ctx->hits.data(), ctx->hits.size() * sizeof(cof_hit));
     } catch (const cof::Error& e) {
+        ctx->pending = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_search_async(cof_ctx* ctx) {
+    if (!ctx) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
+    if (!ctx->have_genome) return COF_E_NOGENOME;
+    try {
+        search_begin(ctx);
+    } catch (const cof::Error& e) {
+        ctx->pending = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_search_wait(cof_ctx* ctx, uint64_t* n_hits) {
+    if (!ctx) return COF_E_INVALID;
+    if (!ctx->pending) {
+        ctx->err = "no search in flight";
+        return COF_E_INVALID;
+    }
+    try {
+        search_end(ctx);
+        if (n_hits) *n_hits = ctx->hits.size();
+    } catch (const cof::Error& e) {
+        ctx->pending = false;
         return fail(ctx, e);
     }
     return COF_OK;
 }
 
 cof_status cof_write_tsv_ex(cof_ctx* ctx, const char* path, uint32_t flags) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx || !path) return COF_E_INVALID;
     if (!ctx->cached) return COF_E_INVALID;
     if (flags & ~COF_WRITE_HOST) return COF_E_INVALID;
@@ -432,6 +498,8 @@ cof_status cof_get_stats(const cof_ctx* ctx, cof_stats* out) {
 }
 
 cof_status cof_set_tuning(cof_ctx* ctx, const cof_tuning* t) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx || !t) return COF_E_INVALID;
     try {
         set_tuning(ctx, *t);
@@ -465,6 +533,8 @@ cof_status cof_get_tile_stats(const cof_ctx* ctx, cof_tile_stat* out, uint64_t c
 }
 
 cof_status cof_chunk_features(cof_ctx* ctx, float* out, uint64_t cap_rows, uint64_t* n_rows) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx || (!out && cap_rows)) return COF_E_INVALID;
     if (!ctx->have_genome) return COF_E_NOGENOME;
     try {
@@ -486,6 +556,8 @@ cof_status cof_chunk_features(cof_ctx* ctx, float* out, uint64_t cap_rows, uint6
 }
 
 cof_status cof_set_cost_model(cof_ctx* ctx, const cof_mlp* m) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx) return COF_E_INVALID;
     try {
         HIPCHECK(hipSetDevice(ctx->device));
@@ -578,6 +650,8 @@ cof_status cof_get_tuning(const cof_ctx* ctx, cof_tuning* out) {
 }
 
 cof_status cof_autotune(cof_ctx* ctx, uint64_t sample_nucl, cof_tuning* chosen) {
+    if (ctx && ctx->pending) return refuse_pending(ctx);
+
     if (!ctx) return COF_E_INVALID;
     if (!ctx->have_genome) return COF_E_NOGENOME;
     try {
@@ -593,6 +667,7 @@ cof_status cof_autotune(cof_ctx* ctx, uint64_t sample_nucl, cof_tuning* chosen) 
 void cof_close(cof_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    if (ctx->pending) (void)hipStreamSynchronize(ctx->stream);   // a search left in flight
     cof::plain_plan_destroy(ctx->plain);
     cof::bulge_plan_destroy(ctx->bulge);
     cof::feature_plan_destroy(ctx->feat);

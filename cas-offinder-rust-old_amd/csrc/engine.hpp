@@ -17,6 +17,10 @@ struct Workspace {
     uint32_t* plain_count = nullptr;   // per-segment counters, 128 bytes apart
     void* plain_out2 = nullptr;    // the segments packed together
     uint32_t plain_cap2 = 0;
+    uint32_t* h_counts = nullptr;  // pinned copy of the segment counters
+    size_t h_counts_cap = 0;
+    bool plain_pending = false;    // a launched scan awaits plain_finish
+    uint64_t plain_scan_end = 0;
     void* queue = nullptr;         // bulge candidates
     uint32_t q_cap = 0;
     void* bulge_out = nullptr;     // BulgeRaw records
@@ -141,6 +145,12 @@ uint64_t plain_tile_nucl();
 uint64_t plain_pad_words();
 void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
                RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
+// plain_run in two halves: plain_launch queues the scan and returns;
+// plain_finish waits for it and fills `out` (the host is free in between).
+void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
+                  uint64_t* n_launches, const PlainTune& tune);
+void plain_finish(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, hipStream_t st, RawView<PlainRaw>& out,
+                  double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
 
 struct BulgePlan;
 BulgePlan* bulge_plan_create(const Query& q);

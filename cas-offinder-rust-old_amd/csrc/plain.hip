@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 
 #include "cof_internal.hpp"
 #include "engine.hpp"
@@ -634,6 +635,7 @@ Workspace::~Workspace() {
     if (st2) (void)hipStreamDestroy(st2);
     if (st3) (void)hipStreamDestroy(st3);
     if (h_plain) (void)hipHostFree(h_plain);
+    if (h_counts) (void)hipHostFree(h_counts);
     if (h_bulge) (void)hipHostFree(h_bulge);
 }
 
@@ -674,9 +676,29 @@ void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem) {
     *cap = need;
 }
 
-void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
-               RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
-    out = RawView<PlainRaw>{};
+namespace {
+
+// Scans of all contexts on one device run one after another: a scan fills
+// the GPU, so overlapping two gains nothing, and back-to-back scans keep each
+// context's event timing its own kernels.  A context's stream waits for the
+// event recorded after the previous scan on the device (any context), while
+// the host stays free to convert the previous context's hits.
+struct ScanOrder {
+    std::mutex m;
+    hipEvent_t last[64] = {};
+};
+ScanOrder& scan_order() {
+    static ScanOrder o;
+    return o;
+}
+
+}  // namespace
+
+// Queues the scan of start positions [0, scan_end) and the copy of the
+// per-segment hit counts to the host; plain_finish completes it.
+void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
+                  uint64_t* n_launches, const PlainTune& tune) {
+    ws.plain_pending = false;
     if (plain_plan_empty(pp) || scan_end == 0) return;
     ws.ensure_events();
     if (!ws.plain_out) {
@@ -684,48 +706,71 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
         ws_grow(&ws.plain_out, &cap, 1u << 20, sizeof(PlainRaw));
         ws.plain_cap = (uint32_t)cap;
     }
+    size_t hc = ws.h_counts_cap;
+    ws.ensure_host((void**)&ws.h_counts, &hc, kHitSegs * kSegStride * sizeof(uint32_t));
+    ws.h_counts_cap = hc;
     const uint64_t kTile = (uint64_t)tune.threads * 64;
-    uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
-    uint32_t counts[kHitSegs * kSegStride];
-    for (;;) {
-        PlainRaw* d_out = (PlainRaw*)ws.plain_out;
-        const uint32_t seg_cap = ws.plain_cap / kHitSegs;
-        HIPCHECK(hipMemsetAsync(ws.plain_count, 0, sizeof counts, st));
-        HIPCHECK(hipEventRecord(ws.ev[0], st));
-        for (const auto& l : pp->launches) {
-            // grids of at most 2^20 tiles keep the tile index in 32 bits
-            for (uint64_t t0 = 0; t0 < n_tiles; t0 += (1u << 20)) {
-                uint32_t nb = (uint32_t)std::min<uint64_t>(n_tiles - t0, 1u << 20);
-                uint64_t first = t0 * kTile;
-                const GroupDev* g = pp->d_groups + l.group_begin;
-                uint32_t ny = tune.pat_chunk ? (l.max_pats + tune.pat_chunk - 1) / tune.pat_chunk : 1u;
-                dim3 grid(nb, std::max(1u, ny));
+    const uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
+    PlainRaw* d_out = (PlainRaw*)ws.plain_out;
+    const uint32_t seg_cap = ws.plain_cap / kHitSegs;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    ScanOrder& order = scan_order();
+    std::lock_guard<std::mutex> order_lock(order.m);
+    hipEvent_t& last = order.last[dev & 63];
+    if (last) HIPCHECK(hipStreamWaitEvent(st, last, 0));
+    else HIPCHECK(hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    HIPCHECK(hipMemsetAsync(ws.plain_count, 0, kHitSegs * kSegStride * sizeof(uint32_t), st));
+    HIPCHECK(hipEventRecord(ws.ev[0], st));
+    for (const auto& l : pp->launches) {
+        // grids of at most 2^20 tiles keep the tile index in 32 bits
+        for (uint64_t t0 = 0; t0 < n_tiles; t0 += (1u << 20)) {
+            uint32_t nb = (uint32_t)std::min<uint64_t>(n_tiles - t0, 1u << 20);
+            uint64_t first = t0 * kTile;
+            const GroupDev* g = pp->d_groups + l.group_begin;
+            uint32_t ny = tune.pat_chunk ? (l.max_pats + tune.pat_chunk - 1) / tune.pat_chunk : 1u;
+            dim3 grid(nb, std::max(1u, ny));
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
         launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
                          pp->d_full, d_out, ws.plain_count, seg_cap);                                         \
         break;
-                switch (l.W) {
-                    COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
-                    COF_PLAIN_LAUNCH(5) COF_PLAIN_LAUNCH(6) COF_PLAIN_LAUNCH(7) COF_PLAIN_LAUNCH(8)
-                    default: throw Error(COF_E_UNSUPPORTED, "compare window too wide");
-                }
-#undef COF_PLAIN_LAUNCH
-                HIPCHECK(hipGetLastError());
-                if (n_launches) (*n_launches)++;
+            switch (l.W) {
+                COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
+                COF_PLAIN_LAUNCH(5) COF_PLAIN_LAUNCH(6) COF_PLAIN_LAUNCH(7) COF_PLAIN_LAUNCH(8)
+                default: throw Error(COF_E_UNSUPPORTED, "compare window too wide");
             }
+#undef COF_PLAIN_LAUNCH
+            HIPCHECK(hipGetLastError());
+            if (n_launches) (*n_launches)++;
         }
-        HIPCHECK(hipEventRecord(ws.ev[1], st));
-        HIPCHECK(hipMemcpyAsync(counts, ws.plain_count, sizeof counts, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipEventRecord(ws.ev[1], st));
+    HIPCHECK(hipEventRecord(last, st));
+    HIPCHECK(hipMemcpyAsync(ws.h_counts, ws.plain_count, kHitSegs * kSegStride * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    ws.plain_pending = true;
+    ws.plain_scan_end = scan_end;
+}
+
+// Waits for a launched scan and brings its hits to the host; a segment that
+// overflowed grows every segment and the scan runs again.
+void plain_finish(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, hipStream_t st, RawView<PlainRaw>& out,
+                  double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
+    out = RawView<PlainRaw>{};
+    if (!ws.plain_pending) return;
+    for (;;) {
         HIPCHECK(hipStreamSynchronize(st));
+        ws.plain_pending = false;
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]));
         if (kernel_ms) *kernel_ms += ms;
+        const uint32_t seg_cap = ws.plain_cap / kHitSegs;
         uint64_t total = 0;
         uint32_t most = 0;
         for (uint32_t k = 0; k < kHitSegs; k++) {
-            total += counts[k * kSegStride];
-            most = std::max(most, counts[k * kSegStride]);
+            total += ws.h_counts[k * kSegStride];
+            most = std::max(most, ws.h_counts[k * kSegStride]);
         }
         if (most > seg_cap) {
             // a segment overflowed: grow every segment past the fullest one and scan again
@@ -734,6 +779,7 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
             size_t cap = ws.plain_cap;
             ws_grow(&ws.plain_out, &cap, need, sizeof(PlainRaw));
             ws.plain_cap = (uint32_t)cap;
+            plain_launch(pp, ws, d_genome, ws.plain_scan_end, st, n_launches, tune);
             continue;
         }
         const uint32_t cnt = (uint32_t)total;
@@ -742,8 +788,8 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
             size_t cap2 = ws.plain_cap2;
             ws_grow(&ws.plain_out2, &cap2, cnt, sizeof(PlainRaw));
             ws.plain_cap2 = (uint32_t)cap2;
-            hipLaunchKernelGGL(plain_compact_kernel, dim3(kHitSegs), dim3(256), 0, st, d_out, ws.plain_count, seg_cap,
-                               (PlainRaw*)ws.plain_out2);
+            hipLaunchKernelGGL(plain_compact_kernel, dim3(kHitSegs), dim3(256), 0, st, (const PlainRaw*)ws.plain_out,
+                               ws.plain_count, seg_cap, (PlainRaw*)ws.plain_out2);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(ws.h_plain, ws.plain_out2, (size_t)cnt * sizeof(PlainRaw), hipMemcpyDeviceToHost,
                                     st));
@@ -752,6 +798,12 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
         out = RawView<PlainRaw>{(const PlainRaw*)ws.h_plain, cnt};
         return;
     }
+}
+
+void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t scan_end, hipStream_t st,
+               RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
+    plain_launch(pp, ws, d_genome, scan_end, st, n_launches, tune);
+    plain_finish(pp, ws, d_genome, st, out, kernel_ms, n_launches, tune);
 }
 
 }  // namespace cof

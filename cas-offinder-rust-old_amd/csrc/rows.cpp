@@ -12,6 +12,8 @@
 // PAM limit); here the chromosome-end drop rule of convert_matches_myers
 // (search.rs:1316-1374) is applied and rows are formatted (main.rs:145-157).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <functional>
@@ -66,7 +68,9 @@ bool plain_predicate(const Genome& g, const Pattern& p, uint64_t pos, uint64_t v
 namespace {
 
 // Persistent worker pool for the converters (creating 16 threads per search
-// costs about a millisecond).
+// costs about a millisecond).  After a job the workers spin for a short while
+// before sleeping, so the second pass of a conversion and the next search's
+// conversion start without a condition-variable wake-up.
 class Pool {
 public:
     static Pool& get() {
@@ -76,16 +80,17 @@ public:
     unsigned size() const { return (unsigned)workers_.size() + 1; }
     // runs f(t) for t in [0, n) on the workers and the calling thread
     void run(unsigned n, const std::function<void(unsigned)>& f) {
-        std::unique_lock<std::mutex> lk(m_);
-        job_ = &f;
-        n_tasks_ = n;
-        next_ = 0;
-        done_ = 0;
-        gen_++;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &f;
+            n_tasks_ = n;
+            next_ = 0;
+            done_ = 0;
+            gen_.fetch_add(1, std::memory_order_release);
+        }
         cv_.notify_all();
-        lk.unlock();
         work();
-        lk.lock();
+        std::unique_lock<std::mutex> lk(m_);
         done_cv_.wait(lk, [&] { return done_ == n_tasks_; });
         job_ = nullptr;
     }
@@ -98,10 +103,14 @@ private:
             workers_.emplace_back([this] {
                 uint64_t seen = 0;
                 for (;;) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    while (gen_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_relaxed) &&
+                           std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(300))
+                        std::this_thread::yield();
                     std::unique_lock<std::mutex> lk(m_);
-                    cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                    cv_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
                     if (stop_) return;
-                    seen = gen_;
+                    seen = gen_.load();
                     lk.unlock();
                     work();
                 }
@@ -135,8 +144,8 @@ private:
     std::condition_variable cv_, done_cv_;
     const std::function<void(unsigned)>* job_ = nullptr;
     unsigned n_tasks_ = 0, next_ = 0, done_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<bool> stop_{false};
 };
 
 // Runs fn(begin, end, out, aux) -> count over slices of [0, n) in parallel,

@@ -205,6 +205,17 @@ cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes
  * bisection (search.rs:1140-1152). */
 cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
 
+/* cof_search (count query) in two calls: cof_search_async queues the device
+ * work of a search and returns; cof_search_wait completes it (hits copied
+ * back and converted, any bulge pattern searched) and writes the count.  In
+ * between the host is free, so a caller with several contexts overlaps one
+ * context's host work with the next one's kernels, as the reference overlaps
+ * its reader, search and writer threads (main.rs:61-274, search.rs:1490-1663).
+ * Until cof_search_wait returns, other calls on the context fail with
+ * COF_E_INVALID (cof_get_stats and cof_close excepted). */
+cof_status cof_search_async(cof_ctx* ctx);
+cof_status cof_search_wait(cof_ctx* ctx, uint64_t* n_hits);
+
 /* Writes the cached hits of the last search as reference TSV rows, header
  * included (main.rs:130-247).  path "-" = stdout.  The rows are formatted on
  * the device from the hits and the resident genome and streamed to the file. */

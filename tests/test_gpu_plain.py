@@ -162,3 +162,37 @@ def test_plane_window_shapes(tmp_path, case):
     got, want, st = run_both(tmp_path, fa, filt, pats, mm)
     assert len(want) > 0
     assert got == want
+
+
+def test_async_search_matches_sync(tmp_path):
+    # cof_search_async / cof_search_wait on two interleaved contexts give the
+    # hits of cof_search; a context with a search in flight refuses other calls
+    guides = ["GGCCGACCTGTCGCTGACGC", "ACCTGTCGCTGACGCGGCCG"]
+    sg = cofpy.SynthGenome(7, [300_000, 90_001], n_frac=0.01, gap=(20, 500),
+                           plants=[(g + "AGG", 0, 20, 4, 40) for g in guides])
+    pats = [g + "NNN" for g in guides]
+    a = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * 2, [5, 5])
+    b = cofpy.Search(pats, ["NNNNNNNNNNNNNNNNNNNNNRG"] * 2, [3, 3])
+    a.load_synth(sg)
+    b.load_synth(sg)
+    want_a, want_b = a.search(), b.search()
+    def key(s):     # hit order depends on the order workgroups append; the set does not
+        return sorted((h.chrom, h.chrom_pos, h.pattern, h.mismatches) for h in s.hits())
+
+    ha, hb = key(a), key(b)
+    for _ in range(2):
+        a.search_async()
+        b.search_async()
+        with pytest.raises(cofpy.CofError) as e:
+            a.write_tsv(tmp_path / "x.tsv")
+        assert e.value.status == cofpy.COF_E_INVALID
+        assert a.wait() == want_a
+        assert b.wait() == want_b
+        assert key(a) == ha
+        assert key(b) == hb
+    with pytest.raises(cofpy.CofError):
+        a.wait()                       # nothing in flight
+    a.search_async()                   # left in flight: close waits for it
+    a.close()
+    b.close()
+    assert want_a > want_b > 0
