@@ -185,34 +185,49 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
     const uint32_t out_cap = seg_cap;
 
-    // 1. genome tile -> LDS (N-masked), IUPAC and zero-nibble detection
+    // 1. genome tile -> LDS (N-masked) as bit4 words and as two 32-position
+    //    bit planes (base bit 1 = A/G, base bit 0 = C/G), with IUPAC and
+    //    zero-nibble (masked N, padding) detection.  Thread tid takes the
+    //    16-byte groups tid + k * kThreads: one group of 4 words is one plane
+    //    word, and every load of the thread is issued before any is used.
+    constexpr int kGroups4 = (kTileWords + 3) / 4;
+    constexpr int kLoadIt = (kPlaneWords + kThreads - 1) / kThreads;
+    const uint4* const g4 = reinterpret_cast<const uint4*>(genome + w0);
+    uint4 lv[kLoadIt];
+#pragma unroll
+    for (int it = 0; it < kLoadIt; it++) {
+        const uint32_t j = tid + (uint32_t)it * kThreads;
+        lv[it] = j < (uint32_t)kGroups4 ? g4[j] : make_uint4(0u, 0u, 0u, 0u);
+    }
     int flags = 0;   // bit 0: an IUPAC letter, bit 1: a zero nibble (masked N, padding)
-    for (uint32_t i = tid; i < (uint32_t)kTileWords; i += kThreads) {
-        uint32_t w = genome[w0 + i];
-        uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
-        w &= ~(f * 0xFu);
-        flags |= (multi_nibbles(w) != 0 ? 1 : 0) | (nz_nibbles(w) != 0x88888888u ? 2 : 0);
-        gw[i] = w;
+#pragma unroll
+    for (int it = 0; it < kLoadIt; it++) {
+        const uint32_t j = tid + (uint32_t)it * kThreads;
+        if (j >= (uint32_t)kPlaneWords) continue;
+        const uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
+        uint32_t h = 0, l = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) {
+            const uint32_t idx = 4 * j + b;
+            uint32_t w = raw[b];
+            const uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
+            w &= ~(f * 0xFu);
+            if (idx < (uint32_t)kTileWords) {
+                flags |= (multi_nibbles(w) != 0 ? 1 : 0) | (nz_nibbles(w) != 0x88888888u ? 2 : 0);
+                gw[idx] = w;
+            } else {
+                w = 0;
+            }
+            h |= gather_nibble_bit0(((w >> 2) | (w >> 3)) & 0x11111111u) << (8 * b);
+            l |= gather_nibble_bit0(((w >> 1) | (w >> 3)) & 0x11111111u) << (8 * b);
+        }
+        ghi[j] = h;
+        glo[j] = l;
     }
     flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
     const int iupac = flags & 1;
+    // the planes serve tiles of plain bases only
     const bool plain_tile = flags == 0;
-    if (plain_tile) {
-        // 32 positions per plane word: base bit 1 (A, G) and base bit 0 (C, G)
-        for (uint32_t j = tid; j < (uint32_t)kPlaneWords; j += kThreads) {
-            uint32_t h = 0, l = 0;
-#pragma unroll
-            for (uint32_t b = 0; b < 4; b++) {
-                const uint32_t idx = 4 * j + b;
-                const uint32_t w = idx < (uint32_t)kTileWords ? gw[idx] : 0u;
-                h |= gather_nibble_bit0(((w >> 2) | (w >> 3)) & 0x11111111u) << (8 * b);
-                l |= gather_nibble_bit0(((w >> 1) | (w >> 3)) & 0x11111111u) << (8 * b);
-            }
-            ghi[j] = h;
-            glo[j] = l;
-        }
-        __syncthreads();
-    }
 
     // Bit4 PAM test: thread tid owns tile words w * kThreads + tid (w < 8), 8
     // starts each; plane PAM test: plane words w * kThreads + tid (w < 2), 32
