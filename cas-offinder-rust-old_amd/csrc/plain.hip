@@ -60,18 +60,21 @@ struct GroupDev {
     uint32_t L, k_lo, n_pam, W;
     // 2-bit plane compare: window from pk_lo, wmask = positions that are not
     // N in every pattern
-    uint32_t planes, pk_lo, wmask, pad;
+    uint32_t planes, pk_lo, wmask, prec_begin;   // prec_begin: the group's first plane record
     uint32_t pam_pos[kMaxPam];
     uint32_t pam_mask[kMaxPam];   // filter code in every nibble
 };
 
 // One pattern as two bit planes over the group's plane window (base T C A G =
 // planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and
-// K = 64 - M: a window hits when popcount(mismatch) + K <= 64.
+// K = 64 - M: a window hits when popcount(mismatch) + K <= 64.  A group's
+// records are padded to a multiple of 8 with records that never hit (K = 200),
+// so the compare runs in whole batches of eight; `plan` is the pattern's index
+// in the plan (its bit4 record and full words).
 struct __attribute__((aligned(16))) PlaneRec {
     uint32_t ph, pl;
     uint32_t K;
-    uint32_t pid;
+    uint32_t plan;
 };
 
 // One pattern of a group: its compare window (the pattern's nibbles from
@@ -394,8 +397,12 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
                 const bool act = active;
                 const uint64_t act_mask = __ballot(act);
-                uint32_t p = pb;
-                for (; p + 8 <= pe; p += 8) {
+                // plane records of patterns [pb, pe); a whole group runs on
+                // its padded record count
+                uint32_t p = G.prec_begin + (pb - G.pat_begin);
+                const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
+                                              : G.prec_begin + ((G.pat_count + 7u) & ~7u);
+                for (; p + 8 <= qe; p += 8) {
                     const PlaneRec* const rp = prec + p;   // uniform base
                     // per pattern: xor, bitop3, v_bcnt accumulating K, and
                     // half a v_min3; one compare for the eight
@@ -410,19 +417,20 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                         for (int u = 0; u < 8; u++) {
                             const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
                             if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
-                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                            const uint32_t gp = rp[u].plan;
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)gp * 8, G.L);
                             uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[gp].pid, amm};
                         }
                     }
                 }
-                for (; p < pe; p++) {
+                for (; p < qe; p++) {
                     const PlaneRec& r = prec[p];
                     const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
                     if (act && (uint32_t)__popc(d) + r.K <= 64u) {
-                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
+                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)r.plan * 8, G.L);
                         uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[r.plan].pid, amm};
                     }
                 }
             } else if (!iupac) {
@@ -553,6 +561,7 @@ PlainPlan* plain_plan_create(const Query& q) {
         G.wmask = 0;
         for (uint32_t k = pk_lo; k < pk_hi; k++)
             if (!all_n[k]) G.wmask |= 1u << (k - pk_lo);
+        G.prec_begin = (uint32_t)precs.size();
         for (uint32_t pi : kv.second) {
             const Pattern& p = q.pats[pi];
             uint32_t cw[8] = {0}, fw[8] = {0};
@@ -568,7 +577,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             r.M = p.M;
             r.pid = pi;
             recs.push_back(r);
-            PlaneRec pr{0, 0, 64u - std::min<uint32_t>(p.M, 64u), pi};
+            PlaneRec pr{0, 0, 64u - std::min<uint32_t>(p.M, 64u), pat_index};
             if (planes)
                 for (uint32_t k = pk_lo; k < pk_hi; k++) {
                     if (all_n[k]) continue;
@@ -580,6 +589,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             full.insert(full.end(), fw, fw + 8);
             pat_index++;
         }
+        while ((precs.size() - G.prec_begin) % 8) precs.push_back(PlaneRec{0, 0, 200u, G.pat_begin});
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
