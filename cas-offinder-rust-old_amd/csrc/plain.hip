@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "cof_internal.hpp"
 #include "engine.hpp"
@@ -61,6 +62,7 @@ struct GroupDev {
     // 2-bit plane compare: window from pk_lo, wmask = positions that are not
     // N in every pattern
     uint32_t planes, pk_lo, wmask, prec_begin;   // prec_begin: the group's first plane record
+    uint32_t prec_padded, pad2[3];               // the group's plane records with fillers (multiple of 16)
     uint32_t pam_pos[kMaxPam];
     uint32_t pam_mask[kMaxPam];   // filter code in every nibble
 };
@@ -68,9 +70,9 @@ struct GroupDev {
 // One pattern as two bit planes over the group's plane window (base T C A G =
 // planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and
 // K = 64 - M: a window hits when popcount(mismatch) + K <= 64.  A group's
-// records are padded to a multiple of 8 with records that never hit (K = 200),
-// so the compare runs in whole batches of eight; `plan` is the pattern's index
-// in the plan (its bit4 record and full words).
+// records are padded to a multiple of 16 with records that never hit
+// (K = 200), so the compare runs in whole batches of 16 or 8; `plan` is the
+// pattern's index in the plan (its bit4 record and full words).
 struct __attribute__((aligned(16))) PlaneRec {
     uint32_t ph, pl;
     uint32_t K;
@@ -402,19 +404,21 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 uint32_t p = G.prec_begin + (pb - G.pat_begin);
                 const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
                                               : G.prec_begin + ((G.pat_count + 7u) & ~7u);
-                for (; p + 8 <= qe; p += 8) {
+                // NB patterns per batch: per pattern xor, bitop3, v_bcnt
+                // accumulating K and half a v_min3, one compare per batch; all
+                // NB records are loaded before the batch computes
+                auto batch = [&](auto nb_tag) {
+                    constexpr int NB = decltype(nb_tag)::value;
                     const PlaneRec* const rp = prec + p;   // uniform base
-                    // per pattern: xor, bitop3, v_bcnt accumulating K, and
-                    // half a v_min3; one compare for the eight
                     uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
-                    for (int u = 0; u < 8; u += 2) {
+                    for (int u = 0; u < NB; u += 2) {
                         const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
                         const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
                         mn = min(mn, min(t0, t1));
                     }
                     if (__ballot(mn <= 64u) & act_mask) {
-                        for (int u = 0; u < 8; u++) {
+                        for (int u = 0; u < NB; u++) {
                             const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
                             if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
                             const uint32_t gp = rp[u].plan;
@@ -423,7 +427,14 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                             if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[gp].pid, amm};
                         }
                     }
-                }
+                    p += NB;
+                };
+                // batches of 16 while more than 8 records remain (a whole
+                // group may run into its fillers, a pattern slice may not),
+                // then at most one batch of 8, then single records
+                const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
+                while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
+                if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
                 for (; p < qe; p++) {
                     const PlaneRec& r = prec[p];
                     const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
@@ -589,7 +600,8 @@ PlainPlan* plain_plan_create(const Query& q) {
             full.insert(full.end(), fw, fw + 8);
             pat_index++;
         }
-        while ((precs.size() - G.prec_begin) % 8) precs.push_back(PlaneRec{0, 0, 200u, G.pat_begin});
+        while ((precs.size() - G.prec_begin) % 16) precs.push_back(PlaneRec{0, 0, 200u, G.pat_begin});
+        G.prec_padded = (uint32_t)(precs.size() - G.prec_begin);
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
