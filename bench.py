@@ -423,6 +423,56 @@ def tsv_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def cli_workload(args):
+    """End to end as the reference CLI runs it (input file -> genome FASTA on disk -> TSV on disk): cof-cli on the
+    GPU vs the oracle's CLI twin on 16 host threads, one headline run (NRG-20-M3, 100 guides) on the 3.1 Gbp
+    genome, both with 4 Mnt batches so the row sets must be equal."""
+    import subprocess
+    total = int(args.genome_bp)
+    lens = chrom_lengths(total, 24, args.seed)
+    runs = make_queries(args.seed)
+    plants = [p for r in runs for p in r["plants"]]
+    r = next(r for r in runs if r["name"] == "NRG-20-M3")
+    sg = cofpy.SynthGenome(args.seed, lens, n_frac=0.05, gap=(10_000, 1_000_000), plants=plants)
+    cli = os.path.join(ROOT, "cas-offinder-rust-old_amd", "_build", "cof-cli")
+    ref = os.path.join(ROOT, "oracle", "_build", "cof-ref")
+    threads = min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory(dir=args.tsv_dir) as d:
+        fa = os.path.join(d, "genome.fa")
+        sg.write_fasta(fa)
+        inp = os.path.join(d, "in.txt")
+        with open(inp, "w") as f:
+            f.write(f"{fa}\n{r['filter']}\n" + "".join(f"{p} {r['mm']}\n" for p in r["pats"]))
+        env = dict(os.environ, COF_BATCH_NUCL=str(1 << 22))
+        gpu_s = []
+        for k in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            subprocess.run([cli, inp, "G", os.path.join(d, "gpu.tsv")], env=env, check=True, capture_output=True)
+            dt = time.perf_counter() - t0
+            log(f"cof-cli: {dt:.2f} s")
+            if k >= args.warmup:
+                gpu_s.append(dt)
+        t0 = time.perf_counter()
+        subprocess.run([ref, inp, "C", os.path.join(d, "ref.tsv"), "--batch-nucl", str(1 << 22), "--threads",
+                        str(threads)], check=True, capture_output=True)
+        ref_s = time.perf_counter() - t0
+        log(f"oracle CLI ({threads} threads): {ref_s:.2f} s")
+        rows_gpu = sorted(open(os.path.join(d, "gpu.tsv")).read().splitlines()[1:])
+        rows_ref = sorted(open(os.path.join(d, "ref.tsv")).read().splitlines()[1:])
+    secs = sum(gpu_s) / len(gpu_s)
+    out = {"metric": "end-to-end CLI wall time (input file to TSV, reference format)", "value": round(secs, 3),
+           "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(secs * 1e3, 1),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (bit4 / 2-bit planes)",
+           "data": "synthetic",
+           "config": {"workload": f"cof-cli on a {total / 1e9:.2f} Gbp FASTA, NRG-20-M3 (100 guides), 4 Mnt batches, "
+                                  "process start to TSV written", "genome_bp": total},
+           "rows": len(rows_gpu), "identical_rows": rows_gpu == rows_ref,
+           "guide_gbp_s_end_to_end": round(len(r["pats"]) * total / secs / 1e9, 1),
+           "cpu_baseline": {"value": round(ref_s, 2), "unit": "s", "cores": threads, "kind": "port",
+                            "sample": "the same input through the oracle's CLI twin (cof-ref), full genome"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,7 +485,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-overlap", action="store_true",
                     help="plain workload: finish each run before starting the next")
-    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv", "cli"],
+                    default="plain")
     ap.add_argument("--tsv-dir", default=None, help="tsv: folder for the file-output timings (default: system temp)")
     ap.add_argument("--train-bp", type=float, default=1e8, help="costmodel: size of each training genome")
     ap.add_argument("--train-genomes", type=int, default=20)
@@ -463,6 +514,8 @@ def main():
         return ingest_workload(args)
     if args.workload == "tsv":
         return tsv_workload(args)
+    if args.workload == "cli":
+        return cli_workload(args)
 
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)
