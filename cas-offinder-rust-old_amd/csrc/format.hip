@@ -242,8 +242,9 @@ void ensure_dev(T** p, size_t* cap, size_t n) {
 
 }  // namespace
 
+// write_tsv_device waits for its last copy before returning, so no device
+// work uses these buffers when the formatter goes away.
 TsvFormatter::~TsvFormatter() {
-    (void)hipDeviceSynchronize();
     for (void* p : {(void*)d_hits, (void*)d_aux, (void*)d_bsum, (void*)d_boff, (void*)d_out, (void*)d_pats,
                     (void*)d_blob, (void*)d_names, (void*)d_name_off, (void*)d_name_len})
         if (p) (void)hipFree(p);
