@@ -592,7 +592,7 @@ def main():
     # VALU issue of the scan from rocprofv3 --pmc SQ_INSTS_VALU on this
     # workload: wave instructions / (kernel time x 256 CU x 4 SIMD x 2.4 GHz /
     # 4 cycles per wave64 instruction), time-weighted over the 12 runs
-    valu_issue = 0.787 if abs(genome_nucl - 3.1e9) < 1e6 else None
+    valu_issue = 0.820 if abs(genome_nucl - 3.1e9) < 1e6 else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
