@@ -222,7 +222,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             const uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
             w &= ~(f * 0xFu);
             if (idx < (uint32_t)kTileWords) {
-                flags |= (multi_nibbles(w) != 0 ? 1 : 0) | (nz_nibbles(w) != 0x88888888u ? 2 : 0);
+                // a nibble with two or more bits set makes the word hold more
+                // bits than nonzero nibbles
+                const uint32_t nz = nz_nibbles(w);
+                flags |= (__popc(w) != __popc(nz) ? 1 : 0) | (nz != 0x88888888u ? 2 : 0);
                 gw[idx] = w;
             } else {
                 w = 0;
