@@ -114,23 +114,31 @@ template <uint32_t C>
 __device__ __forceinline__ uint32_t pam_cell_c(uint32_t acc, uint32_t h, uint32_t l) {
     return __builtin_amdgcn_bitop3_b32(acc, h, l, (C << 4) & 0xF0u);
 }
-__device__ __forceinline__ uint32_t pam_cell(uint32_t acc, uint32_t h, uint32_t l, uint32_t code) {
+// the same for two words at once (one scalar dispatch per PAM cell)
+template <uint32_t C>
+__device__ __forceinline__ void pam_cell2_c(uint32_t& a0, uint32_t& a1, uint32_t h0, uint32_t l0, uint32_t h1,
+                                            uint32_t l1) {
+    a0 = pam_cell_c<C>(a0, h0, l0);
+    a1 = pam_cell_c<C>(a1, h1, l1);
+}
+__device__ __forceinline__ void pam_cell2(uint32_t& a0, uint32_t& a1, uint32_t h0, uint32_t l0, uint32_t h1,
+                                          uint32_t l1, uint32_t code) {
     switch (code) {
-        case 1: return pam_cell_c<1>(acc, h, l);
-        case 2: return pam_cell_c<2>(acc, h, l);
-        case 3: return pam_cell_c<3>(acc, h, l);
-        case 4: return pam_cell_c<4>(acc, h, l);
-        case 5: return pam_cell_c<5>(acc, h, l);
-        case 6: return pam_cell_c<6>(acc, h, l);
-        case 7: return pam_cell_c<7>(acc, h, l);
-        case 8: return pam_cell_c<8>(acc, h, l);
-        case 9: return pam_cell_c<9>(acc, h, l);
-        case 10: return pam_cell_c<10>(acc, h, l);
-        case 11: return pam_cell_c<11>(acc, h, l);
-        case 12: return pam_cell_c<12>(acc, h, l);
-        case 13: return pam_cell_c<13>(acc, h, l);
-        case 14: return pam_cell_c<14>(acc, h, l);
-        default: return acc;      // 15: any base
+        case 1: pam_cell2_c<1>(a0, a1, h0, l0, h1, l1); break;
+        case 2: pam_cell2_c<2>(a0, a1, h0, l0, h1, l1); break;
+        case 3: pam_cell2_c<3>(a0, a1, h0, l0, h1, l1); break;
+        case 4: pam_cell2_c<4>(a0, a1, h0, l0, h1, l1); break;
+        case 5: pam_cell2_c<5>(a0, a1, h0, l0, h1, l1); break;
+        case 6: pam_cell2_c<6>(a0, a1, h0, l0, h1, l1); break;
+        case 7: pam_cell2_c<7>(a0, a1, h0, l0, h1, l1); break;
+        case 8: pam_cell2_c<8>(a0, a1, h0, l0, h1, l1); break;
+        case 9: pam_cell2_c<9>(a0, a1, h0, l0, h1, l1); break;
+        case 10: pam_cell2_c<10>(a0, a1, h0, l0, h1, l1); break;
+        case 11: pam_cell2_c<11>(a0, a1, h0, l0, h1, l1); break;
+        case 12: pam_cell2_c<12>(a0, a1, h0, l0, h1, l1); break;
+        case 13: pam_cell2_c<13>(a0, a1, h0, l0, h1, l1); break;
+        case 14: pam_cell2_c<14>(a0, a1, h0, l0, h1, l1); break;
+        default: break;           // 15: any base
     }
 }
 
@@ -264,13 +272,12 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             for (uint32_t k = 0; k < G.n_pam; k++) {
                 const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
                 const uint32_t qo = pk >> 5, sh = pk & 31u;
-#pragma unroll
-                for (int w = 0; w < 2; w++) {
-                    const uint32_t q = (uint32_t)w * kThreads + tid + qo;
-                    const uint32_t h = __builtin_amdgcn_alignbit(ghi[q + 1], ghi[q], sh);
-                    const uint32_t l = __builtin_amdgcn_alignbit(glo[q + 1], glo[q], sh);
-                    ok[w] = pam_cell(ok[w], h, l, code);
-                }
+                const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
+                const uint32_t h0 = __builtin_amdgcn_alignbit(ghi[q0 + 1], ghi[q0], sh);
+                const uint32_t l0 = __builtin_amdgcn_alignbit(glo[q0 + 1], glo[q0], sh);
+                const uint32_t h1 = __builtin_amdgcn_alignbit(ghi[q1 + 1], ghi[q1], sh);
+                const uint32_t l1 = __builtin_amdgcn_alignbit(glo[q1 + 1], glo[q1], sh);
+                pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
             }
 #pragma unroll
             for (int w = 0; w < 2; w++) {
@@ -310,23 +317,29 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 cnt += (uint32_t)__popc(m);
             }
         }
-        // block-wide exclusive scan of the per-thread counts
-        uint32_t v = cnt;
+        // block-wide exclusive scan of the per-thread counts (each <= 64):
+        // within the wave bit by bit, a ballot per bit and its lane count
+        // below (mbcnt), so no cross-lane permute waits on LDS
+        uint32_t excl = 0, wtot = 0;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t u = __shfl_up(v, d);
-            if (lane >= (uint32_t)d) v += u;
+        for (int b = 0; b < 7; b++) {
+            const uint64_t bb = __ballot((cnt >> b) & 1u);
+            excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u)) << b;
+            wtot += (uint32_t)__popcll(bb) << b;
         }
-        if (lane == 63) wave_sum[wave] = v;
-        __syncthreads();
-        uint32_t nc = 0, off = 0;
+        uint32_t nc = wtot, off = 0;
+        if (kThreads > 64) {
+            if (lane == 0) wave_sum[wave] = wtot;
+            __syncthreads();
+            nc = 0;
 #pragma unroll
-        for (int x = 0; x < kThreads / 64; x++) {
-            uint32_t ws = wave_sum[x];
-            off += (uint32_t)x < wave ? ws : 0u;
-            nc += ws;
+            for (int x = 0; x < kThreads / 64; x++) {
+                uint32_t ws = wave_sum[x];
+                off += (uint32_t)x < wave ? ws : 0u;
+                nc += ws;
+            }
         }
-        uint32_t pos = off + v - cnt;
+        uint32_t pos = off + excl;
         if (plain_tile) {
 #pragma unroll
             for (int w = 0; w < 2; w++) {
