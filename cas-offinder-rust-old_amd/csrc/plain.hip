@@ -244,7 +244,13 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         ghi[j] = h;
         glo[j] = l;
     }
-    flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    if constexpr (kThreads == 64) {
+        // one wave: a barrier for the LDS writes and two ballots
+        __syncthreads();
+        flags = (__ballot(flags & 1) != 0 ? 1 : 0) | (__ballot(flags & 2) != 0 ? 2 : 0);
+    } else {
+        flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    }
     const int iupac = flags & 1;
     // the planes serve tiles of plain bases only
     const bool plain_tile = flags == 0;
