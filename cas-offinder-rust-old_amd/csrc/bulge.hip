@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                 qb = atomicAdd(q_count, n_st);
             my_cand += n_st;
         }
-        qb = __shfl(qb, 0);
+        qb = __builtin_amdgcn_readlane(qb, 0);
         for (uint32_t i = lane; i < n_st; i += 64) {
             const uint32_t v = my_stage[i];
             if (qb + i < q_cap)
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
             uint32_t cnt = (uint32_t)__popcll(bal);
             uint32_t wb = 0;
             if (lane == 0 && cnt) wb = atomicAdd(&n_cand, cnt);
-            wb = __shfl(wb, 0);
+            wb = __builtin_amdgcn_readlane(wb, 0);
             if (ok) cand[wb + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] = (uint16_t)e;
         }
         __syncthreads();
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             const uint32_t leader = (uint32_t)__builtin_ctzll(em);
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(sink.count, (uint32_t)__popcll(em));
-            base = __shfl(base, (int)leader);
+            base = __builtin_amdgcn_readfirstlane(base);   // the leader is the first active lane
             const uint32_t idx =
                 base + __builtin_amdgcn_mbcnt_hi((uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0u));
             if (idx < sink.cap) sink.out[idx] = r;
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         if (!drained && n_idle >= sink.refill) {
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd(q_next, n_idle);
-            b0 = __shfl(b0, 0);
+            b0 = __builtin_amdgcn_readlane(b0, 0);
             if (b0 + n_idle >= n_queue) drained = true;
             if (!busy) {
                 uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
