@@ -142,6 +142,15 @@ __device__ __forceinline__ void pam_cell2(uint32_t& a0, uint32_t& a1, uint32_t h
     }
 }
 
+// Workgroup barrier for LDS exchange.  A one-wave workgroup only needs the
+// compiler not to move LDS accesses across it: a wave's LDS operations
+// complete in issue order.
+template <int THREADS>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (THREADS == 64) __builtin_amdgcn_wave_barrier();
+    else __syncthreads();
+}
+
 // bit 0 of nibble j -> bit j (other bits zero)
 __device__ __forceinline__ uint32_t gather_nibble_bit0(uint32_t m) {
     m = (m | (m >> 3)) & 0x03030303u;
@@ -245,8 +254,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         glo[j] = l;
     }
     if constexpr (kThreads == 64) {
-        // one wave: a barrier for the LDS writes and two ballots
-        __syncthreads();
+        // one wave: two ballots
+        tile_sync<kThreads>();
         flags = (__ballot(flags & 1) != 0 ? 1 : 0) | (__ballot(flags & 2) != 0 ? 2 : 0);
     } else {
         flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 }
             }
         }
-        __syncthreads();
+        tile_sync<kThreads>();
         // 3. compare windows against the group's patterns
         const bool planes = plain_tile && G.planes;   // uniform
         for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
@@ -493,7 +502,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 }
             }
         }
-        __syncthreads();
+        tile_sync<kThreads>();
     }
 }
 
