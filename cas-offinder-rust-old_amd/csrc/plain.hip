@@ -37,6 +37,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -66,7 +68,10 @@ struct GroupDev {
     // 2-bit plane compare: window from pk_lo, wmask = positions that are not
     // N in every pattern
     uint32_t planes, pk_lo, wmask, prec_begin;   // prec_begin: the group's first plane record
-    uint32_t prec_padded, pad2[3];               // the group's plane records with fillers (multiple of 16)
+    uint32_t prec_padded;                        // the group's plane records with fillers (multiple of 16)
+    // matrix-core filter (0: off) and the group's first operand fragment
+    // (units of 16 bytes)
+    uint32_t mfma, frag_begin, pad2;
     uint32_t pam_pos[kMaxPam];
     uint32_t pam_mask[kMaxPam];   // filter code in every nibble
 };
@@ -183,6 +188,118 @@ __device__ __forceinline__ uint32_t recount_swar(const uint32_t* gw, uint32_t i,
     return amm;
 }
 
+// Matrix-core filter of plain-base windows (groups whose patterns are plain
+// bases on the window; tiles of plain bases only).  Mismatches on window
+// positions 0-19 are a product of 0/1 genome features and signed pattern
+// weights: per position with genome nibble n (T C A G = 1 2 4 8) the
+// features are the bytes n & 1, n & 2, n & 4 (one AND of a bit4 word gives
+// four of them), and a pattern base b weighs them T (4,0,0), C (0,2,0),
+// A (0,0,1), G (-4,-2,-1), so feature . weight = 4 [n == b] - 4 [b == G].
+// With a bias feature (genome 1, pattern 4 (#G - Lw + M), #G and Lw counted
+// over window positions 0-19) the score of a window and a pattern is
+// 4 (M - mismatches on positions 0-19): >= 0 for every hit, and exactly the
+// hits when the window has at most 20 positions.  Windows that score >= 0
+// are checked on the 2-bit planes before they are recorded.
+//
+// One v_mfma_i32_16x16x64_i8 scores 16 windows (rows) against 16 patterns
+// (columns) over 64 K bytes.  Lane l holds row/column l % 16 and K bytes
+// 16 (l / 16) .. +15 of both operands; output v of lane l is row
+// 4 (l / 16) + v, column l % 16.  Lane group c = l / 16 reads one bit4 word
+// at window position s_c = 0, 3, 10, 13: bytes 0-11 are the features of its
+// even nibbles (positions s_c + 2e, e = 0..3; three ANDs), bytes 12-14 those
+// of nibble x_c = 1, 5, 1, 5 (position s_c + x_c) and byte 15 is the bias
+// (c = 3), so the four groups cover positions 0-19 once each.  Pattern
+// fragments (fillers: weights 0, bias -128) are built on the host, 16 bytes
+// per lane per block of 16 patterns.
+typedef int32_t mf_v4i __attribute__((ext_vector_type(4)));
+
+// One chunk of NB pattern blocks (held in registers) against every window of
+// the list: per 16 windows, NB independent MFMAs and an AND of their scores;
+// the hit path runs only when some score is >= 0.
+template <int NB, int THREADS>
+__device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* ghi, const uint32_t* glo,
+                                           const uint16_t* cand, uint32_t nc, const GroupDev& G, uint32_t bc,
+                                           const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
+                                           const PatRec* __restrict__ pats, const uint32_t* __restrict__ full_words,
+                                           uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
+                                           uint32_t lane, uint32_t wave) {
+    const uint32_t lg = lane >> 4, r = lane & 15u;
+    const uint32_t start = G.pk_lo + (lg == 0 ? 0u : lg == 1 ? 3u : lg == 2 ? 10u : 13u);
+    const uint32_t xsh = (lg & 1u) ? 20u : 4u;   // nibble x_c of the word
+    const uint32_t bias_one = lg == 3 ? 0x01000000u : 0u;
+    mf_v4i bf[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
+        bf[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
+    }
+    for (uint32_t cb = wave * 16u; cb < nc; cb += 16u * (THREADS / 64)) {
+        // rows past the list repeat its last window; their outputs are
+        // dropped below
+        const uint32_t p = (uint32_t)cand[min(cb + r, nc - 1u)] + start;
+        const uint32_t B = __builtin_amdgcn_alignbit(gw[(p >> 3) + 1], gw[p >> 3], p << 2);
+        const uint32_t n = __builtin_amdgcn_ubfe(B, xsh, 4u);
+        const mf_v4i a = {(int32_t)(B & 0x01010101u), (int32_t)(B & 0x02020202u), (int32_t)(B & 0x04040404u),
+                          (int32_t)(((n * 0x010101u) & 0x040201u) | bias_one)};
+        mf_v4i d[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            d[b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
+        // sign bit of the AND of all scores: clear iff some score >= 0
+        uint32_t red = 0xFFFFFFFFu;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[b][0], (uint32_t)d[b][1], 0x80);
+            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[b][2], (uint32_t)d[b][3], 0x80);
+        }
+        if (!__ballot((int32_t)red >= 0)) continue;
+        // window row 4 lg + v against pattern column r of block bc + b: the
+        // plane compare decides, then the hit is recorded
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const uint32_t c = cb + 4u * lg + (uint32_t)v;
+                if (d[b][v] < 0 || c >= nc) continue;
+                const PlaneRec& rec = prec[G.prec_begin + (bc + b) * 16u + r];
+                const uint32_t i = cand[c];
+                const uint32_t q = i + G.pk_lo;
+                const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
+                const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                if ((uint32_t)__popc((gh ^ rec.ph) | (gl ^ rec.pl)) + rec.K > 64u) continue;
+                const uint32_t amm = recount_swar(gw, i, full_words + (size_t)rec.plan * 8, G.L);
+                const uint32_t idx = atomicAdd(out_count, 1u);
+                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[rec.plan].pid, amm};
+            }
+    }
+}
+
+// The group's pattern blocks [pb, pe) in chunks of up to 8 blocks.
+template <int THREADS>
+__device__ __forceinline__ void mfma_compare(const uint32_t* gw, const uint32_t* ghi, const uint32_t* glo,
+                                             const uint16_t* cand, uint32_t nc, const GroupDev& G, uint32_t pb,
+                                             uint32_t pe, const uint4* __restrict__ mfrag,
+                                             const PlaneRec* __restrict__ prec, const PatRec* __restrict__ pats,
+                                             const uint32_t* __restrict__ full_words, uint64_t s0, PlainRaw* out,
+                                             uint32_t* out_count, uint32_t out_cap, uint32_t lane, uint32_t wave) {
+    const uint32_t blk_lo = (pb - G.pat_begin) >> 4;
+    const uint32_t blk_hi = (pe - G.pat_begin + 15u) >> 4;
+    for (uint32_t bc = blk_lo; bc < blk_hi; bc += 8) {
+        const uint32_t nb = min(8u, blk_hi - bc);   // uniform
+#define COF_MFMA_CHUNK(N)                                                                                        \
+    case N:                                                                                                      \
+        mfma_chunk<N, THREADS>(gw, ghi, glo, cand, nc, G, bc, mfrag, prec, pats, full_words, s0, out, out_count, \
+                               out_cap, lane, wave);                                                             \
+        break;
+        switch (nb) {
+            COF_MFMA_CHUNK(1) COF_MFMA_CHUNK(2) COF_MFMA_CHUNK(3) COF_MFMA_CHUNK(4)
+            COF_MFMA_CHUNK(5) COF_MFMA_CHUNK(6) COF_MFMA_CHUNK(7) COF_MFMA_CHUNK(8)
+            default: break;
+        }
+#undef COF_MFMA_CHUNK
+    }
+}
+
 template <int W, int THREADS>
 __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
@@ -190,6 +307,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
     const PlaneRec* __restrict__ prec,        // [pattern] 2-bit planes (groups with planes)
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
+    const uint4* __restrict__ mfrag,          // matrix-core pattern fragments
     PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
     uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
     constexpr int kTile = THREADS * 64;
@@ -379,6 +497,12 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         tile_sync<kThreads>();
         // 3. compare windows against the group's patterns
         const bool planes = plain_tile && G.planes;   // uniform
+        if (planes && G.mfma && (pat_chunk & 15u) == 0) {
+            mfma_compare<kThreads>(gw, ghi, glo, cand, nc, G, pb, pe, mfrag, prec, pats, full_words, s0, out,
+                                   out_count, out_cap, lane, wave);
+            tile_sync<kThreads>();
+            continue;
+        }
         for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
             uint32_t c = c0 + lane;
             bool active = c < nc;
@@ -538,8 +662,75 @@ struct PlainPlan {
     PatRec* d_pats = nullptr;
     PlaneRec* d_prec = nullptr;
     uint32_t* d_full = nullptr;
+    uint4* d_frag = nullptr;
     uint32_t max_L = 0;
 };
+
+namespace {
+
+// Matrix-core operand fragments of a plane group (layout and weights as in
+// mfma_chunk): for each block of 16 plane records, 64 lanes x 16 bytes.
+// Returns false (the group stays on the plane compare) when a bias does not
+// fit a signed byte.
+bool build_mfma_frags(GroupDev& G, const std::vector<const Pattern*>& gp, std::vector<uint4>& frags) {
+    if (!G.planes || G.wmask == 0) return false;
+    static const uint32_t kStart[4] = {0, 3, 10, 13}, kExtra[4] = {1, 5, 1, 5};
+    static const int kWeight[4][3] = {{4, 0, 0}, {0, 2, 0}, {0, 0, 1}, {-4, -2, -1}};   // T C A G
+    const uint32_t m20 = G.wmask & 0xFFFFFu;   // the filter's positions
+    auto weight = [&](const Pattern& p, uint32_t pos, int k) -> int {
+        if (!((m20 >> pos) & 1u)) return 0;
+        const uint32_t b = p.b4[G.pk_lo + pos];
+        return kWeight[b == 1 ? 0 : b == 2 ? 1 : b == 4 ? 2 : 3][k];
+    };
+    const int Lw = __builtin_popcount(m20);
+    std::vector<int> bias(gp.size());
+    for (size_t i = 0; i < gp.size(); i++) {
+        int nG = 0;
+        for (uint32_t pos = 0; pos < 20; pos++)
+            if (((m20 >> pos) & 1u) && gp[i]->b4[G.pk_lo + pos] == 8) nG++;
+        bias[i] = 4 * (nG - Lw + (int)std::min<uint32_t>(gp[i]->M, 64u));
+        if (bias[i] < -128 || bias[i] > 127) return false;
+    }
+    if (G.wmask >> 20) {
+        // wider windows: the filter passes pairs with <= M mismatches on
+        // positions 0-19, each checked on the planes.  Expected passes per 16
+        // windows on uniform bases (binomial tail, P(match) = 1/4) must stay
+        // small, or the plane compare alone is cheaper.
+        double passes = 0;
+        for (const Pattern* p : gp) {
+            double tail = 0, c = 1;   // c = C(Lw, k)
+            for (int k = 0; k <= std::min<int>((int)p->M, Lw); k++) {
+                tail += c * std::pow(0.75, k) * std::pow(0.25, Lw - k);
+                c = c * (Lw - k) / (k + 1);
+            }
+            passes += 16.0 * tail;
+        }
+        if (passes > 0.25) return false;
+    }
+    G.mfma = 1;
+    G.frag_begin = (uint32_t)frags.size();
+    const uint32_t nblk = G.prec_padded / 16;
+    for (uint32_t blk = 0; blk < nblk; blk++)
+        for (uint32_t lane = 0; lane < 64; lane++) {
+            const uint32_t c = lane >> 4, slot = blk * 16 + (lane & 15u);
+            int8_t by[16] = {0};
+            if (slot < gp.size()) {
+                const Pattern& p = *gp[slot];
+                for (int d = 0; d < 3; d++)
+                    for (uint32_t e = 0; e < 4; e++) by[4 * d + e] = (int8_t)weight(p, kStart[c] + 2 * e, d);
+                for (int e = 0; e < 3; e++) by[12 + e] = (int8_t)weight(p, kStart[c] + kExtra[c], e);
+                if (c == 3) by[15] = (int8_t)bias[slot];
+            } else if (c == 3) {
+                by[15] = -128;   // filler: never passes
+            }
+            uint4 v;
+            std::memcpy(&v, by, 16);
+            frags.push_back(v);
+        }
+    return true;
+}
+
+}  // namespace
 
 PlainPlan* plain_plan_create(const Query& q) {
     auto* pp = new PlainPlan();
@@ -556,6 +747,7 @@ PlainPlan* plain_plan_create(const Query& q) {
     std::vector<PatRec> recs;
     std::vector<PlaneRec> precs;
     std::vector<uint32_t> full;
+    std::vector<uint4> frags;
     std::map<int, std::vector<GroupDev>> by_w;
     uint32_t pat_index = 0;
     for (auto& kv : by_key) {
@@ -637,6 +829,9 @@ PlainPlan* plain_plan_create(const Query& q) {
         }
         while ((precs.size() - G.prec_begin) % 16) precs.push_back(PlaneRec{0, 0, 200u, G.pat_begin});
         G.prec_padded = (uint32_t)(precs.size() - G.prec_begin);
+        std::vector<const Pattern*> gpats;
+        for (uint32_t pi : kv.second) gpats.push_back(&q.pats[pi]);
+        build_mfma_frags(G, gpats, frags);
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
@@ -656,6 +851,7 @@ PlainPlan* plain_plan_create(const Query& q) {
         up(pp->d_pats, recs);
         up(pp->d_prec, precs);
         up(pp->d_full, full);
+        if (!frags.empty()) up(pp->d_frag, frags);
     }
     return pp;
 }
@@ -666,6 +862,7 @@ void plain_plan_destroy(PlainPlan* pp) {
     (void)hipFree(pp->d_pats);
     (void)hipFree(pp->d_prec);
     (void)hipFree(pp->d_full);
+    (void)hipFree(pp->d_frag);
     delete pp;
 }
 
@@ -679,11 +876,11 @@ namespace {
 template <int W>
 void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
                   uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const PlaneRec* prec,
-                  const uint32_t* full, PlainRaw* out, uint32_t* count, uint32_t cap) {
+                  const uint32_t* full, const uint4* frag, PlainRaw* out, uint32_t* count, uint32_t cap) {
 #define COF_PLAIN_SHAPE(T)                                                                                  \
     case T:                                                                                                 \
         hipLaunchKernelGGL((plain_scan_kernel<W, T>), grid, dim3(T), 0, st, d_genome, scan_end, first, g, n_groups, \
-                           pats, prec, full, out, count, cap, t.pat_chunk);                                 \
+                           pats, prec, full, frag, out, count, cap, t.pat_chunk);                           \
         break;
     switch (t.threads) {
         COF_PLAIN_SHAPE(64) COF_PLAIN_SHAPE(128) COF_PLAIN_SHAPE(256) COF_PLAIN_SHAPE(512)
@@ -805,7 +1002,7 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
         launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, d_out, ws.plain_count, seg_cap);                                         \
+                         pp->d_full, pp->d_frag, d_out, ws.plain_count, seg_cap);                             \
         break;
             switch (l.W) {
                 COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)

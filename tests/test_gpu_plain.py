@@ -196,3 +196,43 @@ def test_async_search_matches_sync(tmp_path):
     a.close()
     b.close()
     assert want_a > want_b > 0
+
+
+@pytest.mark.parametrize("glen,n,mms,threads,chunk", [
+    (20, 150, (0, 2, 4, 6, 7), 64, 0),     # 10 blocks of 16 patterns per strand: passes of 8 + 2
+    (20, 150, (1, 5), 256, 48),            # slices of 3 blocks, four waves per tile
+    (25, 37, (3, 5, 6), 64, 0),            # 25-position window: positions 0-19 filter, planes decide
+    (17, 20, (5,), 128, 16),               # one block per slice
+])
+def test_matrix_core_pattern_blocks(tmp_path, glen, n, mms, threads, chunk):
+    # Plane groups on plain-base tiles are scored 16 windows x 16 patterns per
+    # matrix-core product over window positions 0-19 (exact up to 20
+    # positions, a filter checked on the planes beyond), with mixed mismatch
+    # limits per pattern and filler patterns past the group's last one; rows
+    # must equal the oracle's.
+    import random
+    rng = random.Random(glen * 1000 + n)
+    guides = ["".join(rng.choice("ACGT") for _ in range(glen)) for _ in range(n)]
+    mm = [mms[i % len(mms)] for i in range(n)]
+    filt = "N" * (glen + 1) + "RG"
+    pats = [x + "NNN" for x in guides]
+    plants = [(x + "AGG", 0, glen, min(m + 1, 8), 2) for x, m in zip(guides, mm)]
+    sg = cofpy.SynthGenome(glen + n, [200_001, 70_007], line_width=61, n_frac=0.01, gap=(30, 900), plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    # the reference takes one mismatch limit per input file: one oracle run
+    # per limit, rows merged
+    want = []
+    for m in sorted(set(mm)):
+        in_text = f"{fa}\n{filt}\n" + "".join(f"{p} {m}\n" for p, pm in zip(pats, mm) if pm == m)
+        oracle.run(in_text, tmp_path / f"ref{m}.tsv", batch_nucl=1 << 22)
+        want += cofpy.read_rows(tmp_path / f"ref{m}.tsv")
+    want.sort()
+    s = cofpy.Search(pats, [filt] * n, mm, batch_nucl=1 << 22)
+    s.set_tuning(plain_threads=threads, plain_pat_chunk=chunk)
+    s.load_file(fa)
+    s.search()
+    s.write_tsv(tmp_path / "gpu.tsv")
+    s.close()
+    assert len(want) > 0
+    assert cofpy.read_rows(tmp_path / "gpu.tsv") == want
