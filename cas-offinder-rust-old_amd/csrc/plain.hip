@@ -233,10 +233,10 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
         const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
         bf[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
     }
+    // the list is padded to whole blocks; rows past the list are dropped in
+    // the hit path
     for (uint32_t cb = wave * 16u; cb < nc; cb += 16u * (THREADS / 64)) {
-        // rows past the list repeat its last window; their outputs are
-        // dropped below
-        const uint32_t p = (uint32_t)cand[min(cb + r, nc - 1u)] + start;
+        const uint32_t p = (uint32_t)cand[cb + r] + start;
         const uint32_t B = __builtin_amdgcn_alignbit(gw[(p >> 3) + 1], gw[p >> 3], p << 2);
         const uint32_t n = __builtin_amdgcn_ubfe(B, xsh, 4u);
         const mf_v4i a = {(int32_t)(B & 0x01010101u), (int32_t)(B & 0x02020202u), (int32_t)(B & 0x04040404u),
@@ -314,14 +314,23 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     constexpr int kTileWords = kTile / 8 + kHaloWords;
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
     constexpr int kThreads = THREADS;
-    __shared__ uint32_t gw[kTileWords];
-    __shared__ uint32_t ghi[kPlaneWords], glo[kPlaneWords];
-    __shared__ uint16_t cand[kTile];
-    __shared__ uint32_t wave_sum[kThreads / 64];
+    // one LDS block, the bit4 words at offset 0 (LDS addresses need no base)
+    struct TileLds {
+        uint32_t gw[kTileWords];
+        uint32_t ghi[kPlaneWords], glo[kPlaneWords];
+        uint16_t cand[kTile];
+        uint32_t wave_sum[kThreads / 64];
+    };
+    __shared__ TileLds lds;
+    uint32_t* const gw = lds.gw;
+    uint32_t* const ghi = lds.ghi;
+    uint32_t* const glo = lds.glo;
+    uint16_t* const cand = lds.cand;
+    uint32_t* const wave_sum = lds.wave_sum;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
-    const uint32_t wave = tid >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: scalar loops over waves
     const uint64_t s0 = first_tile_nib + (uint64_t)blockIdx.x * kTile;
     const uint64_t w0 = s0 >> 3;
     // this workgroup's hit segment
@@ -494,10 +503,14 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 }
             }
         }
+        const bool planes = plain_tile && G.planes;   // uniform
+        const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
+        // the matrix-core compare reads whole blocks of 16 list entries: pad
+        // the list with start 0 (outputs of rows past the list are dropped)
+        if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
         tile_sync<kThreads>();
         // 3. compare windows against the group's patterns
-        const bool planes = plain_tile && G.planes;   // uniform
-        if (planes && G.mfma && (pat_chunk & 15u) == 0) {
+        if (mfma) {
             mfma_compare<kThreads>(gw, ghi, glo, cand, nc, G, pb, pe, mfrag, prec, pats, full_words, s0, out,
                                    out_count, out_cap, lane, wave);
             tile_sync<kThreads>();
