@@ -75,6 +75,8 @@ uint32_t* alloc_genome(cof_ctx* ctx, uint64_t n_bytes) {
 }
 
 void finish_genome(cof_ctx* ctx) {
+    ctx->ws.img_units = 0;   // the scan image follows the new genome at the next plain scan
+    ctx->ws.genome_nib = ctx->g.n_nibbles();
     ctx->g.index();
     ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
     ctx->have_genome = true;

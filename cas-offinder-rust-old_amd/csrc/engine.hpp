@@ -21,6 +21,15 @@ struct Workspace {
     size_t h_counts_cap = 0;
     bool plain_pending = false;    // a launched scan awaits plain_finish
     uint64_t plain_scan_end = 0;
+    // scan image of the resident genome (plain.hip): its two bit planes and
+    // one flag byte per 4096-start unit, built by the first plain scan after
+    // a genome load and reused by later scans
+    uint32_t* img_hi = nullptr;
+    uint32_t* img_lo = nullptr;
+    uint8_t* img_flags = nullptr;
+    uint64_t img_units = 0;        // units held (0: no valid image)
+    uint64_t img_cap_units = 0;    // units allocated
+    uint64_t genome_nib = 0;       // nibbles of the resident genome (the image covers them all)
     void* queue = nullptr;         // bulge candidates
     uint32_t q_cap = 0;
     void* bulge_out = nullptr;     // BulgeRaw records

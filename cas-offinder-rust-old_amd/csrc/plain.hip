@@ -308,6 +308,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const PlaneRec* __restrict__ prec,        // [pattern] 2-bit planes (groups with planes)
     const uint32_t* __restrict__ full_words,  // [pattern][8] full pattern nibbles
     const uint4* __restrict__ mfrag,          // matrix-core pattern fragments
+    const uint32_t* __restrict__ img_hi, const uint32_t* __restrict__ img_lo,   // scan image planes
+    const uint8_t* __restrict__ img_flags,    // scan image flags per 4096-start unit
     PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
     uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
     constexpr int kTile = THREADS * 64;
@@ -338,13 +340,19 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
     const uint32_t out_cap = seg_cap;
 
-    // 1. genome tile -> LDS (N-masked) as bit4 words and as two 32-position
-    //    bit planes (base bit 1 = A/G, base bit 0 = C/G), with IUPAC and
-    //    zero-nibble (masked N, padding) detection.  Thread tid takes the
-    //    16-byte groups tid + k * kThreads: one group of 4 words is one plane
-    //    word, and every load of the thread is issued before any is used.
+    // 1. genome tile -> LDS.  The genome's scan image gives the tile's flags
+    //    (IUPAC letters; zero nibbles: masked N, padding) as the OR of its
+    //    4096-start units, and its two 32-position bit planes (base bit 1 =
+    //    A/G, base bit 0 = C/G), which serve tiles of plain bases only.  The
+    //    bit4 words are N-masked (0xF -> 0, search.rs:512-531) where the tile
+    //    has zero nibbles; a tile without them holds no N.  Every load of a
+    //    thread is issued before any is used.
+    int flags = 0;
+#pragma unroll
+    for (int u = 0; u < kThreads / 64; u++) flags |= img_flags[(s0 >> 12) + u];   // uniform
     constexpr int kGroups4 = (kTileWords + 3) / 4;
-    constexpr int kLoadIt = (kPlaneWords + kThreads - 1) / kThreads;
+    constexpr int kLoadIt = (kGroups4 + kThreads - 1) / kThreads;
+    constexpr int kPlaneIt = (kPlaneWords + kThreads - 1) / kThreads;
     const uint4* const g4 = reinterpret_cast<const uint4*>(genome + w0);
     uint4 lv[kLoadIt];
 #pragma unroll
@@ -352,41 +360,42 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         const uint32_t j = tid + (uint32_t)it * kThreads;
         lv[it] = j < (uint32_t)kGroups4 ? g4[j] : make_uint4(0u, 0u, 0u, 0u);
     }
-    int flags = 0;   // bit 0: an IUPAC letter, bit 1: a zero nibble (masked N, padding)
+    uint32_t ph[kPlaneIt], pl[kPlaneIt];
+    if (flags == 0) {
+#pragma unroll
+        for (int it = 0; it < kPlaneIt; it++) {
+            const uint32_t j = tid + (uint32_t)it * kThreads;
+            const bool in = j < (uint32_t)kPlaneWords;
+            ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
+            pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
+        }
+    }
 #pragma unroll
     for (int it = 0; it < kLoadIt; it++) {
         const uint32_t j = tid + (uint32_t)it * kThreads;
-        if (j >= (uint32_t)kPlaneWords) continue;
-        const uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
-        uint32_t h = 0, l = 0;
+        uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
+        if (flags) {
 #pragma unroll
-        for (uint32_t b = 0; b < 4; b++) {
-            const uint32_t idx = 4 * j + b;
-            uint32_t w = raw[b];
-            const uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
-            w &= ~(f * 0xFu);
-            if (idx < (uint32_t)kTileWords) {
-                // a nibble with two or more bits set makes the word hold more
-                // bits than nonzero nibbles
-                const uint32_t nz = nz_nibbles(w);
-                flags |= (__popc(w) != __popc(nz) ? 1 : 0) | (nz != 0x88888888u ? 2 : 0);
-                gw[idx] = w;
-            } else {
-                w = 0;
+            for (int b = 0; b < 4; b++) {
+                const uint32_t f = raw[b] & (raw[b] >> 1) & (raw[b] >> 2) & (raw[b] >> 3) & 0x11111111u;   // nibble == 0xF
+                raw[b] &= ~(f * 0xFu);
             }
-            h |= gather_nibble_bit0(((w >> 2) | (w >> 3)) & 0x11111111u) << (8 * b);
-            l |= gather_nibble_bit0(((w >> 1) | (w >> 3)) & 0x11111111u) << (8 * b);
         }
-        ghi[j] = h;
-        glo[j] = l;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (4 * j + b < (uint32_t)kTileWords) gw[4 * j + b] = raw[b];
     }
-    if constexpr (kThreads == 64) {
-        // one wave: two ballots
-        tile_sync<kThreads>();
-        flags = (__ballot(flags & 1) != 0 ? 1 : 0) | (__ballot(flags & 2) != 0 ? 2 : 0);
-    } else {
-        flags = __syncthreads_or(flags & 1) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    if (flags == 0) {
+#pragma unroll
+        for (int it = 0; it < kPlaneIt; it++) {
+            const uint32_t j = tid + (uint32_t)it * kThreads;
+            if (j < (uint32_t)kPlaneWords) {
+                ghi[j] = ph[it];
+                glo[j] = pl[it];
+            }
+        }
     }
+    tile_sync<kThreads>();
     const int iupac = flags & 1;
     // the planes serve tiles of plain bases only
     const bool plain_tile = flags == 0;
@@ -643,6 +652,48 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     }
 }
 
+// Scan image of the genome, one wave per 4096-start unit u: the flags of
+// words [512 u, 512 u + 512 + halo) after the N-mask (bit 0: an IUPAC letter,
+// i.e. a nibble with two or more bits; bit 1: a zero nibble, i.e. masked N or
+// padding) and the two bit planes of plane words [128 u, 128 u + 128)
+// (positions 32 j .. 32 j + 31; base bit 1 = A/G, base bit 0 = C/G).  A scan
+// tile of k units takes the OR of their flags, which covers its words and
+// its halo.
+__global__ __launch_bounds__(64) void scan_image_kernel(const uint32_t* __restrict__ genome, uint32_t* __restrict__ hi,
+                                                         uint32_t* __restrict__ lo, uint8_t* __restrict__ flags_out) {
+    constexpr uint32_t kUnitWords = 512;
+    constexpr uint32_t kFlagWords = kUnitWords + kHaloWords;
+    const uint64_t u = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint4* const g4 = reinterpret_cast<const uint4*>(genome + u * kUnitWords);
+    int fl = 0;
+    for (uint32_t j = tid; j < (kFlagWords + 3) / 4; j += 64) {
+        const uint4 v = g4[j];
+        const uint32_t raw[4] = {v.x, v.y, v.z, v.w};
+        uint32_t h = 0, l = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) {
+            uint32_t w = raw[b];
+            const uint32_t f = w & (w >> 1) & (w >> 2) & (w >> 3) & 0x11111111u;   // nibble == 0xF
+            w &= ~(f * 0xFu);
+            if (4 * j + b < kFlagWords) {
+                // a nibble with two or more bits set makes the word hold more
+                // bits than nonzero nibbles
+                const uint32_t nz = nz_nibbles(w);
+                fl |= (__popc(w) != __popc(nz) ? 1 : 0) | (nz != 0x88888888u ? 2 : 0);
+            }
+            h |= gather_nibble_bit0(((w >> 2) | (w >> 3)) & 0x11111111u) << (8 * b);
+            l |= gather_nibble_bit0(((w >> 1) | (w >> 3)) & 0x11111111u) << (8 * b);
+        }
+        if (j < kUnitWords / 4) {
+            hi[u * (kUnitWords / 4) + j] = h;
+            lo[u * (kUnitWords / 4) + j] = l;
+        }
+    }
+    fl = (__ballot(fl & 1) != 0 ? 1 : 0) | (__ballot(fl & 2) != 0 ? 2 : 0);
+    if (tid == 0) flags_out[u] = (uint8_t)fl;
+}
+
 // Packs the hit segments into one array: segment s goes after segments
 // 0..s-1 (counts already checked against the segment capacity).
 __global__ __launch_bounds__(256) void plain_compact_kernel(const PlainRaw* __restrict__ in,
@@ -889,11 +940,13 @@ namespace {
 template <int W>
 void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
                   uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const PlaneRec* prec,
-                  const uint32_t* full, const uint4* frag, PlainRaw* out, uint32_t* count, uint32_t cap) {
+                  const uint32_t* full, const uint4* frag, const Workspace& ws, PlainRaw* out, uint32_t* count,
+                  uint32_t cap) {
 #define COF_PLAIN_SHAPE(T)                                                                                  \
     case T:                                                                                                 \
         hipLaunchKernelGGL((plain_scan_kernel<W, T>), grid, dim3(T), 0, st, d_genome, scan_end, first, g, n_groups, \
-                           pats, prec, full, frag, out, count, cap, t.pat_chunk);                           \
+                           pats, prec, full, frag, ws.img_hi, ws.img_lo, ws.img_flags, out, count, cap,      \
+                           t.pat_chunk);                                                                    \
         break;
     switch (t.threads) {
         COF_PLAIN_SHAPE(64) COF_PLAIN_SHAPE(128) COF_PLAIN_SHAPE(256) COF_PLAIN_SHAPE(512)
@@ -910,7 +963,7 @@ Workspace::~Workspace() {
     for (void* p : {plain_out, plain_out2, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
                     qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
                     (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
-                    (void*)round_ctr})
+                    (void*)round_ctr, (void*)img_hi, (void*)img_lo, (void*)img_flags})
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : rev) if (e) (void)hipEventDestroy(e);
@@ -974,6 +1027,30 @@ ScanOrder& scan_order() {
     return o;
 }
 
+// Builds the scan image for starts [0, n_nib) unless the current one covers
+// them: units up to the last one of the widest tile shape, plus one for the
+// planes of the last tile's halo.
+void ensure_scan_image(Workspace& ws, const uint32_t* d_genome, uint64_t n_nib, hipStream_t st) {
+    const uint64_t widest = plain_tile_nucl();
+    const uint64_t units = (n_nib + widest - 1) / widest * (widest / 4096) + 1;
+    if (ws.img_units >= units) return;
+    if (ws.img_cap_units < units) {
+        for (void* p : {(void*)ws.img_hi, (void*)ws.img_lo, (void*)ws.img_flags})
+            if (p) (void)hipFree(p);
+        ws.img_hi = ws.img_lo = nullptr;
+        ws.img_flags = nullptr;
+        ws.img_cap_units = 0;
+        HIPCHECK(hipMalloc(&ws.img_hi, units * 128 * sizeof(uint32_t)));
+        HIPCHECK(hipMalloc(&ws.img_lo, units * 128 * sizeof(uint32_t)));
+        HIPCHECK(hipMalloc(&ws.img_flags, units));
+        ws.img_cap_units = units;
+    }
+    hipLaunchKernelGGL(scan_image_kernel, dim3((uint32_t)units), dim3(64), 0, st, d_genome, ws.img_hi, ws.img_lo,
+                       ws.img_flags);
+    HIPCHECK(hipGetLastError());
+    ws.img_units = units;
+}
+
 }  // namespace
 
 // Queues the scan of start positions [0, scan_end) and the copy of the
@@ -1002,6 +1079,7 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
     hipEvent_t& last = order.last[dev & 63];
     if (last) HIPCHECK(hipStreamWaitEvent(st, last, 0));
     else HIPCHECK(hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    ensure_scan_image(ws, d_genome, std::max(scan_end, ws.genome_nib), st);
     HIPCHECK(hipMemsetAsync(ws.plain_count, 0, kHitSegs * kSegStride * sizeof(uint32_t), st));
     HIPCHECK(hipEventRecord(ws.ev[0], st));
     for (const auto& l : pp->launches) {
@@ -1015,7 +1093,7 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
         launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, pp->d_frag, d_out, ws.plain_count, seg_cap);                             \
+                         pp->d_full, pp->d_frag, ws, d_out, ws.plain_count, seg_cap);                         \
         break;
             switch (l.W) {
                 COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
