@@ -241,26 +241,25 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
         const uint32_t n = __builtin_amdgcn_ubfe(B, xsh, 4u);
         const mf_v4i a = {(int32_t)(B & 0x01010101u), (int32_t)(B & 0x02020202u), (int32_t)(B & 0x04040404u),
                           (int32_t)(((n * 0x010101u) & 0x040201u) | bias_one)};
-        mf_v4i d[NB];
-#pragma unroll
-        for (int b = 0; b < NB; b++)
-            d[b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
         // sign bit of the AND of all scores: clear iff some score >= 0
         uint32_t red = 0xFFFFFFFFu;
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[b][0], (uint32_t)d[b][1], 0x80);
-            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[b][2], (uint32_t)d[b][3], 0x80);
+            const mf_v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
+            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[0], (uint32_t)d[1], 0x80);
+            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[2], (uint32_t)d[3], 0x80);
         }
         if (!__ballot((int32_t)red >= 0)) continue;
-        // window row 4 lg + v against pattern column r of block bc + b: the
+        // window row 4 lg + v against pattern column r of block bc + b (scores
+        // computed again, so they need no registers in the common case): the
         // plane compare decides, then the hit is recorded
 #pragma unroll
-        for (int b = 0; b < NB; b++)
+        for (int b = 0; b < NB; b++) {
+            const mf_v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const uint32_t c = cb + 4u * lg + (uint32_t)v;
-                if (d[b][v] < 0 || c >= nc) continue;
+                if (d[v] < 0 || c >= nc) continue;
                 const PlaneRec& rec = prec[G.prec_begin + (bc + b) * 16u + r];
                 const uint32_t i = cand[c];
                 const uint32_t q = i + G.pk_lo;
@@ -271,6 +270,7 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
                 const uint32_t idx = atomicAdd(out_count, 1u);
                 if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[rec.plan].pid, amm};
             }
+        }
     }
 }
 
