@@ -320,7 +320,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     struct TileLds {
         uint32_t gw[kTileWords];
         uint32_t ghi[kPlaneWords], glo[kPlaneWords];
-        uint16_t cand[kTile];
+        uint16_t cand[kTile / 2];   // one half tile's windows
         uint32_t wave_sum[kThreads / 64];
     };
     __shared__ TileLds lds;
@@ -416,7 +416,6 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         }
         // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
         uint32_t m8[8];
-        uint32_t cnt = 0;
         if (plain_tile) {
             // planes: one v_bitop3 per PAM cell per 32 starts
             uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
@@ -437,8 +436,9 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 if (scan_end <= first) m = 0;
                 else if (scan_end < first + 32) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
                 m8[w] = m;
-                cnt += (uint32_t)__popc(m);
             }
+#pragma unroll
+            for (int w = 2; w < 8; w++) m8[w] = 0;
         } else {
             // bit4, 8 positions per word: nibble j of X_k is the genome at
             // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
@@ -465,190 +465,217 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 if (scan_end <= first) m = 0;
                 else if (scan_end < first + 8) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
                 m8[w] = m;
-                cnt += (uint32_t)__popc(m);
             }
         }
-        // block-wide exclusive scan of the per-thread counts (each <= 64):
-        // within the wave bit by bit, a ballot per bit and its lane count
-        // below (mbcnt), so no cross-lane permute waits on LDS
-        uint32_t excl = 0, wtot = 0;
+        // block-wide exclusive scan of per-thread counts (each <= 64): within
+        // the wave bit by bit, a ballot per bit and its lane count below
+        // (mbcnt), so no cross-lane permute waits on LDS; nc = the total
+        auto block_scan = [&](uint32_t cnt, uint32_t& nc, uint32_t& pos) {
+            uint32_t excl = 0, wtot = 0;
 #pragma unroll
-        for (int b = 0; b < 7; b++) {
-            const uint64_t bb = __ballot((cnt >> b) & 1u);
-            excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u)) << b;
-            wtot += (uint32_t)__popcll(bb) << b;
-        }
-        uint32_t nc = wtot, off = 0;
-        if (kThreads > 64) {
-            if (lane == 0) wave_sum[wave] = wtot;
-            __syncthreads();
-            nc = 0;
-#pragma unroll
-            for (int x = 0; x < kThreads / 64; x++) {
-                uint32_t ws = wave_sum[x];
-                off += (uint32_t)x < wave ? ws : 0u;
-                nc += ws;
+            for (int b = 0; b < 7; b++) {
+                const uint64_t bb = __ballot((cnt >> b) & 1u);
+                excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u)) << b;
+                wtot += (uint32_t)__popcll(bb) << b;
             }
-        }
-        uint32_t pos = off + excl;
-        if (plain_tile) {
+            uint32_t off = 0;
+            nc = wtot;
+            if (kThreads > 64) {
+                if (lane == 0) wave_sum[wave] = wtot;
+                __syncthreads();
+                nc = 0;
 #pragma unroll
-            for (int w = 0; w < 2; w++) {
-                uint32_t m = m8[w];
-                while (m) {
-                    uint32_t j = (uint32_t)__builtin_ctz(m);
-                    cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 32u + j);
-                    m &= m - 1u;
+                for (int x = 0; x < kThreads / 64; x++) {
+                    uint32_t ws = wave_sum[x];
+                    off += (uint32_t)x < wave ? ws : 0u;
+                    nc += ws;
                 }
+                __syncthreads();   // wave_sum is written again by the next scan
             }
-        } else {
+            pos = off + excl;
+        };
+        uint32_t cnt_all = 0;
+#pragma unroll
+        for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
+        uint32_t nc_all, pos_all;
+        block_scan(cnt_all, nc_all, pos_all);
+        // the window list holds half a tile: the tile's windows are compared
+        // in one pass when they fit, else in two passes over the halves of
+        // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3)
+        const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
+        for (uint32_t h = 0; h < npass; h++) {
+            uint32_t mh[8];
 #pragma unroll
             for (int w = 0; w < 8; w++) {
-                uint32_t m = m8[w];
-                while (m) {
-                    uint32_t j = (uint32_t)__builtin_ctz(m);
-                    cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
-                    m &= m - 1u;
+                const uint32_t half = plain_tile ? (uint32_t)w : (uint32_t)w >> 2;
+                mh[w] = (npass == 1 || half == h) ? m8[w] : 0u;
+            }
+            uint32_t nc = nc_all, pos = pos_all;
+            if (npass == 2) {
+                uint32_t cnt = 0;
+#pragma unroll
+                for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
+                block_scan(cnt, nc, pos);
+            }
+            if (plain_tile) {
+#pragma unroll
+                for (int w = 0; w < 2; w++) {
+                    uint32_t m = mh[w];
+                    while (m) {
+                        uint32_t j = (uint32_t)__builtin_ctz(m);
+                        cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 32u + j);
+                        m &= m - 1u;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    uint32_t m = mh[w];
+                    while (m) {
+                        uint32_t j = (uint32_t)__builtin_ctz(m);
+                        cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
+                        m &= m - 1u;
+                    }
                 }
             }
-        }
-        const bool planes = plain_tile && G.planes;   // uniform
-        const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
-        // the matrix-core compare reads whole blocks of 16 list entries: pad
-        // the list with start 0 (outputs of rows past the list are dropped)
-        if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
-        tile_sync<kThreads>();
-        // 3. compare windows against the group's patterns
-        if (mfma) {
-            mfma_compare<kThreads>(gw, ghi, glo, cand, nc, G, pb, pe, mfrag, prec, pats, full_words, s0, out,
-                                   out_count, out_cap, lane, wave);
+            const bool planes = plain_tile && G.planes;   // uniform
+            const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
+            // the matrix-core compare reads whole blocks of 16 list entries: pad
+            // the list with start 0 (outputs of rows past the list are dropped)
+            if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
             tile_sync<kThreads>();
-            continue;
-        }
-        for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
-            uint32_t c = c0 + lane;
-            bool active = c < nc;
-            uint32_t i = active ? cand[c] : 0;
-            // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
-            auto bit4_compare = [&](bool act) {
-                uint32_t q = i + G.k_lo;
-                uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
-                uint32_t win[W];
+            // 3. compare windows against the group's patterns
+            if (mfma) {
+                mfma_compare<kThreads>(gw, ghi, glo, cand, nc, G, pb, pe, mfrag, prec, pats, full_words, s0, out,
+                                       out_count, out_cap, lane, wave);
+                tile_sync<kThreads>();
+                continue;
+            }
+            for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
+                uint32_t c = c0 + lane;
+                bool active = c < nc;
+                uint32_t i = active ? cand[c] : 0;
+                // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
+                auto bit4_compare = [&](bool act) {
+                    uint32_t q = i + G.k_lo;
+                    uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
+                    uint32_t win[W];
 #pragma unroll
-                for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                // four patterns per iteration: their records are fetched with
-                // one batch of scalar loads (pointer offsets, no per-pattern
-                // address arithmetic), and one branch covers the four hits
-                uint32_t p = pb;
-                for (; p + 4 <= pe; p += 4) {
-                    const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
-                    uint32_t hit = 0;
+                    for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
+                    // four patterns per iteration: their records are fetched with
+                    // one batch of scalar loads (pointer offsets, no per-pattern
+                    // address arithmetic), and one branch covers the four hits
+                    uint32_t p = pb;
+                    for (; p + 4 <= pe; p += 4) {
+                        const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
+                        uint32_t hit = 0;
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                        for (int u = 0; u < 4; u++) {
+                            uint32_t sum = 0;
+#pragma unroll
+                            for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
+                            hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
+                        }
+                        if (act && hit) {
+                            for (int u = 0; u < 4; u++) {
+                                if (!((hit >> u) & 1u)) continue;
+                                const PatRec& r = pats[p + u];
+                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                                uint32_t idx = atomicAdd(out_count, 1u);
+                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                            }
+                        }
+                    }
+                    for (; p < pe; p++) {
+                        const PatRec& r = pats[p];
                         uint32_t sum = 0;
 #pragma unroll
-                        for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
-                        hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
-                    }
-                    if (act && hit) {
-                        for (int u = 0; u < 4; u++) {
-                            if (!((hit >> u) & 1u)) continue;
-                            const PatRec& r = pats[p + u];
-                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                        for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
+                        if (act && (int32_t)sum >= r.thr) {
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
                             uint32_t idx = atomicAdd(out_count, 1u);
                             if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                         }
                     }
-                }
-                for (; p < pe; p++) {
-                    const PatRec& r = pats[p];
-                    uint32_t sum = 0;
+                };
+                if (planes) {
+                    // plain bases only: kmm = window positions whose planes differ
+                    // (positions that are N in every pattern match any base)
+                    const uint32_t q = i + G.pk_lo;
+                    const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
+                    const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                    const bool act = active;
+                    const uint64_t act_mask = __ballot(act);
+                    // plane records of patterns [pb, pe); a whole group runs on
+                    // its padded record count
+                    uint32_t p = G.prec_begin + (pb - G.pat_begin);
+                    const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
+                                                  : G.prec_begin + ((G.pat_count + 7u) & ~7u);
+                    // NB patterns per batch: per pattern xor, bitop3, v_bcnt
+                    // accumulating K and half a v_min3, one compare per batch; all
+                    // NB records are loaded before the batch computes
+                    auto batch = [&](auto nb_tag) {
+                        constexpr int NB = decltype(nb_tag)::value;
+                        const PlaneRec* const rp = prec + p;   // uniform base
+                        uint32_t mn = 0xFFFFFFFFu;
 #pragma unroll
-                    for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
-                    if (act && (int32_t)sum >= r.thr) {
-                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
-                        uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                    }
-                }
-            };
-            if (planes) {
-                // plain bases only: kmm = window positions whose planes differ
-                // (positions that are N in every pattern match any base)
-                const uint32_t q = i + G.pk_lo;
-                const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
-                const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
-                const bool act = active;
-                const uint64_t act_mask = __ballot(act);
-                // plane records of patterns [pb, pe); a whole group runs on
-                // its padded record count
-                uint32_t p = G.prec_begin + (pb - G.pat_begin);
-                const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
-                                              : G.prec_begin + ((G.pat_count + 7u) & ~7u);
-                // NB patterns per batch: per pattern xor, bitop3, v_bcnt
-                // accumulating K and half a v_min3, one compare per batch; all
-                // NB records are loaded before the batch computes
-                auto batch = [&](auto nb_tag) {
-                    constexpr int NB = decltype(nb_tag)::value;
-                    const PlaneRec* const rp = prec + p;   // uniform base
-                    uint32_t mn = 0xFFFFFFFFu;
-#pragma unroll
-                    for (int u = 0; u < NB; u += 2) {
-                        const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
-                        const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
-                        mn = min(mn, min(t0, t1));
-                    }
-                    if (__ballot(mn <= 64u) & act_mask) {
-                        for (int u = 0; u < NB; u++) {
-                            const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
-                            if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
-                            const uint32_t gp = rp[u].plan;
-                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)gp * 8, G.L);
+                        for (int u = 0; u < NB; u += 2) {
+                            const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
+                            const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
+                            mn = min(mn, min(t0, t1));
+                        }
+                        if (__ballot(mn <= 64u) & act_mask) {
+                            for (int u = 0; u < NB; u++) {
+                                const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
+                                if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
+                                const uint32_t gp = rp[u].plan;
+                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)gp * 8, G.L);
+                                uint32_t idx = atomicAdd(out_count, 1u);
+                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[gp].pid, amm};
+                            }
+                        }
+                        p += NB;
+                    };
+                    // batches of 16 while more than 8 records remain (a whole
+                    // group may run into its fillers, a pattern slice may not),
+                    // then at most one batch of 8, then single records
+                    const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
+                    while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
+                    if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
+                    for (; p < qe; p++) {
+                        const PlaneRec& r = prec[p];
+                        const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
+                        if (act && (uint32_t)__popc(d) + r.K <= 64u) {
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)r.plan * 8, G.L);
                             uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[gp].pid, amm};
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[r.plan].pid, amm};
                         }
                     }
-                    p += NB;
-                };
-                // batches of 16 while more than 8 records remain (a whole
-                // group may run into its fillers, a pattern slice may not),
-                // then at most one batch of 8, then single records
-                const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
-                while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
-                if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
-                for (; p < qe; p++) {
-                    const PlaneRec& r = prec[p];
-                    const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
-                    if (act && (uint32_t)__popc(d) + r.K <= 64u) {
-                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)r.plan * 8, G.L);
-                        uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[r.plan].pid, amm};
-                    }
-                }
-            } else if (!iupac) {
-                bit4_compare(active);
-            } else {
-                for (uint32_t p = pb; p < pe; p++) {
-                    const uint32_t* fw = full_words + (size_t)p * 8;
-                    int32_t pop = 0;
-                    uint32_t amm = 0;
-                    for (uint32_t k = 0; k < G.L; k++) {
-                        uint32_t q = i + k;
-                        uint32_t g = nib_of(gw[q >> 3], q & 7u);
-                        uint32_t pb = nib_of(fw[k >> 3], k & 7u);
-                        pop += __popc(pb & g);
-                        bool single = g != 0 && (g & (g - 1)) == 0;
-                        amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
-                    }
-                    uint32_t M = pats[p].M;
-                    if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
-                        uint32_t idx = atomicAdd(out_count, 1u);
-                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
+                } else if (!iupac) {
+                    bit4_compare(active);
+                } else {
+                    for (uint32_t p = pb; p < pe; p++) {
+                        const uint32_t* fw = full_words + (size_t)p * 8;
+                        int32_t pop = 0;
+                        uint32_t amm = 0;
+                        for (uint32_t k = 0; k < G.L; k++) {
+                            uint32_t q = i + k;
+                            uint32_t g = nib_of(gw[q >> 3], q & 7u);
+                            uint32_t pb = nib_of(fw[k >> 3], k & 7u);
+                            pop += __popc(pb & g);
+                            bool single = g != 0 && (g & (g - 1)) == 0;
+                            amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
+                        }
+                        uint32_t M = pats[p].M;
+                        if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
+                            uint32_t idx = atomicAdd(out_count, 1u);
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
+                        }
                     }
                 }
             }
+            tile_sync<kThreads>();
         }
-        tile_sync<kThreads>();
     }
 }
 

@@ -198,13 +198,14 @@ def test_async_search_matches_sync(tmp_path):
     assert want_a > want_b > 0
 
 
-@pytest.mark.parametrize("glen,n,mms,threads,chunk", [
-    (20, 150, (0, 2, 4, 6, 7), 64, 0),     # 10 blocks of 16 patterns per strand: passes of 8 + 2
-    (20, 150, (1, 5), 256, 48),            # slices of 3 blocks, four waves per tile
-    (25, 37, (3, 5, 6), 64, 0),            # 25-position window: positions 0-19 filter, planes decide
-    (17, 20, (5,), 128, 16),               # one block per slice
+@pytest.mark.parametrize("glen,n,mms,threads,chunk,pam", [
+    (20, 150, (0, 2, 4, 6, 7), 64, 0, "RG"),   # 10 blocks of 16 patterns per strand: passes of 8 + 2
+    (20, 150, (1, 5), 256, 48, "RG"),          # slices of 3 blocks, four waves per tile
+    (25, 37, (3, 5, 6), 64, 0, "RG"),          # 25-position window: positions 0-19 filter, planes decide
+    (17, 20, (5,), 128, 16, "RG"),             # one block per slice
+    (20, 40, (1, 3), 256, 0, "NN"),            # no PAM: every start is a window, lists of two half tiles
 ])
-def test_matrix_core_pattern_blocks(tmp_path, glen, n, mms, threads, chunk):
+def test_matrix_core_pattern_blocks(tmp_path, glen, n, mms, threads, chunk, pam):
     # Plane groups on plain-base tiles are scored 16 windows x 16 patterns per
     # matrix-core product over window positions 0-19 (exact up to 20
     # positions, a filter checked on the planes beyond), with mixed mismatch
@@ -214,7 +215,7 @@ def test_matrix_core_pattern_blocks(tmp_path, glen, n, mms, threads, chunk):
     rng = random.Random(glen * 1000 + n)
     guides = ["".join(rng.choice("ACGT") for _ in range(glen)) for _ in range(n)]
     mm = [mms[i % len(mms)] for i in range(n)]
-    filt = "N" * (glen + 1) + "RG"
+    filt = "N" * (glen + 1) + pam
     pats = [x + "NNN" for x in guides]
     plants = [(x + "AGG", 0, glen, min(m + 1, 8), 2) for x, m in zip(guides, mm)]
     sg = cofpy.SynthGenome(glen + n, [200_001, 70_007], line_width=61, n_frac=0.01, gap=(30, 900), plants=plants)
