@@ -587,12 +587,19 @@ def main():
     hbm_achieved = alg_bytes / launch_s
     # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     # this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a 256 MiB
-    # device copy reads on the same counters); only valid at the default size
-    traffic = 1.649e9 + 1.89e7 if abs(genome_nucl - 3.1e9) < 1e6 else None
-    # VALU issue of the scan from rocprofv3 --pmc SQ_INSTS_VALU on this
-    # workload: wave instructions / (kernel time x 256 CU x 4 SIMD x 2.4 GHz /
-    # 4 cycles per wave64 instruction), time-weighted over the 12 runs
-    valu_issue = 0.820 if abs(genome_nucl - 3.1e9) < 1e6 else None
+    # device copy reads on the same counters), mean of the 12 runs: the packed
+    # genome (1.55 GB) + the scan image's two bit planes (0.78 GB) + halos and
+    # pattern fragments; only valid at the default size
+    default_size = abs(genome_nucl - 3.1e9) < 1e6
+    traffic = 2.578e9 + 1.40e7 if default_size else None
+    # issue rates of the scan from rocprofv3 --pmc on this workload, time-weighted
+    # over the 12 runs: VALU = SQ_INSTS_VALU / (kernel time x 256 CU x 4 SIMD x
+    # 2.4 GHz / 4 cycles per wave64 instruction); matrix cores = i8 ops
+    # (SQ_INSTS_VALU_MFMA_MOPS_I8 x 512) / kernel time against 5.0 POP/s, and
+    # SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs)
+    valu_issue = 0.693 if default_size else None
+    mfma_tops = 1478 if default_size else None
+    mfma_busy = 0.318 if default_size else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
@@ -617,7 +624,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 (bit4 AND+popcount)",
+        "dtype": "i8 (matrix-core window scores) + u32 (bit4/plane ops)",
         "data": "synthetic",
         "config": {"workload": f"plain GPU mismatch scan: {genome_nucl / 1e9:.2f} Gbp synthetic genome (24 chroms, "
                                "41% GC, 5% N), 100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both "
@@ -628,15 +635,18 @@ def main():
                      "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": "plain_scan_kernel", "launch_ms": round(launch_s * 1e3, 3),
                      "bytes_per_launch": round(alg_bytes),
-                     "note": "the scan is integer-ALU bound, not HBM bound: see roofline_int32"},
+                     "note": "the scan is bound by matrix-core and VALU issue, not by HBM: see counters"},
         "roofline_int32": {"bound": "int32-valu", "achieved": round(achieved / 1e12, 3),
                            "peak": round(INT32_PEAK / 1e12, 1), "unit": "TOP/s",
                            "frac": round(achieved / INT32_PEAK, 3),
                            "note": "achieved = brute-force int32 ops 2*(3*ceil(L/8)+1) per guide-bp / plain-scan "
                                    "kernel time; the kernel tests the PAM first and so executes fewer ops"},
-        "counters": {"valu_issue_frac": valu_issue,
-                     "note": "measured VALU instruction issue of plain_scan_kernel against the gfx950 issue peak "
-                             "(rocprofv3 SQ_INSTS_VALU, default workload only)"},
+        "counters": {"valu_issue_frac": valu_issue, "mfma_i8_tops": mfma_tops,
+                     "mfma_i8_frac": round(mfma_tops / 5000, 3) if mfma_tops else None,
+                     "mfma_busy_frac": mfma_busy,
+                     "note": "measured issue of plain_scan_kernel (rocprofv3 --pmc, default workload only): VALU "
+                             "instructions against the gfx950 issue peak; i8 matrix-core ops against the dense "
+                             "int8 peak of 5.0 POP/s and matrix-core busy cycles"},
         "cpu_baseline": baseline,
         "kernel_guide_gbp_s": round(units / (sum(kms) / 1e3) / 1e9, 1),
         "per_run": per_run,
