@@ -483,6 +483,10 @@ def main():
     ap.add_argument("--cpu-sample-bp", type=float, default=8e8)
     ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--queue-depth", type=int, default=1,
+                    help="plain workload: searches queued ahead of the one being finished (depths 3 and 6 "
+                         "measured slower than 1: each context has its own stream, more streams than the "
+                         "process's hardware queues)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="plain workload: finish each run before starting the next")
     ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv", "cli"],
@@ -540,9 +544,12 @@ def main():
     launches = [0] * len(runs)
     hits = [0] * len(runs)
 
+    depth = max(1, args.queue_depth)
+
     def one_step(record):
-        # run i+1's kernels are queued before run i's hits are copied back and
-        # converted (cof_search_async / cof_search_wait), unless --no-overlap
+        # runs i+1 .. i+depth are queued before run i's hits are copied back
+        # and converted (cof_search_async / cof_search_wait; the scans still run
+        # one after another on the GPU), unless --no-overlap
         def finish(i):
             hits[i] = ctxs[i].wait() if not args.no_overlap else ctxs[i].search()
             if record:
@@ -554,12 +561,13 @@ def main():
         for i, s in enumerate(ctxs):
             if not args.no_overlap:
                 s.search_async()
-                if i:
-                    finish(i - 1)
+                if i >= depth:
+                    finish(i - depth)
             else:
                 finish(i)
         if not args.no_overlap:
-            finish(len(ctxs) - 1)
+            for j in range(max(0, len(ctxs) - depth), len(ctxs)):
+                finish(j)
 
     for w in range(args.warmup):
         one_step(False)
@@ -630,7 +638,7 @@ def main():
                                "41% GC, 5% N), 100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both "
                                "strands, 1000 planted near-hits per guide; 12 reference-format runs per step",
                    "genome_bp": genome_nucl, "guides_per_run": 100, "runs": len(runs),
-                   "runs_overlapped": not args.no_overlap},
+                   "runs_overlapped": not args.no_overlap, "queue_depth": None if args.no_overlap else depth},
         "roofline": {"bound": "hbm", "achieved": round(hbm_achieved / 1e9, 1), "peak": round(HBM_PEAK / 1e9),
                      "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": "plain_scan_kernel", "launch_ms": round(launch_s * 1e3, 3),
