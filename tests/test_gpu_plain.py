@@ -237,3 +237,38 @@ def test_matrix_core_pattern_blocks(tmp_path, glen, n, mms, threads, chunk, pam)
     s.close()
     assert len(want) > 0
     assert cofpy.read_rows(tmp_path / "gpu.tsv") == want
+
+
+def test_scan_image_follows_genome_reloads(tmp_path):
+    # The scan image (bit planes and unit flags of the resident genome) is
+    # built by the first plain search after a genome load: one context loads
+    # three genomes in turn (larger, smaller, larger again), and each search
+    # must equal a fresh context's and the oracle's rows.
+    import random
+    rng = random.Random(5)
+    guides = ["".join(rng.choice("ACGT") for _ in range(20)) for _ in range(24)]
+    pats = [g + "NNN" for g in guides]
+    filt = "N" * 21 + "RG"
+    n = len(pats)
+    specs = [(31, [300_001, 120_000]), (32, [70_001]), (33, [410_000, 9_999, 150_001])]
+    s = cofpy.Search(pats, [filt] * n, [4] * n, batch_nucl=1 << 22)
+    for seed, lens in specs:
+        sg = cofpy.SynthGenome(seed, lens, line_width=61, n_frac=0.02, gap=(30, 900),
+                               plants=[(g + "AGG", 0, 20, 3, 3) for g in guides])
+        fa = tmp_path / f"g{seed}.fa"
+        sg.write_fasta(fa)
+        s.load_file(fa)
+        s.search()
+        s.write_tsv(tmp_path / f"reuse{seed}.tsv")
+        fresh = cofpy.Search(pats, [filt] * n, [4] * n, batch_nucl=1 << 22)
+        fresh.load_file(fa)
+        fresh.search()
+        fresh.write_tsv(tmp_path / f"fresh{seed}.tsv")
+        fresh.close()
+        oracle.run(f"{fa}\n{filt}\n" + "".join(f"{p} 4\n" for p in pats), tmp_path / f"ref{seed}.tsv",
+                   batch_nucl=1 << 22)
+        want = cofpy.read_rows(tmp_path / f"ref{seed}.tsv")
+        assert len(want) > 0
+        assert cofpy.read_rows(tmp_path / f"reuse{seed}.tsv") == want
+        assert cofpy.read_rows(tmp_path / f"fresh{seed}.tsv") == want
+    s.close()
