@@ -483,10 +483,10 @@ def main():
     ap.add_argument("--cpu-sample-bp", type=float, default=8e8)
     ap.add_argument("--bulge-cpu-sample-bp", type=float, default=5e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--queue-depth", type=int, default=1,
-                    help="plain workload: searches queued ahead of the one being finished (depths 3 and 6 "
-                         "measured slower than 1: each context has its own stream, more streams than the "
-                         "process's hardware queues)")
+    ap.add_argument("--queue-depth", type=int, default=2,
+                    help="plain workload: searches queued ahead of the one being finished (measured: depth 2 "
+                         "fastest; 1 leaves the GPU idle behind a hit-heavy run's conversion, 3 and 6 put "
+                         "more streams in flight than the process's hardware queues)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="plain workload: finish each run before starting the next")
     ap.add_argument("--workload", choices=["plain", "bulge", "batched", "costmodel", "ingest", "tsv", "cli"],
