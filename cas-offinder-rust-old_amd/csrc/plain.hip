@@ -251,21 +251,28 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
         }
         if (!__ballot((int32_t)red >= 0)) continue;
         // window row 4 lg + v against pattern column r of block bc + b (scores
-        // computed again, so they need no registers in the common case): the
-        // plane compare decides, then the hit is recorded
+        // computed again, so they need no registers in the common case; blocks
+        // without a passing score are skipped as a whole): on windows wider
+        // than 20 positions the plane compare decides, then the hit is recorded
+        const bool exact = (G.wmask >> 20) == 0u;   // uniform
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             const mf_v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
+            if (!__ballot(max(max(d[0], d[1]), max(d[2], d[3])) >= 0)) continue;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const uint32_t c = cb + 4u * lg + (uint32_t)v;
                 if (d[v] < 0 || c >= nc) continue;
                 const PlaneRec& rec = prec[G.prec_begin + (bc + b) * 16u + r];
                 const uint32_t i = cand[c];
-                const uint32_t q = i + G.pk_lo;
-                const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
-                const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
-                if ((uint32_t)__popc((gh ^ rec.ph) | (gl ^ rec.pl)) + rec.K > 64u) continue;
+                if (!exact) {
+                    const uint32_t q = i + G.pk_lo;
+                    const uint32_t gh =
+                        __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
+                    const uint32_t gl =
+                        __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                    if ((uint32_t)__popc((gh ^ rec.ph) | (gl ^ rec.pl)) + rec.K > 64u) continue;
+                }
                 const uint32_t amm = recount_swar(gw, i, full_words + (size_t)rec.plan * 8, G.L);
                 const uint32_t idx = atomicAdd(out_count, 1u);
                 if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[rec.plan].pid, amm};
