@@ -324,7 +324,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
     constexpr int kThreads = THREADS;
     // one LDS block, the bit4 words at offset 0 (LDS addresses need no base)
-    struct TileLds {
+    struct alignas(16) TileLds {
         uint32_t gw[kTileWords];
         uint32_t ghi[kPlaneWords], glo[kPlaneWords];
         uint16_t cand[kTile / 2];   // one half tile's windows
@@ -385,12 +385,16 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const uint32_t f = raw[b] & (raw[b] >> 1) & (raw[b] >> 2) & (raw[b] >> 3) & 0x11111111u;   // nibble == 0xF
-                raw[b] &= ~(f * 0xFu);
+                raw[b] &= ~((f << 4) - f);
             }
         }
+        if (4 * j + 3 < (uint32_t)kTileWords) {   // a whole 16-byte group: one LDS store
+            *reinterpret_cast<uint4*>(&gw[4 * j]) = make_uint4(raw[0], raw[1], raw[2], raw[3]);
+        } else {
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            if (4 * j + b < (uint32_t)kTileWords) gw[4 * j + b] = raw[b];
+            for (int b = 0; b < 4; b++)
+                if (4 * j + b < (uint32_t)kTileWords) gw[4 * j + b] = raw[b];
+        }
     }
     if (flags == 0) {
 #pragma unroll
