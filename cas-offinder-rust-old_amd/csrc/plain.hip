@@ -12,23 +12,25 @@
 // Design (one workgroup = one tile of 64 x THREADS consecutive start
 // positions, optionally one slice of a group's patterns per grid row):
 //   1. the tile's genome words (+ halo) are loaded with 16-byte loads, all
-//      issued before any is used, and stored into LDS with the N-mask
-//      (0xF -> 0, search.rs:512-531) applied on the fly, together with two
-//      32-position bit planes (base bit 1, base bit 0); SWAR tests flag tiles
-//      holding IUPAC genome letters (R, Y, ...) or zero nibbles;
+//      issued before any is used, and stored into LDS, N-masked (0xF -> 0,
+//      search.rs:512-531) on tiles that hold zero nibbles.  The genome's scan
+//      image (built once per loaded genome) gives the tile's flags (IUPAC
+//      genome letters, zero nibbles) and, for tiles of plain bases only, its
+//      two 32-position bit planes (base bit 1, base bit 0);
 //   2. per pattern group (patterns sharing one effective filter), every thread
 //      tests the PAM of its 64 starts and the passing starts are compacted
-//      into an LDS list by a block-wide scan.  Tiles of plain bases only (no
-//      masked N, padding or IUPAC letter) test a PAM cell on 32 starts with
-//      one v_bitop3 over the planes; other tiles test 8 starts per word (SWAR
-//      over nibbles);
-//   3. each lane takes one compacted start, extracts its compare window with
-//      v_alignbit funnel shifts, and loops over the group's patterns (uniform
-//      scalar loads).  Groups whose patterns are plain bases on every position
-//      that is not N in all of them compare the planes on plain-base tiles: a
-//      window of up to 32 positions against one pattern is xor + bitop3 +
-//      v_bcnt, 16 patterns per batch of scalar loads.  Other tiles and groups
-//      AND the bit4 words and v_bcnt them (W words per pattern).
+//      into an LDS list of half a tile by a block-wide scan (two passes over
+//      the tile's halves when they do not fit).  Plain-base tiles test a PAM
+//      cell on 32 starts with one v_bitop3 over the planes; other tiles test 8
+//      starts per word (SWAR over nibbles);
+//   3. groups whose patterns are plain bases on every position that is not N
+//      in all of them score plain-base tiles' windows on the matrix cores (16
+//      windows x 16 patterns per v_mfma_i32_16x16x64_i8, see mfma_chunk), or,
+//      for windows wider than 20 positions where that filter would pass too
+//      many pairs, compare the planes (xor + bitop3 + v_bcnt per pattern, 16
+//      patterns per batch of scalar loads).  Other tiles and groups extract
+//      each window with v_alignbit funnel shifts, AND the bit4 words and
+//      v_bcnt them (W words per pattern);
 //   4. hits (with the writer's mismatch recount) go to one of 64 record
 //      segments, each with its own counter.
 // Without IUPAC letters in the window, genome single bases make the writer's
