@@ -140,7 +140,9 @@ typedef struct {
     uint32_t plain_threads;        /* plain-scan workgroup: 64 (default), 128, 256 or 512 threads,
                                       64 start positions per thread */
     uint32_t plain_pat_chunk;      /* patterns per plain-scan workgroup; a larger group is split
-                                      over grid rows (0: whole group) */
+                                      over grid rows (0: whole group).  The matrix-core compare
+                                      serves 0 and multiples of 16; other values use the plane
+                                      compare (same results) */
     uint32_t bulge_first_tiles;    /* filter tiles (4096 end positions each) in the synchronous
                                       first round of the bulge path (default 1024) */
     uint32_t bulge_round_tiles;    /* tiles per later round (0: sized from the first round's
