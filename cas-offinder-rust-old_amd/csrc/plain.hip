@@ -113,40 +113,18 @@ __device__ __forceinline__ uint32_t multi_nibbles(uint32_t w) {
     uint32_t o = (w | (w >> 1)) & 0x55555555u;
     return ((x | (x >> 2) | (o & (o >> 2))) & 0x11111111u) << 3;
 }
-// acc & (the base with planes (h, l) is in the filter set `code`), 32
-// positions per operation: one v_bitop3 whose truth table is code << 4
-// (bit 4 + 2h + l set iff base (h, l) = T C A G -> code bit 0 1 2 3 is in
-// the set); `code` is uniform, so the switch is a scalar branch.
-template <uint32_t C>
-__device__ __forceinline__ uint32_t pam_cell_c(uint32_t acc, uint32_t h, uint32_t l) {
-    return __builtin_amdgcn_bitop3_b32(acc, h, l, (C << 4) & 0xF0u);
-}
-// the same for two words at once (one scalar dispatch per PAM cell)
-template <uint32_t C>
-__device__ __forceinline__ void pam_cell2_c(uint32_t& a0, uint32_t& a1, uint32_t h0, uint32_t l0, uint32_t h1,
-                                            uint32_t l1) {
-    a0 = pam_cell_c<C>(a0, h0, l0);
-    a1 = pam_cell_c<C>(a1, h1, l1);
-}
+// a0, a1 &= (the base with planes (h, l) is in the filter set `code`), 32
+// positions per word (base (h, l) = T C A G -> code bit 0 1 2 3).
 __device__ __forceinline__ void pam_cell2(uint32_t& a0, uint32_t& a1, uint32_t h0, uint32_t l0, uint32_t h1,
                                           uint32_t l1, uint32_t code) {
-    switch (code) {
-        case 1: pam_cell2_c<1>(a0, a1, h0, l0, h1, l1); break;
-        case 2: pam_cell2_c<2>(a0, a1, h0, l0, h1, l1); break;
-        case 3: pam_cell2_c<3>(a0, a1, h0, l0, h1, l1); break;
-        case 4: pam_cell2_c<4>(a0, a1, h0, l0, h1, l1); break;
-        case 5: pam_cell2_c<5>(a0, a1, h0, l0, h1, l1); break;
-        case 6: pam_cell2_c<6>(a0, a1, h0, l0, h1, l1); break;
-        case 7: pam_cell2_c<7>(a0, a1, h0, l0, h1, l1); break;
-        case 8: pam_cell2_c<8>(a0, a1, h0, l0, h1, l1); break;
-        case 9: pam_cell2_c<9>(a0, a1, h0, l0, h1, l1); break;
-        case 10: pam_cell2_c<10>(a0, a1, h0, l0, h1, l1); break;
-        case 11: pam_cell2_c<11>(a0, a1, h0, l0, h1, l1); break;
-        case 12: pam_cell2_c<12>(a0, a1, h0, l0, h1, l1); break;
-        case 13: pam_cell2_c<13>(a0, a1, h0, l0, h1, l1); break;
-        case 14: pam_cell2_c<14>(a0, a1, h0, l0, h1, l1); break;
-        default: break;           // 15: any base
-    }
+    // branch-free: the set's bits as uniform masks, selected by l then by h
+    // (bitop3 table 0xCA: first operand ? second : third)
+    const uint32_t cT = (code & 1u) ? ~0u : 0u, cC = (code & 2u) ? ~0u : 0u;
+    const uint32_t cA = (code & 4u) ? ~0u : 0u, cG = (code & 8u) ? ~0u : 0u;
+    a0 &= __builtin_amdgcn_bitop3_b32(h0, __builtin_amdgcn_bitop3_b32(l0, cG, cA, 0xCA),
+                                      __builtin_amdgcn_bitop3_b32(l0, cC, cT, 0xCA), 0xCA);
+    a1 &= __builtin_amdgcn_bitop3_b32(h1, __builtin_amdgcn_bitop3_b32(l1, cG, cA, 0xCA),
+                                      __builtin_amdgcn_bitop3_b32(l1, cC, cT, 0xCA), 0xCA);
 }
 
 // Workgroup barrier for LDS exchange.  A one-wave workgroup only needs the
