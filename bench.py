@@ -605,9 +605,9 @@ def main():
     # 2.4 GHz / 4 cycles per wave64 instruction); matrix cores = i8 ops
     # (SQ_INSTS_VALU_MFMA_MOPS_I8 x 512) / kernel time against 5.0 POP/s, and
     # SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs)
-    valu_issue = 0.706 if default_size else None
-    mfma_tops = 1558 if default_size else None
-    mfma_busy = 0.328 if default_size else None
+    valu_issue = 0.733 if default_size else None
+    mfma_tops = 1596 if default_size else None
+    mfma_busy = 0.336 if default_size else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}

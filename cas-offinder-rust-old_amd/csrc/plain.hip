@@ -21,8 +21,8 @@
 //      tests the PAM of its 64 starts and the passing starts are compacted
 //      into an LDS list of half a tile by a block-wide scan (two passes over
 //      the tile's halves when they do not fit).  Plain-base tiles test a PAM
-//      cell on 32 starts with one v_bitop3 over the planes; other tiles test 8
-//      starts per word (SWAR over nibbles);
+//      cell on 32 starts with three v_bitop3 selects over the planes and an
+//      AND; other tiles test 8 starts per word (SWAR over nibbles);
 //   3. groups whose patterns are plain bases on every position that is not N
 //      in all of them score plain-base tiles' windows on the matrix cores (16
 //      windows x 16 patterns per v_mfma_i32_16x16x64_i8, see mfma_chunk), or,
