@@ -24,7 +24,8 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_mlp_create", "cof_mlp_destroy", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward",
                "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
                "cof_load_genome_file_ex", "cof_device_count", "cof_device_name", "cof_write_tsv_ex",
-               "cof_search_async", "cof_search_wait"]
+               "cof_search_async", "cof_search_wait", "cof_hit_seqs", "cof_chrom_name", "cof_stream_begin",
+               "cof_stream_chunk", "cof_stream_end", "cof_search_batch"]
 COF_WRITE_HOST = 1
 COF_LOAD_HOST = 1
 COF_LOAD_DEVICE = 2
@@ -36,7 +37,7 @@ COF_MLP_EAGER = 1
 class CofTuning(C.Structure):
     _fields_ = [("plain_threads", C.c_uint32), ("plain_pat_chunk", C.c_uint32), ("bulge_first_tiles", C.c_uint32),
                 ("bulge_round_tiles", C.c_uint32), ("verify_blocks_per_cu", C.c_uint32),
-                ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32)]
+                ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32), ("plain_hit_cap", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -126,10 +127,20 @@ def lib():
         L.cof_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32, fp]
         L.cof_chunk_features.argtypes = [C.c_void_p, fp, C.c_uint64, C.POINTER(C.c_uint64)]
         L.cof_set_cost_model.argtypes = [C.c_void_p, C.c_void_p]
+        L.cof_hit_seqs.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_char_p, C.c_char_p, C.c_size_t,
+                                   C.POINTER(C.c_uint32)]
+        L.cof_chrom_name.argtypes = [C.c_void_p, C.c_uint32]
+        L.cof_chrom_name.restype = C.c_char_p
+        L.cof_stream_begin.argtypes = [C.c_void_p]
+        L.cof_stream_chunk.argtypes = [C.c_void_p, C.c_char_p, C.c_uint64, C.c_uint64, C.c_char_p]
+        L.cof_stream_end.argtypes = [C.c_void_p]
+        L.cof_search_batch.argtypes = [C.c_void_p, C.c_char_p, C.c_uint64, C.POINTER(CofChunk), C.c_uint64,
+                                       C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
         for f in ("cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_genome", "cof_search", "cof_write_tsv",
                   "cof_get_stats", "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                   "cof_mlp_create", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward", "cof_mlp_loss_grad",
-                  "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model"):
+                  "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model", "cof_hit_seqs", "cof_stream_begin",
+                  "cof_stream_chunk", "cof_stream_end", "cof_search_batch"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib
@@ -296,6 +307,45 @@ class Search:
         buf = (CofHit * max(n, 1))()
         self._check(lib().cof_search(self.h, buf, n, None), "search")
         return buf[:n]
+
+    def hit_seqs(self, first=0, n=None, stride=128):
+        """[(dna_seq, rna_seq)] of hits [first, first + n) of the last search (bytes; plain DNA may hold NUL)."""
+        if n is None:
+            n = self.search_count_cached() - first
+        dna = C.create_string_buffer(max(1, n * stride))
+        rna = C.create_string_buffer(max(1, n * stride))
+        lens = (C.c_uint32 * max(1, n))()
+        self._check(lib().cof_hit_seqs(self.h, first, n, dna, rna, stride, lens), "hit_seqs")
+        return [(dna.raw[k * stride:k * stride + lens[k]], rna.raw[k * stride:k * stride + lens[k]])
+                for k in range(n)]
+
+    def search_count_cached(self):
+        n = C.c_uint64()
+        st = lib().cof_search(self.h, (CofHit * 1)(), 0, C.byref(n))
+        if st not in (COF_OK, COF_E_OVERFLOW):
+            self._check(st, "search")
+        return n.value
+
+    def chrom_name(self, chrom):
+        v = lib().cof_chrom_name(self.h, chrom)
+        return None if v is None else v.decode()
+
+    def stream(self, chunks):
+        """cof_stream_begin, one cof_stream_chunk per (name, start, end, data), cof_stream_end."""
+        L = lib()
+        self._check(L.cof_stream_begin(self.h), "stream_begin")
+        for name, start, end, data in chunks:
+            self._check(L.cof_stream_chunk(self.h, name.encode(), start, end, data), "stream_chunk")
+        self._check(L.cof_stream_end(self.h), "stream_end")
+
+    def search_batch(self, packed, chunks, names, has_head):
+        """cof_search_batch over one batch: packed bytes, [(chrom, start, end, byte_offset)], names."""
+        arr = (CofChunk * max(1, len(chunks)))(*[CofChunk(c, 0, s, e, o) for c, s, e, o in chunks])
+        nm = (C.c_char_p * max(1, len(names)))(*[x.encode() for x in names])
+        n = C.c_uint64()
+        self._check(lib().cof_search_batch(self.h, packed, len(packed), arr, len(chunks), nm, len(names),
+                                           1 if has_head else 0, C.byref(n)), "search_batch")
+        return n.value
 
     def write_tsv(self, path, host=False):
         """Reference TSV rows of the last search; formatted on the device unless host=True."""

@@ -35,6 +35,12 @@ struct cof_ctx {
     std::vector<float> chunk_cost;
     cof::BulgeTune bulge_tune;
     bool have_genome = false;
+    // the genome is one search batch (cof_search_batch) instead of the
+    // reference batcher's layout over the whole genome: -1 no, else has_head
+    int single_batch = -1;
+    // chunks of a genome being streamed in (cof_stream_*)
+    bool streaming = false;
+    cof::Genome stream_g;
     bool cached = false;
     bool pending = false;          // between cof_search_async and cof_search_wait
     std::chrono::steady_clock::time_point search_t0;
@@ -74,11 +80,17 @@ uint32_t* alloc_genome(cof_ctx* ctx, uint64_t n_bytes) {
     return ctx->d_genome;
 }
 
+std::vector<cof::Batch> layout_batches(const cof_ctx* ctx) {
+    if (ctx->single_batch < 0) return cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
+    if (ctx->g.chunks.empty()) return {};
+    return {cof::Batch{0, ctx->g.chunks.size(), ctx->single_batch > 0}};
+}
+
 void finish_genome(cof_ctx* ctx) {
     ctx->ws.img_units = 0;   // the scan image follows the new genome at the next plain scan
     ctx->ws.genome_nib = ctx->g.n_nibbles();
     ctx->g.index();
-    ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
+    ctx->batches = layout_batches(ctx);
     ctx->have_genome = true;
     ctx->cached = false;
     ctx->stats.genome_nucl = 0;
@@ -111,6 +123,8 @@ int first_byte(const char* path) {
 cof_status load_file(cof_ctx* ctx, const char* path, uint32_t flags) {
     if (!ctx || !path) return COF_E_INVALID;
     if (ctx->pending) return refuse_pending(ctx);
+    ctx->streaming = false;
+    ctx->single_batch = -1;
     try {
         HIPCHECK(hipSetDevice(ctx->device));
         auto t0 = std::chrono::steady_clock::now();
@@ -170,6 +184,7 @@ cof_tuning get_tuning(const cof_ctx* ctx) {
     t.verify_blocks_per_cu = ctx->bulge_tune.verify_per_cu;
     t.bulge_queue_cap = ctx->bulge_tune.queue_cap;
     t.verify_refill = ctx->bulge_tune.refill;
+    t.plain_hit_cap = ctx->plain_tune.hit_cap;
     return t;
 }
 
@@ -182,6 +197,8 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
         pt.threads = t.plain_threads;
     }
     pt.pat_chunk = t.plain_pat_chunk;
+    if (t.plain_hit_cap % 64) throw cof::Error(COF_E_INVALID, "plain_hit_cap must be a multiple of 64");
+    pt.hit_cap = t.plain_hit_cap;
     if (t.bulge_first_tiles) bt.first_tiles = t.bulge_first_tiles;
     bt.round_tiles = t.bulge_round_tiles;
     if (t.verify_blocks_per_cu) {
@@ -295,8 +312,7 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
         ctx->bulge = nb;
         bool batch_changed = nq.batch_nucl != ctx->q.batch_nucl;
         ctx->q = std::move(nq);
-        if (ctx->have_genome && batch_changed)
-            ctx->batches = cof::make_batches(ctx->g.chunks.size(), ctx->q.batch_nucl);
+        if (ctx->have_genome && batch_changed) ctx->batches = layout_batches(ctx);
         ctx->cached = false;
         ctx->hits.clear();
         ctx->aux.clear();
@@ -313,29 +329,99 @@ cof_status cof_load_genome_file_ex(cof_ctx* ctx, const char* path, uint32_t flag
     return load_file(ctx, path, flags);
 }
 
+namespace {
+
+// A caller-built chunk table and packed genome -> resident genome.
+void load_packed(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks, uint64_t n_chunks,
+                 const char* const* names, uint32_t n_chroms) {
+    HIPCHECK(hipSetDevice(ctx->device));
+    cof::Genome g;
+    for (uint32_t i = 0; i < n_chroms; i++) g.chrom_names.push_back(names[i] ? names[i] : "");
+    uint64_t expect = 0;
+    for (uint64_t i = 0; i < n_chunks; i++) {
+        const cof_chunk& c = chunks[i];
+        if (c.chrom >= n_chroms || c.end < c.start || c.end - c.start > cof::kChunkNucl || c.byte_offset != expect)
+            throw cof::Error(COF_E_INVALID, "chunk table is not a tight packing of <=64 Knt chunks");
+        if (i > 0 && !(chunks[i - 1].end == c.start || c.start == 0))
+            throw cof::Error(COF_E_INVALID, "chromosome chunks out of order");
+        g.chunks.push_back({c.chrom, c.start, c.end, c.byte_offset});
+        expect += (c.end - c.start + 1) / 2;
+    }
+    if (expect != n_bytes) throw cof::Error(COF_E_INVALID, "packed size does not match the chunk table");
+    g.n_bytes = n_bytes;
+    g.packed.assign(packed, packed + n_bytes);
+    g.packed.resize(n_bytes + 256, 0);
+    ctx->g = std::move(g);
+    upload_genome(ctx);
+}
+
+}  // namespace
+
 cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
                            uint64_t n_chunks, const char* const* names, uint32_t n_chroms) {
     if (!ctx || (!packed && n_bytes) || (!chunks && n_chunks) || (!names && n_chroms)) return COF_E_INVALID;
     if (ctx->pending) return refuse_pending(ctx);
+    ctx->streaming = false;
+    ctx->single_batch = -1;
     try {
-        HIPCHECK(hipSetDevice(ctx->device));
-        cof::Genome g;
-        for (uint32_t i = 0; i < n_chroms; i++) g.chrom_names.push_back(names[i] ? names[i] : "");
-        uint64_t expect = 0;
-        for (uint64_t i = 0; i < n_chunks; i++) {
-            const cof_chunk& c = chunks[i];
-            if (c.chrom >= n_chroms || c.end < c.start || c.end - c.start > cof::kChunkNucl || c.byte_offset != expect)
-                throw cof::Error(COF_E_INVALID, "chunk table is not a tight packing of <=64 Knt chunks");
-            if (i > 0 && !(chunks[i - 1].end == c.start || c.start == 0))
-                throw cof::Error(COF_E_INVALID, "chromosome chunks out of order");
-            g.chunks.push_back({c.chrom, c.start, c.end, c.byte_offset});
-            expect += (c.end - c.start + 1) / 2;
-        }
-        if (expect != n_bytes) throw cof::Error(COF_E_INVALID, "packed size does not match the chunk table");
-        g.n_bytes = n_bytes;
-        g.packed.assign(packed, packed + n_bytes);
-        g.packed.resize(n_bytes + 256, 0);
-        ctx->g = std::move(g);
+        load_packed(ctx, packed, n_bytes, chunks, n_chunks, names, n_chroms);
+    } catch (const cof::Error& e) {
+        ctx->have_genome = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_stream_begin(cof_ctx* ctx) {
+    if (!ctx) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
+    ctx->stream_g = cof::Genome{};
+    ctx->streaming = true;
+    ctx->have_genome = false;
+    ctx->cached = false;
+    return COF_OK;
+}
+
+cof_status cof_stream_chunk(cof_ctx* ctx, const char* chr_name, uint64_t chunk_start, uint64_t chunk_end,
+                            const uint8_t* data) {
+    if (!ctx || !chr_name || (!data && chunk_end > chunk_start)) return COF_E_INVALID;
+    if (!ctx->streaming) {
+        ctx->err = "cof_stream_chunk outside cof_stream_begin / cof_stream_end";
+        return COF_E_INVALID;
+    }
+    cof::Genome& g = ctx->stream_g;
+    if (chunk_end < chunk_start || chunk_end - chunk_start > cof::kChunkNucl) {
+        ctx->err = "a chunk holds at most 65536 nucleotides";
+        return COF_E_INVALID;
+    }
+    // chunks arrive in chromosome order (search.rs:1401-1404); a chunk that
+    // starts at 0 opens a chromosome
+    if (chunk_start == 0) {
+        g.chrom_names.push_back(chr_name);
+    } else if (g.chunks.empty() || g.chunks.back().end != chunk_start || g.chrom_names.back() != chr_name) {
+        ctx->err = "search expects chromosome chunks to arrive in order";
+        return COF_E_INVALID;
+    }
+    const uint64_t nb = (chunk_end - chunk_start + 1) / 2;
+    g.chunks.push_back({(uint32_t)(g.chrom_names.size() - 1), chunk_start, chunk_end, g.n_bytes});
+    g.packed.insert(g.packed.end(), data, data + nb);
+    if ((chunk_end - chunk_start) & 1) g.packed.back() &= 0x0F;   // the padding nibble of an odd chunk is 0
+    g.n_bytes += nb;
+    return COF_OK;
+}
+
+cof_status cof_stream_end(cof_ctx* ctx) {
+    if (!ctx) return COF_E_INVALID;
+    if (!ctx->streaming) {
+        ctx->err = "cof_stream_end without cof_stream_begin";
+        return COF_E_INVALID;
+    }
+    ctx->streaming = false;
+    ctx->single_batch = -1;
+    try {
+        ctx->stream_g.packed.resize(ctx->stream_g.n_bytes + 256, 0);
+        ctx->g = std::move(ctx->stream_g);
+        ctx->stream_g = cof::Genome{};
         upload_genome(ctx);
     } catch (const cof::Error& e) {
         ctx->have_genome = false;
@@ -415,16 +501,67 @@ cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits
             search_end(ctx);
         }
         if (n_hits) *n_hits = ctx->hits.size();
-        if (cap < ctx->hits.size()) {
-            if (out && cap) memcpy(out, ctx->hits.data(), cap * sizeof(cof_hit));
-            return COF_E_OVERFLOW;
-        }
+        if (cap < ctx->hits.size()) return COF_E_OVERFLOW;   // nothing written
         if (out && !ctx->hits.empty()) memcpy(out, ctx->hits.data(), ctx->hits.size() * sizeof(cof_hit));
     } catch (const cof::Error& e) {
         ctx->pending = false;
         return fail(ctx, e);
     }
     return COF_OK;
+}
+
+cof_status cof_search_batch(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
+                            uint64_t n_chunks, const char* const* names, uint32_t n_chroms, uint32_t has_head,
+                            uint64_t* n_hits) {
+    if (!ctx || (!packed && n_bytes) || (!chunks && n_chunks) || (!names && n_chroms) || has_head > 1)
+        return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
+    if (has_head && n_chunks < 2) {
+        ctx->err = "a batch with a head chunk holds at least two chunks";
+        return COF_E_INVALID;
+    }
+    ctx->streaming = false;
+    try {
+        ctx->single_batch = (int)has_head;
+        load_packed(ctx, packed, n_bytes, chunks, n_chunks, names, n_chroms);
+        search_begin(ctx);
+        search_end(ctx);
+        if (n_hits) *n_hits = ctx->hits.size();
+    } catch (const cof::Error& e) {
+        ctx->pending = false;
+        ctx->have_genome = false;
+        return fail(ctx, e);
+    }
+    return COF_OK;
+}
+
+cof_status cof_hit_seqs(cof_ctx* ctx, uint64_t first, uint64_t n, char* dna, char* rna, size_t stride,
+                        uint32_t* lens) {
+    if (!ctx || (n && (!dna || !rna || !lens))) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
+    if (!ctx->cached || first > ctx->hits.size() || n > ctx->hits.size() - first) {
+        ctx->err = "hit range outside the hits of the last search";
+        return COF_E_INVALID;
+    }
+    // every string fits, or nothing is written
+    for (uint64_t k = first; k < first + n; k++) {
+        const cof_hit& h = ctx->hits[k];
+        const cof::Pattern& p = ctx->q.pats[h.pattern];
+        if ((p.bulge ? h.n_ops : p.L) > stride) return COF_E_OVERFLOW;
+    }
+    std::string d, r;
+    for (uint64_t k = 0; k < n; k++) {
+        cof::hit_seqs(ctx->g, ctx->q, ctx->hits[first + k], ctx->aux[first + k], d, r);
+        memcpy(dna + k * stride, d.data(), d.size());
+        memcpy(rna + k * stride, r.data(), r.size());
+        lens[k] = (uint32_t)d.size();
+    }
+    return COF_OK;
+}
+
+const char* cof_chrom_name(const cof_ctx* ctx, uint32_t chrom) {
+    if (!ctx || !ctx->have_genome || chrom >= ctx->g.chrom_names.size()) return nullptr;
+    return ctx->g.chrom_names[chrom].c_str();
 }
 
 cof_status cof_search_async(cof_ctx* ctx) {

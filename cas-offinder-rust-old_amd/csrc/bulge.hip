@@ -86,6 +86,15 @@ struct DChunk {
     uint32_t chrom_start;     // chunk starts a chromosome
 };
 
+// LDS exchange between the lanes of one wave: no hardware barrier is needed
+// (a wave's LDS operations complete in issue order), only ordering at
+// wavefront scope, which the fences give without emitting instructions.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint32_t lds_nib(const uint32_t* w, uint32_t q) { return (w[q >> 3] >> ((q & 7u) * 4u)) & 0xFu; }
 __device__ __forceinline__ uint32_t glob_nib(const uint32_t* __restrict__ g, uint64_t q) {
     return (g[q >> 3] >> ((uint32_t)(q & 7u) * 4u)) & 0xFu;
@@ -228,7 +237,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     // exceeds q_cap it stops growing (the host redoes such a round), so the
     // 32-bit count cannot wrap
     auto flush = [&](uint32_t pat_base) {
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
         uint32_t qb = q_cap;
         if (lane == 0) {
             if (__hip_atomic_load(q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= q_cap)
@@ -242,7 +251,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                 queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, pat_base + (v & 0xFFFFFu), T.last_chunk, tile_id, 0u};
         }
         n_st = 0;
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
     };
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {

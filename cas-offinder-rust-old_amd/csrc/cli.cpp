@@ -1,7 +1,10 @@
 // cof-cli: drop-in for cas-offinder-cli (cas-offinder-cli/src/main.rs:55-305)
 // on one MI355X.  Usage: cof-cli {input|-} {C|G|A} {output|-}
-// The device letter is accepted for compatibility; the search always runs on
-// HIP device 0 (COF_DEVICE overrides).  Input parsing follows
+// The device letter is one character (cli_utils.rs:4-16) and picks the
+// reference path whose output the run reproduces: C the CPU path's 4 Mnt
+// batches, G and A the GPU path's 1 Gnt batches (search.rs:21-50); the search
+// itself always runs on HIP device 0 (COF_DEVICE overrides, COF_BATCH_NUCL
+// overrides the batch size).  Input parsing follows
 // cas-offinder-cli/src/cli_utils.rs:37-148.  A run that writes to a file also
 // writes the run summary <output>.log (log_writer.rs:27-53, main.rs:277-304).
 #include <sys/stat.h>
@@ -99,7 +102,7 @@ int main(int argc, char** argv) {
     }
     auto t0 = std::chrono::steady_clock::now();
     std::string dev = argv[2];
-    if (dev.empty() || (dev[0] != 'C' && dev[0] != 'G' && dev[0] != 'A'))
+    if (dev.size() != 1 || (dev[0] != 'C' && dev[0] != 'G' && dev[0] != 'A'))
         return die("2nd argument must be one of {C|G|A}");
     std::string text;
     {
@@ -159,6 +162,7 @@ int main(int argc, char** argv) {
     q.max_mismatches = mm.data();
     q.max_dna_bulges = dd.data();
     q.max_rna_bulges = rr.data();
+    q.batch_nucl = dev[0] == 'C' ? (1ull << 22) : (1ull << 30);   // search.rs:21-22, 44-50
     if (const char* bn = getenv("COF_BATCH_NUCL")) q.batch_nucl = strtoull(bn, nullptr, 10);
     int device = 0;
     if (const char* d = getenv("COF_DEVICE")) device = atoi(d);

@@ -200,5 +200,9 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& ba
 void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches,
                    RawView<BulgeRaw> raw, HitVec& out, AuxVec& aux);
 void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux, FILE* f);
+// Match.dna_seq / Match.rna_seq of one final hit, before the writer's case
+// marking (plain: N-masked window, NUL for genome N; bulge: aligned strings).
+void hit_seqs(const Genome& g, const Query& q, const cof_hit& h, const HitAux& aux, std::string& dna,
+              std::string& rna);
 
 }  // namespace cof

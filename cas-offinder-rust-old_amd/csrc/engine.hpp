@@ -14,6 +14,7 @@ namespace cof {
 struct Workspace {
     void* plain_out = nullptr;     // PlainRaw records, 64 segments of plain_cap / 64
     uint32_t plain_cap = 0;
+    uint32_t plain_cap_tuned = 0;  // the PlainTune::hit_cap the buffer was last sized to
     uint32_t* plain_count = nullptr;   // per-segment counters, 128 bytes apart
     void* plain_out2 = nullptr;    // the segments packed together
     uint32_t plain_cap2 = 0;
@@ -85,6 +86,7 @@ void ws_grow(void** ptr, size_t* cap, size_t need, size_t elem);
 struct PlainTune {
     uint32_t threads = 64;      // workgroup size; a workgroup scans 64 starts per thread
     uint32_t pat_chunk = 0;     // patterns per workgroup (grid.y slices a group); 0 = whole group
+    uint32_t hit_cap = 0;       // hit records the device buffer starts with (0: 2^20, grown on overflow)
 };
 
 // Schedule of the bulge path (results do not depend on it).

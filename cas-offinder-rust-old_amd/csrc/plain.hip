@@ -127,13 +127,21 @@ __device__ __forceinline__ void pam_cell2(uint32_t& a0, uint32_t& a1, uint32_t h
                                       __builtin_amdgcn_bitop3_b32(l1, cC, cT, 0xCA), 0xCA);
 }
 
-// Workgroup barrier for LDS exchange.  A one-wave workgroup only needs the
-// compiler not to move LDS accesses across it: a wave's LDS operations
-// complete in issue order.
+// Workgroup barrier for LDS exchange.  A one-wave workgroup needs no
+// hardware barrier (a wave's LDS operations complete in issue order), only
+// memory ordering at wavefront scope: the release fence keeps this wave's
+// LDS writes before the barrier and the acquire fence its later reads after
+// it, so lanes see each other's writes whatever the compiler can prove about
+// aliasing.  At wavefront scope the fences emit no instructions.
 template <int THREADS>
 __device__ __forceinline__ void tile_sync() {
-    if constexpr (THREADS == 64) __builtin_amdgcn_wave_barrier();
-    else __syncthreads();
+    if constexpr (THREADS == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
 }
 
 // bit 0 of nibble j -> bit j (other bits zero)
@@ -408,7 +416,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
         uint32_t m8[8];
         if (plain_tile) {
-            // planes: one v_bitop3 per PAM cell per 32 starts
+            // planes: per PAM cell and 32 starts, two bitop3 selects by the l
+            // plane, one by the h plane and an AND (pam_cell2)
             uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
             for (uint32_t k = 0; k < G.n_pam; k++) {
                 const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
@@ -1078,10 +1087,14 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
     ws.plain_pending = false;
     if (plain_plan_empty(pp) || scan_end == 0) return;
     ws.ensure_events();
-    if (!ws.plain_out) {
+    if (!ws.plain_out || (tune.hit_cap && tune.hit_cap != ws.plain_cap_tuned)) {
+        // first scan, or a tuned starting size: (re)size the hit buffer
+        if (ws.plain_out) (void)hipFree(ws.plain_out);
+        ws.plain_out = nullptr;
         size_t cap = 0;
-        ws_grow(&ws.plain_out, &cap, 1u << 20, sizeof(PlainRaw));
+        ws_grow(&ws.plain_out, &cap, tune.hit_cap ? tune.hit_cap : (1u << 20), sizeof(PlainRaw));
         ws.plain_cap = (uint32_t)cap;
+        ws.plain_cap_tuned = tune.hit_cap;
     }
     size_t hc = ws.h_counts_cap;
     ws.ensure_host((void**)&ws.h_counts, &hc, kHitSegs * kSegStride * sizeof(uint32_t));

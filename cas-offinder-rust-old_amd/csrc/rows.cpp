@@ -27,12 +27,15 @@ namespace cof {
 namespace {
 
 struct BatchIndex {
-    std::vector<int64_t> nonhead_batch;    // per chunk
+    std::vector<int64_t> nonhead_batch;    // per chunk: the batch holding it as a member (-1: none)
+    std::vector<int64_t> head_batch;       // per chunk: the batch holding it as its head chunk (-1: none)
     std::vector<uint64_t> end_nib;         // per batch: end of its last chunk's bytes (nibbles)
     std::vector<uint64_t> active_begin;    // per batch: first end position scanned (bulge path)
     BatchIndex(const Genome& g, const std::vector<Batch>& bs) {
         nonhead_batch.assign(g.chunks.size(), -1);
+        head_batch.assign(g.chunks.size(), -1);
         for (size_t b = 0; b < bs.size(); b++) {
+            if (bs[b].has_head) head_batch[bs[b].first] = (int64_t)b;
             for (size_t c = bs[b].first + (bs[b].has_head ? 1 : 0); c < bs[b].last; c++) nonhead_batch[c] = (int64_t)b;
             uint64_t e = bs[b].last < g.chunks.size() ? g.chunks[bs[b].last].byte_off : g.n_bytes;
             end_nib.push_back(e * 2);
@@ -194,14 +197,12 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
         const Chunk& ch = g.chunks[c];
         uint64_t chrom_pos = ch.start + (r.pos - ch.nib_begin());
         bool past_end = chrom_pos + p.L > ch.end;
-        int64_t b = bi.nonhead_batch[c];
-        if (b < 0) continue;
-        bool last_in_b = c + 1 == bs[(size_t)b].last;
+        const int64_t b = bi.nonhead_batch[c], hb = bi.head_batch[c];
+        if (b < 0 && hb < 0) continue;
         bool next_new_chrom = c + 1 < g.chunks.size() && g.chunks[c + 1].start == 0;
         uint32_t mm = 0;
         uint64_t view_end = 0;
         bool emit = false;
-        // reported by batch b, where c is not the head
         // The device evaluated the predicate on the resident genome, which
         // equals the batch view whenever the window ends inside it; only
         // windows running past the view end are evaluated again here.
@@ -210,14 +211,20 @@ void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs
             if (r.pos + p.L <= ve) { mm = r.flags; return true; }
             return plain_predicate(g, p, r.pos, ve, &mm);
         };
-        if (!((last_in_b || next_new_chrom) && past_end)) emit = check(bi.end_nib[(size_t)b]);
-        // reported by batch b+1, where c is the head (search.rs:540-546); the
-        // two cases exclude each other (the second needs past_end)
-        if (!emit && last_in_b && (size_t)b + 1 < bs.size()) {
+        // reported by batch b, where c is not the head
+        if (b >= 0) {
+            bool last_in_b = c + 1 == bs[(size_t)b].last;
+            if (!((last_in_b || next_new_chrom) && past_end)) emit = check(bi.end_nib[(size_t)b]);
+        }
+        // reported by batch hb, where c is the head (search.rs:540-546); the
+        // two cases exclude each other (c is the last chunk of batch b, so
+        // the first case needs the window inside c, the second a start within
+        // L - 1 of c's end)
+        if (!emit && hb >= 0) {
             uint64_t c1 = g.chunks[c + 1].byte_off * 2 - ch.nib_begin();
             uint64_t min_start = c1 >= p.L - 1 ? c1 - (p.L - 1) : 0;
             uint64_t local = r.pos - ch.nib_begin();
-            if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)b + 1]);
+            if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)hb]);
         }
         if (!emit) continue;
         if (out) {
@@ -275,6 +282,42 @@ void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs
     });
 }
 
+// Match.dna_seq / Match.rna_seq of a final hit (chrom_chunk.rs:19-29).
+// Plain path (convert_matches, search.rs:1456-1475): the window of the
+// N-masked batch view decoded (genome N and bytes past the view as NUL) and
+// the pattern decoded; bulge path (convert_matches_myers, search.rs:1352-1372
+// with traceback.rs:250-290): the aligned strings with '-' at bulges.
+// Reverse-strand hits reverse-complement both (bit4ops.rs:207-232).
+void hit_seqs(const Genome& g, const Query& q, const cof_hit& h, const HitAux& a, std::string& dna, std::string& rna) {
+    const Pattern& p = q.pats[h.pattern];
+    rna.clear();
+    dna.clear();
+    if (!p.bulge) {
+        for (uint32_t k = 0; k < p.L; k++) {
+            uint64_t i = a.nib + k;
+            uint8_t v = i < a.view_end ? nib(g.packed.data(), i) : 0;
+            if (v == 0xF) v = 0;
+            dna.push_back(kCodec.dec[v]);
+            rna.push_back(kCodec.dec[p.b4[k]]);
+        }
+    } else {
+        uint64_t t = a.nib;
+        uint32_t pi = 0;
+        for (uint32_t k = 0; k < h.n_ops; k++) {
+            uint32_t op = (uint32_t)(h.ops[k / 32] >> (2 * (k % 32))) & 3;
+            if (op <= 1) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
+            else if (op == 3) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back('-'); }
+            else { rna.push_back('-'); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
+        }
+    }
+    if (!p.forward) {
+        std::reverse(dna.begin(), dna.end());
+        for (auto& c : dna) c = comp_char(c);
+        std::reverse(rna.begin(), rna.end());
+        for (auto& c : rna) c = comp_char(c);
+    }
+}
+
 // Rows of main.rs:130-247.
 void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux, FILE* f) {
     fputs("#Bulge type\tcrRNA\tDNA\tChromosome\tPosition\tDirection\tMismatches\tBulge Size\n", f);
@@ -282,25 +325,10 @@ void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec
     for (size_t hi = 0; hi < hits.size(); hi++) {
         const cof_hit& h = hits[hi];
         const Pattern& p = q.pats[h.pattern];
-        rna.clear();
-        dna.clear();
+        hit_seqs(g, q, h, aux[hi], dna, rna);
         const char* type = "X";
         uint32_t bsize = 0;
         if (!p.bulge) {
-            uint64_t pos = aux[hi].nib, view_end = aux[hi].view_end;
-            for (uint32_t k = 0; k < p.L; k++) {
-                uint64_t i = pos + k;
-                uint8_t v = i < view_end ? nib(g.packed.data(), i) : 0;
-                if (v == 0xF) v = 0;
-                dna.push_back(kCodec.dec[v]);
-                rna.push_back(kCodec.dec[p.b4[k]]);
-            }
-            if (!p.forward) {
-                std::reverse(dna.begin(), dna.end());
-                for (auto& c : dna) c = comp_char(c);
-                std::reverse(rna.begin(), rna.end());
-                for (auto& c : rna) c = comp_char(c);
-            }
             for (size_t k = 0; k < dna.size(); k++) {         // main.rs:198-217
                 char& d = dna[k];
                 bool rn = is_n_char(rna[k]);
@@ -310,21 +338,6 @@ void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec
                 else if (!cmp_chars((uint8_t)d, (uint8_t)rna[k])) d = (char)(d | 0x20);
             }
         } else {
-            // aligned strings from the ops (search.rs:152-206 / traceback.rs:355-386)
-            uint64_t t = aux[hi].nib;
-            uint32_t pi = 0;
-            for (uint32_t k = 0; k < h.n_ops; k++) {
-                uint32_t op = (uint32_t)(h.ops[k / 32] >> (2 * (k % 32))) & 3;
-                if (op <= 1) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
-                else if (op == 3) { rna.push_back(kCodec.dec[p.b4[pi++]]); dna.push_back('-'); }
-                else { rna.push_back('-'); dna.push_back(kCodec.dec[nib(g.packed.data(), t++)]); }
-            }
-            if (!p.forward) {
-                std::reverse(dna.begin(), dna.end());
-                for (auto& c : dna) c = comp_char(c);
-                std::reverse(rna.begin(), rna.end());
-                for (auto& c : rna) c = comp_char(c);
-            }
             uint32_t tot = h.dna_bulge + h.rna_bulge;
             type = tot == 0 ? "X" : h.dna_bulge > 0 ? "DNA" : "RNA";
             bsize = tot;

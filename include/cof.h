@@ -153,6 +153,9 @@ typedef struct {
                                       in halves */
     uint32_t verify_refill;        /* a traceback wave takes new candidates once this many of its
                                       64 lanes are idle (1-64, default 8) */
+    uint32_t plain_hit_cap;        /* plain-scan hit records the device buffer starts with (a
+                                      multiple of 64; 0: 2^20).  A scan whose hits do not fit grows
+                                      the buffer and runs again */
 } cof_tuning;
 
 /* Per filter tile of the last bulge search (tiles in genome order). */
@@ -201,11 +204,54 @@ cof_status cof_load_genome(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes
  * *n_hits.  With out == NULL the search always runs (a count query); with a
  * buffer, the hits of the last search are returned when they are cached,
  * otherwise the search runs first.  Returns COF_E_OVERFLOW when cap < total
- * (the first cap hits are written; the cache is kept, so a retry with a larger
- * buffer does not search again).  Replaces the per-batch loop of
+ * and then writes nothing to `out` (the cache is kept, so a retry with a
+ * larger buffer does not search again).  Replaces the per-batch loop of
  * search_device_cuda{,_myers} (search.rs:249-320,1054-1191) and its overflow
  * bisection (search.rs:1140-1152). */
 cof_status cof_search(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
+
+/* The rest of the reference's Match record (cas-offinder-lib/src/chrom_chunk.rs:19-29)
+ * for hits [first, first + n) of the last search: Match.dna_seq and
+ * Match.rna_seq byte for byte as convert_matches (search.rs:1456-1475: the
+ * window of the N-masked batch, genome N and bytes past the batch as NUL,
+ * and the pattern, both decoded) and convert_matches_myers (search.rs:1352-1372:
+ * the aligned strings with '-' at bulges) build them, reverse-complemented
+ * for reverse-strand hits.  Hit k's strings go to dna + k * stride and
+ * rna + k * stride (not NUL-terminated: a plain DNA string may hold NUL),
+ * their common length to lens[k].  COF_E_OVERFLOW, with nothing written,
+ * when a string is longer than stride (plain: pattern length; bulge: n_ops). */
+cof_status cof_hit_seqs(cof_ctx* ctx, uint64_t first, uint64_t n, char* dna, char* rna, size_t stride,
+                        uint32_t* lens);
+/* Match.chr_name: name of chromosome `chrom` (cof_hit.chrom) of the resident
+ * genome, valid until the genome is replaced; NULL when out of range. */
+const char* cof_chrom_name(const cof_ctx* ctx, uint32_t chrom);
+
+/* Genome by chunks, for a caller that receives them one by one as
+ * search(recv: Receiver<ChromChunkInfo>, ...) does (search.rs:1490-1500,
+ * 1525-1572): cof_stream_begin, then one cof_stream_chunk per
+ * ChromChunkInfo (chrom_chunk.rs:4-11: chr_name, chunk_start, chunk_end,
+ * and data = ceil((end - start) / 2) bytes of bit4, even nucleotide in the
+ * low nibble), in chromosome order (a chunk starting at 0 opens a
+ * chromosome), then cof_stream_end makes the genome resident.  The result
+ * equals cof_load_genome of the same chunks; searches run after the end. */
+cof_status cof_stream_begin(cof_ctx* ctx);
+cof_status cof_stream_chunk(cof_ctx* ctx, const char* chr_name, uint64_t chunk_start, uint64_t chunk_end,
+                            const uint8_t* data);
+cof_status cof_stream_end(cof_ctx* ctx);
+
+/* One batch of the backend seam search_chunk_cuda{,_myers} (search.rs:322-329,
+ * 1193-1203): the batch is one SearchChunkInfo (chunks_to_searchchunk,
+ * search.rs:1376-1427) given as a chunk table over its tight-packed bytes,
+ * has_head = SearchChunkMeta.has_overlap_head (the first chunk only gives
+ * backward context: plain starts before chunk 1 - (L - 1) and bulge ends
+ * before chunk 1 are not reported, search.rs:540-546, 921-935).  The batch
+ * replaces the resident genome and is searched as the reference searches
+ * one batch, converted as convert_matches{,_myers} convert its result; the
+ * hits are then read with cof_search (cached), cof_hit_seqs and
+ * cof_chrom_name.  *n_hits = number of hits. */
+cof_status cof_search_batch(cof_ctx* ctx, const uint8_t* packed, uint64_t n_bytes, const cof_chunk* chunks,
+                            uint64_t n_chunks, const char* const* chrom_names, uint32_t n_chroms, uint32_t has_head,
+                            uint64_t* n_hits);
 
 /* cof_search (count query) in two calls: cof_search_async queues the device
  * work of a search and returns; cof_search_wait completes it (hits copied

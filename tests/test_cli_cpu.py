@@ -28,6 +28,7 @@ def test_too_few_arguments_and_bad_device(tmp_path):
     r = _run(["in.txt"])
     assert r.returncode == 1 and "Usage:" in r.stderr
     (tmp_path / "in.txt").write_text("g.fa\nNNNRG\nACNNN 1\n")
-    r = _run([str(tmp_path / "in.txt"), "X", str(tmp_path / "o.tsv")])
-    assert r.returncode == 1 and "{C|G|A}" in r.stderr
-    assert not (tmp_path / "o.tsv").exists()
+    for dev in ("X", "G0", "CG", ""):          # one letter of C, G, A (cli_utils.rs:4-16)
+        r = _run([str(tmp_path / "in.txt"), dev, str(tmp_path / "o.tsv")])
+        assert r.returncode == 1 and "{C|G|A}" in r.stderr
+        assert not (tmp_path / "o.tsv").exists()

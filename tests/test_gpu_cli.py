@@ -6,6 +6,7 @@ rows and the header must be identical.  Covers a plain query, a bulge query,
 stdin input and a malformed input (status and message, no crash).
 """
 import os
+import random
 import subprocess
 
 import pytest
@@ -72,3 +73,40 @@ def test_cli_rejects_bad_input(tmp_path):
     assert r.returncode != 0 and "error" in r.stderr.lower()
     r = _run_cli(["-", "X", str(tmp_path / "o.tsv")], stdin=f"{fa}\nNNNNNNNNNNNNNNNNNNNNNRG\n{guide}NNN 3\n")
     assert r.returncode != 0
+
+
+def test_device_letter_picks_reference_batches(tmp_path):
+    # C reproduces the CPU path's 4 Mnt batches, G the GPU path's 1 Gnt ones
+    # (search.rs:21-50).  11-nt lines make every full chunk 65527 nt (odd), so
+    # a 0 nibble ends each chunk; at a 4 Mnt batch seam the start whose window
+    # is the chunk's last 23 nt plus that 0 nibble is reported by neither
+    # batch (the first drops it at the batch end, the second starts at
+    # chunk 1 - (L - 1), search.rs:540-546, 1449-1454), while one 1 Gnt batch
+    # reports it.  A guide copy planted there shows the difference.
+    rng = random.Random(5)
+    guide = "".join(rng.choice("ACGT") for _ in range(24))
+    chunk = 65527
+    seq = rng.choices("ACGT", k=64 * chunk + 5000)
+    at = 63 * chunk + chunk - 23
+    seq[at:at + 23] = list(guide[:23])
+    seq = "".join(seq)
+    fa = tmp_path / "g.fa"
+    with open(fa, "w") as f:
+        f.write(">chr1\n")
+        for i in range(0, len(seq), 11):
+            f.write(seq[i:i + 11] + "\n")
+    text = f"{fa}\n{'N' * 24}\n{guide} 1\n"
+    (tmp_path / "in.txt").write_text(text)
+    rows = {}
+    for dev, batch in (("C", 1 << 22), ("G", 1 << 30)):
+        oracle.run(text, tmp_path / f"ref{dev}.tsv", batch_nucl=batch)
+        want = cofpy.read_rows(tmp_path / f"ref{dev}.tsv")
+        env = {k: v for k, v in os.environ.items() if k != "COF_BATCH_NUCL"}
+        r = subprocess.run([CLI, str(tmp_path / "in.txt"), dev, str(tmp_path / f"out{dev}.tsv")], capture_output=True,
+                           text=True, env=env, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = cofpy.read_rows(tmp_path / f"out{dev}.tsv")
+        assert got == want
+        rows[dev] = set(got)
+    assert rows["C"] < rows["G"]
+    assert any(r.split("\t")[4] == str(at) for r in rows["G"] - rows["C"])

@@ -1,13 +1,14 @@
-"""Device FASTA and 2bit ingest (cof_load_genome_file_ex) give the same genome as the host readers.
+"""Device FASTA and 2bit ingest (cof_load_genome_file_ex) give the oracle's genome.
 
-The host reader restates read_fasta.rs:9-71 (pinned by the reference's
-upstream1000 fixtures in test_oracle.py).  Here FASTA files with several line
-shapes are loaded both ways and searched with a dense all-N query whose rows
-print the genome text around every hit, plus the reference query and
-fixtures; irregular files fall back to the host reader (or are refused when
-the device path is required).  2bit files (the reference fixture and
-synthetic files with N-block edge cases, read_2bit.rs:33-118) are decoded on
-the device and compared with the host reader and the oracle.
+FASTA files with several line shapes are loaded both ways (device encoder and
+host reader) and searched with a dense all-N query whose rows print the
+genome text around every hit; both must give the oracle's rows for the same
+file (the oracle's reader restates read_fasta.rs:9-71 and is pinned by the
+reference's upstream1000 fixtures in test_oracle.py).  Also: the reference
+query and fixtures, irregular files (host fallback, or refusal when the
+device path is required), a folder of *.fa files (read_fasta_folder.rs:8-23),
+and 2bit files (the reference fixture and synthetic files with N-block edge
+cases, read_2bit.rs:33-118) decoded on the device.
 """
 import os
 import random
@@ -15,6 +16,7 @@ import random
 import pytest
 
 import cofpy
+import oracle
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 pytestmark = pytest.mark.gpu
@@ -36,14 +38,22 @@ def _dense_query(seed):
     return ["".join(rng.choice("ACGT") for _ in range(12)) for _ in range(3)], "N" * 12, 5
 
 
+def _oracle_rows(path, guides, filt, mm, tmp_path):
+    oracle.run(f"{path}\n{filt}\n" + "".join(f"{g} {mm}\n" for g in guides), tmp_path / "ref.tsv",
+               batch_nucl=1 << 22)
+    return cofpy.read_rows(tmp_path / "ref.tsv")
+
+
 def _compare(path, tmp_path, seed=1, expect_device=True):
     guides, filt, mm = _dense_query(seed)
+    want = _oracle_rows(path, guides, filt, mm, tmp_path)
     host, st_h = _rows(path, cofpy.COF_LOAD_HOST, guides, filt, mm, tmp_path, "h")
     dev, st_d = _rows(path, 0, guides, filt, mm, tmp_path, "d")
     assert st_h.load_device == 0
     assert st_d.load_device == (1 if expect_device else 0)
-    assert len(host) > 1000
-    assert dev == host
+    assert len(want) > 1000
+    assert dev == want
+    assert host == want
 
 
 @pytest.mark.parametrize("width", [60, 61, 80, 1000])
@@ -89,6 +99,27 @@ def test_irregular_lines_fall_back(tmp_path):
     s.close()
 
 
+def test_folder_of_fasta_files(tmp_path):
+    # read_fasta_folder.rs:8-23: every regular *.fa file of the folder, other
+    # files and sub-folders skipped; the folder goes through the host reader
+    d = tmp_path / "genome"
+    d.mkdir()
+    for k, (lens, width) in enumerate([([90_001, 3_000], 60), ([70_000], 61), ([65_536 * 2 + 11], 80)]):
+        sg = cofpy.SynthGenome(900 + k, lens, line_width=width, n_frac=0.02, gap=(20, 400), iupac_frac=0.001)
+        sg.write_fasta(d / f"part{k}.fa")
+    cofpy.SynthGenome(5, [50_000]).write_fasta(d / "skipped.fasta")
+    (d / "notes.txt").write_text("not a genome\n")
+    (d / "sub.fa").mkdir()
+    guides, filt, mm = _dense_query(13)
+    want = _oracle_rows(d, guides, filt, mm, tmp_path)
+    got, st = _rows(d, 0, guides, filt, mm, tmp_path, "f")
+    assert st.load_device == 0
+    assert len(want) > 1000 and got == want
+    with pytest.raises(cofpy.CofError) as e:
+        _rows(d, cofpy.COF_LOAD_DEVICE, guides, filt, mm, tmp_path, "x")
+    assert e.value.status == cofpy.COF_E_UNSUPPORTED
+
+
 def test_reference_fixture_both_ways(tmp_path):
     # integration_bulge.rs:159-175 query on upstream1000.fa: 117 rows either way
     fa = os.path.join(GOLDEN, "upstream1000.fa")
@@ -111,7 +142,6 @@ def test_2bit_device_decode_edge_cases(tmp_path):
     # pieces of 64 Knt with N blocks across, on and at the ends of piece
     # boundaries, unsorted and overlapping blocks (the reader's one-block-back
     # walk), mask blocks, an empty and a 3-nt chromosome
-    import oracle
     import twobit
     from test_twobit_cpu import P, _records
     rng = random.Random(17)

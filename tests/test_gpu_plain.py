@@ -272,3 +272,27 @@ def test_scan_image_follows_genome_reloads(tmp_path):
         assert cofpy.read_rows(tmp_path / f"reuse{seed}.tsv") == want
         assert cofpy.read_rows(tmp_path / f"fresh{seed}.tsv") == want
     s.close()
+
+
+@pytest.mark.parametrize("pam,three_prime", [("TTTV", False), ("NGCG", True), ("NBHY", True), ("SMN", True)])
+def test_pam_sets_with_c_on_plain_tiles(tmp_path, pam, three_prime):
+    # filter codes holding C (C, Y, S, M, B, H, V) on a genome with no IUPAC
+    # letters, so nearly every tile takes the bit-plane PAM test and each
+    # of its base selects (T, C, A, G) is exercised
+    import random
+    rng = random.Random(len(pam) * 13 + three_prime)
+    guides = _guides(rng, 4, 20)
+    if three_prime:
+        filt = "N" * 20 + pam
+        pats = [g + "N" * len(pam) for g in guides]
+        plants = [(g + pam, 0, 20, 3, 30) for g in guides]
+    else:
+        filt = pam + "N" * 20
+        pats = ["N" * len(pam) + g for g in guides]
+        plants = [(pam + g, len(pam), 20, 3, 30) for g in guides]
+    sg = cofpy.SynthGenome(2000 + len(pam), [300_000, 170_001], n_frac=0.005, gap=(50, 2000), plants=plants)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    got, want, _ = run_both(tmp_path, fa, filt, pats, 4)
+    assert len(want) >= 30
+    assert got == want
