@@ -25,7 +25,7 @@ COF_SYMBOLS = ["cof_open", "cof_set_query", "cof_load_genome_file", "cof_load_ge
                "cof_mlp_loss_grad", "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model",
                "cof_load_genome_file_ex", "cof_device_count", "cof_device_name", "cof_write_tsv_ex",
                "cof_search_async", "cof_search_wait", "cof_hit_seqs", "cof_chrom_name", "cof_stream_begin",
-               "cof_stream_chunk", "cof_stream_end", "cof_search_batch"]
+               "cof_stream_chunk", "cof_stream_end", "cof_search_batch", "cof_search_static"]
 COF_WRITE_HOST = 1
 COF_LOAD_HOST = 1
 COF_LOAD_DEVICE = 2
@@ -102,6 +102,7 @@ def lib():
         L.cof_load_genome.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(CofChunk), C.c_uint64,
                                       C.POINTER(C.c_char_p), C.c_uint32]
         L.cof_search.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
+        L.cof_search_static.argtypes = [C.c_void_p, C.POINTER(CofHit), C.c_uint64, C.POINTER(C.c_uint64)]
         L.cof_write_tsv.argtypes = [C.c_void_p, C.c_char_p]
         L.cof_write_tsv_ex.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
         L.cof_search_async.argtypes = [C.c_void_p]
@@ -140,7 +141,7 @@ def lib():
                   "cof_get_stats", "cof_set_tuning", "cof_get_tuning", "cof_autotune", "cof_get_tile_stats",
                   "cof_mlp_create", "cof_mlp_get_params", "cof_mlp_set_params", "cof_mlp_forward", "cof_mlp_loss_grad",
                   "cof_mlp_train", "cof_chunk_features", "cof_set_cost_model", "cof_hit_seqs", "cof_stream_begin",
-                  "cof_stream_chunk", "cof_stream_end", "cof_search_batch"):
+                  "cof_stream_chunk", "cof_stream_end", "cof_search_batch", "cof_search_static"):
             getattr(L, f).restype = C.c_int
         _lib = L
     return _lib

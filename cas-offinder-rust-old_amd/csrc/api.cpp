@@ -41,6 +41,11 @@ struct cof_ctx {
     // chunks of a genome being streamed in (cof_stream_*)
     bool streaming = false;
     cof::Genome stream_g;
+    // allocation-free plain search (cof_search_static): the batch layout's
+    // index, and host buffers sized when the context opens
+    cof::BatchIndex bidx;
+    std::unique_ptr<cof::PlainRaw[]> static_stage;
+    uint32_t static_counts[64 * 32];
     bool cached = false;
     bool pending = false;          // between cof_search_async and cof_search_wait
     std::chrono::steady_clock::time_point search_t0;
@@ -86,11 +91,20 @@ std::vector<cof::Batch> layout_batches(const cof_ctx* ctx) {
     return {cof::Batch{0, ctx->g.chunks.size(), ctx->single_batch > 0}};
 }
 
+void set_batches(cof_ctx* ctx) {
+    ctx->batches = layout_batches(ctx);
+    ctx->bidx = cof::BatchIndex(ctx->g, ctx->batches);
+}
+
 void finish_genome(cof_ctx* ctx) {
-    ctx->ws.img_units = 0;   // the scan image follows the new genome at the next plain scan
+    ctx->ws.img_units = 0;   // the scan image follows the new genome
     ctx->ws.genome_nib = ctx->g.n_nibbles();
     ctx->g.index();
-    ctx->batches = layout_batches(ctx);
+    set_batches(ctx);
+    // the genome's scan image is part of making it resident, so that a plain
+    // search (cof_search_static included) finds it built
+    cof::plain_prepare_image(ctx->ws, ctx->d_genome, ctx->stream);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
     ctx->have_genome = true;
     ctx->cached = false;
     ctx->stats.genome_nucl = 0;
@@ -277,6 +291,8 @@ cof_status cof_open(int hip_device, const cof_query* q, cof_ctx** out) {
         ctx->device = hip_device;
         HIPCHECK(hipSetDevice(hip_device));
         HIPCHECK(hipStreamCreate(&ctx->stream));
+        cof::plain_static_init(hip_device);
+        ctx->static_stage.reset(new cof::PlainRaw[cof::plain_static_max_hits()]);
         ctx->q = cof::prepare_query(q);
         ctx->plain = cof::plain_plan_create(ctx->q);
         ctx->bulge = cof::bulge_plan_create(ctx->q);
@@ -312,7 +328,7 @@ cof_status cof_set_query(cof_ctx* ctx, const cof_query* q) {
         ctx->bulge = nb;
         bool batch_changed = nq.batch_nucl != ctx->q.batch_nucl;
         ctx->q = std::move(nq);
-        if (ctx->have_genome && batch_changed) ctx->batches = layout_batches(ctx);
+        if (ctx->have_genome && batch_changed) set_batches(ctx);
         ctx->cached = false;
         ctx->hits.clear();
         ctx->aux.clear();
@@ -562,6 +578,36 @@ cof_status cof_hit_seqs(cof_ctx* ctx, uint64_t first, uint64_t n, char* dna, cha
 const char* cof_chrom_name(const cof_ctx* ctx, uint32_t chrom) {
     if (!ctx || !ctx->have_genome || chrom >= ctx->g.chrom_names.size()) return nullptr;
     return ctx->g.chrom_names[chrom].c_str();
+}
+
+cof_status cof_search_static(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits) {
+    if (!ctx) return COF_E_INVALID;
+    if (ctx->pending) return refuse_pending(ctx);
+    if (!ctx->have_genome) return COF_E_NOGENOME;
+    if (ctx->q.any_bulge) {
+        ctx->err = "cof_search_static serves plain queries (every guide with D = R = 0)";
+        return COF_E_UNSUPPORTED;
+    }
+    try {
+        HIPCHECK(hipSetDevice(ctx->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        cof::RawView<cof::PlainRaw> raw;
+        uint64_t launches = 0;
+        cof::plain_search_static(ctx->plain, ctx->ws, ctx->d_genome, ctx->g.n_nibbles(), ctx->stream, ctx->plain_tune,
+                                 ctx->static_counts, ctx->static_stage.get(), raw, &launches);
+        const uint64_t n = cof::convert_plain_into(ctx->g, ctx->q, ctx->batches, ctx->bidx, raw, nullptr);
+        if (n_hits) *n_hits = n;
+        if (out && cap < n) return COF_E_OVERFLOW;   // nothing written
+        if (out) cof::convert_plain_into(ctx->g, ctx->q, ctx->batches, ctx->bidx, raw, out);
+        ctx->stats.n_plain_launches = launches;
+        ctx->stats.n_raw_hits = raw.size();
+        ctx->stats.n_hits = n;
+        ctx->stats.search_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    } catch (const cof::Error& e) {
+        return fail(ctx, e);
+    }
+    return COF_OK;
 }
 
 cof_status cof_search_async(cof_ctx* ctx) {

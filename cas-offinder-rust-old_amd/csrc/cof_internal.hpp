@@ -195,6 +195,23 @@ struct RawView {
     const T& operator[](size_t i) const { return p[i]; }
 };
 
+// Per chunk and per batch facts of a batch layout the converters look up.
+struct BatchIndex {
+    std::vector<int64_t> nonhead_batch;    // per chunk: the batch holding it as a member (-1: none)
+    std::vector<int64_t> head_batch;       // per chunk: the batch holding it as its head chunk (-1: none)
+    std::vector<uint64_t> end_nib;         // per batch: end of its last chunk's bytes (nibbles)
+    std::vector<uint64_t> active_begin;    // per batch: first end position scanned (bulge path)
+    BatchIndex() = default;
+    BatchIndex(const Genome& g, const std::vector<Batch>& bs);
+};
+// One raw plain hit -> final hit: the batch that reports it, the seam and
+// drop rules, the writer predicate on that batch's view (false: not reported).
+bool place_plain_hit(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                     const PlainRaw& r, cof_hit& h, HitAux& a);
+// The same over raw hits in order, one thread, into out (null: count only);
+// returns the number of final hits.  Allocates nothing.
+uint64_t convert_plain_into(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                            RawView<PlainRaw> raw, cof_hit* out);
 void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& batches,
                    RawView<PlainRaw> raw, HitVec& out, AuxVec& aux);
 void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches,

@@ -163,6 +163,22 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
 void plain_finish(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, hipStream_t st, RawView<PlainRaw>& out,
                   double* kernel_ms, uint64_t* n_launches, const PlainTune& tune);
 
+// Allocation-free plain search (cof_search_static).  plain_static_init, once
+// per device with that device current, looks up the device's copy of the
+// static scratch array; plain_search_static then scans with the hit segments,
+// counters and packed hits there (throws COF_E_NOMEM when a hit segment would
+// overflow), copies the counters to h_counts (kHitSegs * 32 words) and the
+// hits to h_stage (plain_static_max_hits() records) and points `out` at them.
+// It needs the genome's scan image, which plain_prepare_image builds when a
+// genome is loaded.
+constexpr uint32_t kMaxStaticDevices = 64;
+void plain_static_init(int device);
+uint64_t plain_static_max_hits();
+void plain_search_static(PlainPlan* pp, const Workspace& ws, const uint32_t* d_genome, uint64_t scan_end,
+                         hipStream_t st, const PlainTune& tune, uint32_t* h_counts, PlainRaw* h_stage,
+                         RawView<PlainRaw>& out, uint64_t* n_launches);
+void plain_prepare_image(Workspace& ws, const uint32_t* d_genome, hipStream_t st);
+
 struct BulgePlan;
 BulgePlan* bulge_plan_create(const Query& q);
 // count-only mode: device applies the chromosome-end drop rule and folds hits

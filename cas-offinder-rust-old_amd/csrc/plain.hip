@@ -49,6 +49,16 @@
 #include "engine.hpp"
 #include "hip_check.hpp"
 
+// Device scratch of the allocation-free plain search (cof_search_static): the
+// 64 hit-segment counters (kSegStride u32 apart), 64 segments of
+// kStaticSegCap records and a packed copy of them.  One copy per GPU, held
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (plain_static_init).
+constexpr uint32_t kStaticSegCap = 8192;
+constexpr size_t kStaticCountBytes = 64 * 32 * sizeof(uint32_t);
+constexpr size_t kStaticSegBytes = (size_t)64 * kStaticSegCap * sizeof(cof::PlainRaw);
+static __device__ __align__(256) uint8_t scratch[kStaticCountBytes + 2 * kStaticSegBytes];
+
 namespace cof {
 
 namespace {
@@ -722,6 +732,12 @@ __global__ __launch_bounds__(64) void scan_image_kernel(const uint32_t* __restri
 }
 
 // Packs the hit segments into one array: segment s goes after segments
+// n words of p set to v (the static search clears its counters with this
+// kernel rather than a memset)
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // 0..s-1 (counts already checked against the segment capacity).
 __global__ __launch_bounds__(256) void plain_compact_kernel(const PlainRaw* __restrict__ in,
                                                             const uint32_t* __restrict__ counts, uint32_t seg_cap,
@@ -1195,6 +1211,107 @@ void plain_run(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64_t 
                RawView<PlainRaw>& out, double* kernel_ms, uint64_t* n_launches, const PlainTune& tune) {
     plain_launch(pp, ws, d_genome, scan_end, st, n_launches, tune);
     plain_finish(pp, ws, d_genome, st, out, kernel_ms, n_launches, tune);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free plain search: the same scan, its hit segments, counters and
+// packed copy in the static `scratch` array instead of the workspace.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_static_m;
+void* g_static_ptr[kMaxStaticDevices] = {};
+}  // namespace
+
+static_assert(kStaticSegCap * 64ull * sizeof(PlainRaw) == kStaticSegBytes, "static scratch layout");
+static_assert(kHitSegs == 64 && kSegStride == 32, "static scratch counter layout");
+
+void plain_static_init(int device) {
+    if (device < 0 || device >= (int)kMaxStaticDevices)
+        throw Error(COF_E_INVALID, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lk(g_static_m);
+    if (g_static_ptr[device]) return;
+    int cur = -1;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != device) throw Error(COF_E_INVALID, "plain_static_init: device is not current");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(COF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_static_ptr[device] = p;
+}
+
+uint64_t plain_static_max_hits() { return (uint64_t)kHitSegs * kStaticSegCap; }
+
+void plain_search_static(PlainPlan* pp, const Workspace& ws, const uint32_t* d_genome, uint64_t scan_end,
+                         hipStream_t st, const PlainTune& tune, uint32_t* h_counts, PlainRaw* h_stage,
+                         RawView<PlainRaw>& out, uint64_t* n_launches) {
+    out = RawView<PlainRaw>{};
+    if (plain_plan_empty(pp) || scan_end == 0) return;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_static_m);
+        if (dev >= 0 && dev < (int)kMaxStaticDevices) base = g_static_ptr[dev];
+    }
+    if (!base) throw Error(COF_E_INVALID, "static scratch not set up on this device (cof_open does it)");
+    const uint64_t widest = plain_tile_nucl();
+    if (ws.img_units < (scan_end + widest - 1) / widest * (widest / 4096) + 1)
+        throw Error(COF_E_INVALID, "the genome's scan image is missing (it is built when a genome is loaded)");
+    uint32_t* const counts = (uint32_t*)base;
+    PlainRaw* const segs = (PlainRaw*)((uint8_t*)base + kStaticCountBytes);
+    PlainRaw* const packed = (PlainRaw*)((uint8_t*)base + kStaticCountBytes + kStaticSegBytes);
+    const uint32_t n_counts = kHitSegs * kSegStride;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(8), dim3(256), 0, st, counts, n_counts, 0u);
+    HIPCHECK(hipGetLastError());
+    const uint64_t kTile = (uint64_t)tune.threads * 64;
+    const uint64_t n_tiles = (scan_end + kTile - 1) / kTile;
+    for (const auto& l : pp->launches) {
+        for (uint64_t t0 = 0; t0 < n_tiles; t0 += (1u << 20)) {
+            uint32_t nb = (uint32_t)std::min<uint64_t>(n_tiles - t0, 1u << 20);
+            uint64_t first = t0 * kTile;
+            const GroupDev* g = pp->d_groups + l.group_begin;
+            uint32_t ny = tune.pat_chunk ? (l.max_pats + tune.pat_chunk - 1) / tune.pat_chunk : 1u;
+            dim3 grid(nb, std::max(1u, ny));
+#define COF_PLAIN_LAUNCH(WW)                                                                                   \
+    case WW:                                                                                                   \
+        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
+                         pp->d_full, pp->d_frag, ws, segs, counts, kStaticSegCap);                            \
+        break;
+            switch (l.W) {
+                COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
+                COF_PLAIN_LAUNCH(5) COF_PLAIN_LAUNCH(6) COF_PLAIN_LAUNCH(7) COF_PLAIN_LAUNCH(8)
+                default: throw Error(COF_E_UNSUPPORTED, "compare window too wide");
+            }
+#undef COF_PLAIN_LAUNCH
+            HIPCHECK(hipGetLastError());
+            if (n_launches) (*n_launches)++;
+        }
+    }
+    HIPCHECK(hipMemcpyAsync(h_counts, counts, kStaticCountBytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    uint64_t total = 0;
+    uint32_t most = 0;
+    for (uint32_t k = 0; k < kHitSegs; k++) {
+        total += h_counts[k * kSegStride];
+        most = std::max(most, h_counts[k * kSegStride]);
+    }
+    if (most > kStaticSegCap)
+        throw Error(COF_E_NOMEM, "plain hits exceed the static scratch: a hit segment holds at most " +
+                                     std::to_string(kStaticSegCap) + " records and one received " +
+                                     std::to_string(most) + " (cof_search has no such limit)");
+    if (total) {
+        hipLaunchKernelGGL(plain_compact_kernel, dim3(kHitSegs), dim3(256), 0, st, (const PlainRaw*)segs, counts,
+                           kStaticSegCap, packed);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(h_stage, packed, total * sizeof(PlainRaw), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    out = RawView<PlainRaw>{h_stage, (size_t)total};
+}
+
+void plain_prepare_image(Workspace& ws, const uint32_t* d_genome, hipStream_t st) {
+    if (ws.genome_nib) ensure_scan_image(ws, d_genome, ws.genome_nib, st);
 }
 
 }  // namespace cof

@@ -24,25 +24,19 @@
 
 namespace cof {
 
-namespace {
-
-struct BatchIndex {
-    std::vector<int64_t> nonhead_batch;    // per chunk: the batch holding it as a member (-1: none)
-    std::vector<int64_t> head_batch;       // per chunk: the batch holding it as its head chunk (-1: none)
-    std::vector<uint64_t> end_nib;         // per batch: end of its last chunk's bytes (nibbles)
-    std::vector<uint64_t> active_begin;    // per batch: first end position scanned (bulge path)
-    BatchIndex(const Genome& g, const std::vector<Batch>& bs) {
-        nonhead_batch.assign(g.chunks.size(), -1);
-        head_batch.assign(g.chunks.size(), -1);
-        for (size_t b = 0; b < bs.size(); b++) {
-            if (bs[b].has_head) head_batch[bs[b].first] = (int64_t)b;
-            for (size_t c = bs[b].first + (bs[b].has_head ? 1 : 0); c < bs[b].last; c++) nonhead_batch[c] = (int64_t)b;
-            uint64_t e = bs[b].last < g.chunks.size() ? g.chunks[bs[b].last].byte_off : g.n_bytes;
-            end_nib.push_back(e * 2);
-            active_begin.push_back(g.chunks[bs[b].first + (bs[b].has_head ? 1 : 0)].byte_off * 2);
-        }
+BatchIndex::BatchIndex(const Genome& g, const std::vector<Batch>& bs) {
+    nonhead_batch.assign(g.chunks.size(), -1);
+    head_batch.assign(g.chunks.size(), -1);
+    for (size_t b = 0; b < bs.size(); b++) {
+        if (bs[b].has_head) head_batch[bs[b].first] = (int64_t)b;
+        for (size_t c = bs[b].first + (bs[b].has_head ? 1 : 0); c < bs[b].last; c++) nonhead_batch[c] = (int64_t)b;
+        uint64_t e = bs[b].last < g.chunks.size() ? g.chunks[bs[b].last].byte_off : g.n_bytes;
+        end_nib.push_back(e * 2);
+        active_begin.push_back(g.chunks[bs[b].first + (bs[b].has_head ? 1 : 0)].byte_off * 2);
     }
-};
+}
+
+namespace {
 
 // Writer predicate of main.rs:158-194 evaluated in pattern orientation on the
 // N-masked window (search.rs:512-531), with nibbles at or past view_end = 0.
@@ -185,61 +179,82 @@ void parallel_convert(size_t n, HitVec& out, AuxVec& aux, Fn fn) {
 }
 }  // namespace
 
+bool place_plain_hit(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                     const PlainRaw& r, cof_hit& h, HitAux& a) {
+    const Pattern& p = q.pats[r.pattern];
+    size_t c = g.chunk_of(r.pos);
+    const Chunk& ch = g.chunks[c];
+    uint64_t chrom_pos = ch.start + (r.pos - ch.nib_begin());
+    bool past_end = chrom_pos + p.L > ch.end;
+    const int64_t b = bi.nonhead_batch[c], hb = bi.head_batch[c];
+    if (b < 0 && hb < 0) return false;
+    bool next_new_chrom = c + 1 < g.chunks.size() && g.chunks[c + 1].start == 0;
+    uint32_t mm = 0;
+    uint64_t view_end = 0;
+    bool emit = false;
+    // The device evaluated the predicate on the resident genome, which
+    // equals the batch view whenever the window ends inside it; only
+    // windows running past the view end are evaluated again here.
+    auto check = [&](uint64_t ve) {
+        view_end = ve;
+        if (r.pos + p.L <= ve) { mm = r.flags; return true; }
+        return plain_predicate(g, p, r.pos, ve, &mm);
+    };
+    // reported by batch b, where c is not the head
+    if (b >= 0) {
+        bool last_in_b = c + 1 == bs[(size_t)b].last;
+        if (!((last_in_b || next_new_chrom) && past_end)) emit = check(bi.end_nib[(size_t)b]);
+    }
+    // reported by batch hb, where c is the head (search.rs:540-546); the
+    // two cases exclude each other (c is the last chunk of batch b, so
+    // the first case needs the window inside c, the second a start within
+    // L - 1 of c's end)
+    if (!emit && hb >= 0) {
+        uint64_t c1 = g.chunks[c + 1].byte_off * 2 - ch.nib_begin();
+        uint64_t min_start = c1 >= p.L - 1 ? c1 - (p.L - 1) : 0;
+        uint64_t local = r.pos - ch.nib_begin();
+        if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)hb]);
+    }
+    if (!emit) return false;
+    h = cof_hit{};
+    h.chrom_pos = chrom_pos;
+    h.chrom = ch.chrom;
+    h.pattern = r.pattern;
+    h.mismatches = (uint16_t)mm;
+    a = HitAux{r.pos, view_end};
+    return true;
+}
+
 void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<PlainRaw> raw,
                    HitVec& out_all, AuxVec& aux_all) {
     BatchIndex bi(g, bs);
     parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, cof_hit* out, HitAux* aux) -> size_t {
-    size_t k = 0;
-    for (size_t ri = b0; ri < b1; ri++) {
-        const PlainRaw& r = raw[ri];
-        const Pattern& p = q.pats[r.pattern];
-        size_t c = g.chunk_of(r.pos);
-        const Chunk& ch = g.chunks[c];
-        uint64_t chrom_pos = ch.start + (r.pos - ch.nib_begin());
-        bool past_end = chrom_pos + p.L > ch.end;
-        const int64_t b = bi.nonhead_batch[c], hb = bi.head_batch[c];
-        if (b < 0 && hb < 0) continue;
-        bool next_new_chrom = c + 1 < g.chunks.size() && g.chunks[c + 1].start == 0;
-        uint32_t mm = 0;
-        uint64_t view_end = 0;
-        bool emit = false;
-        // The device evaluated the predicate on the resident genome, which
-        // equals the batch view whenever the window ends inside it; only
-        // windows running past the view end are evaluated again here.
-        auto check = [&](uint64_t ve) {
-            view_end = ve;
-            if (r.pos + p.L <= ve) { mm = r.flags; return true; }
-            return plain_predicate(g, p, r.pos, ve, &mm);
-        };
-        // reported by batch b, where c is not the head
-        if (b >= 0) {
-            bool last_in_b = c + 1 == bs[(size_t)b].last;
-            if (!((last_in_b || next_new_chrom) && past_end)) emit = check(bi.end_nib[(size_t)b]);
+        size_t k = 0;
+        cof_hit h;
+        HitAux a;
+        for (size_t ri = b0; ri < b1; ri++) {
+            if (!place_plain_hit(g, q, bs, bi, raw[ri], h, a)) continue;
+            if (out) {
+                out[k] = h;
+                aux[k] = a;
+            }
+            k++;
         }
-        // reported by batch hb, where c is the head (search.rs:540-546); the
-        // two cases exclude each other (c is the last chunk of batch b, so
-        // the first case needs the window inside c, the second a start within
-        // L - 1 of c's end)
-        if (!emit && hb >= 0) {
-            uint64_t c1 = g.chunks[c + 1].byte_off * 2 - ch.nib_begin();
-            uint64_t min_start = c1 >= p.L - 1 ? c1 - (p.L - 1) : 0;
-            uint64_t local = r.pos - ch.nib_begin();
-            if (local >= min_start && !(next_new_chrom && past_end)) emit = check(bi.end_nib[(size_t)hb]);
-        }
-        if (!emit) continue;
-        if (out) {
-            cof_hit h{};
-            h.chrom_pos = chrom_pos;
-            h.chrom = ch.chrom;
-            h.pattern = r.pattern;
-            h.mismatches = (uint16_t)mm;
-            out[k] = h;
-            aux[k] = HitAux{r.pos, view_end};
-        }
+        return k;
+    });
+}
+
+uint64_t convert_plain_into(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                            RawView<PlainRaw> raw, cof_hit* out) {
+    uint64_t k = 0;
+    cof_hit h;
+    HitAux a;
+    for (size_t ri = 0; ri < raw.size(); ri++) {
+        if (!place_plain_hit(g, q, bs, bi, raw[ri], h, a)) continue;
+        if (out) out[k] = h;
         k++;
     }
     return k;
-    });
 }
 
 void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<BulgeRaw> raw,

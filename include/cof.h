@@ -253,6 +253,23 @@ cof_status cof_search_batch(cof_ctx* ctx, const uint8_t* packed, uint64_t n_byte
                             uint64_t n_chunks, const char* const* chrom_names, uint32_t n_chroms, uint32_t has_head,
                             uint64_t* n_hits);
 
+/* cof_search for plain queries (every guide with D = R = 0) without any
+ * allocation in the call: no hipMalloc, hipFree, pinned or pageable host
+ * allocation.  No reference counterpart; it is for callers that cannot
+ * allocate (serving loops, harnesses that forbid allocation while they run).
+ * The scan's hit segments, their counters and the packed hits live in one
+ * static device array per GPU (16 MiB + 8 KiB, held until the process
+ * ends): a search whose hits put more than 8,192 records into one of its 64
+ * segments (at most 524,288 raw hits in all) fails with COF_E_NOMEM and a
+ * message in cof_last_error, where cof_search has no limit.  Every call
+ * searches (nothing is cached): the hit count goes to *n_hits; with out !=
+ * NULL and cap >= that count the hits (the records cof_search returns, in
+ * unspecified order) are written to out, else COF_E_OVERFLOW and nothing is
+ * written.  Calls on one GPU share its array: calls that overlap in time on
+ * one GPU (from several host threads) race.  COF_E_UNSUPPORTED for a query
+ * with bulge guides. */
+cof_status cof_search_static(cof_ctx* ctx, cof_hit* out, uint64_t cap, uint64_t* n_hits);
+
 /* cof_search (count query) in two calls: cof_search_async queues the device
  * work of a search and returns; cof_search_wait completes it (hits copied
  * back and converted, any bulge pattern searched) and writes the count.  In
