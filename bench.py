@@ -16,6 +16,9 @@ library's stream.  cpu_baseline times the CPU oracle (a restatement of the
 reference's CPU path) on a bounded sample of the same workload.
 """
 import argparse
+import csv
+import glob
+import hashlib
 import json
 import math
 import os
@@ -31,11 +34,181 @@ import cofpy  # noqa: E402
 
 INT32_PEAK = 256 * 64 * 2.4e9       # gfx950: 256 CUs x 64 lanes x 2.4 GHz int32 VALU ops/s
 HBM_PEAK = 8.0e12
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions/s: 4 SIMDs per CU, 4 cycles each
+MFMA_I8_PEAK = 5.0e15                  # dense int8 matrix-core ops/s
+DIGESTS = os.path.join(ROOT, "tests", "golden", "headline_3.1Gbp_digest.json")
+COUNTERS = os.path.join(ROOT, "profiles", "plain_scan_counters.json")
+PLAIN_SRC = os.path.join(ROOT, "cas-offinder-rust-old_amd", "csrc", "plain.hip")
 PAMS = [("NRG", True), ("NNGRRT", True), ("TTTN", False)]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def file_sha256(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def rows_digest(path):
+    """(row count, SHA-256 of the sorted data rows, each followed by a newline) of a reference TSV."""
+    rows = cofpy.read_rows(path)
+    h = hashlib.sha256()
+    for r in rows:
+        h.update(r.encode())
+        h.update(b"\n")
+    return len(rows), h.hexdigest()
+
+
+def measured_counters():
+    """Counter figures of plain_scan_kernel from the committed profile summary, or None when the kernel
+    source changed since they were measured (they would describe another kernel)."""
+    if not os.path.exists(COUNTERS):
+        return None, "no committed counter summary"
+    with open(COUNTERS) as f:
+        doc = json.load(f)
+    if doc.get("plain_hip_sha256") != file_sha256(PLAIN_SRC):
+        return None, "profiles/plain_scan_counters.json was measured on another plain.hip"
+    return doc, "profiles/plain_scan_counters.json (measured on this plain.hip)"
+
+
+# ----------------------------------------------------------------------------- rocprofv3 summaries
+def _csv_rows(d, suffix):
+    out = []
+    for path in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(path, newline="") as f:
+            out += list(csv.DictReader(f))
+    return out
+
+
+def _dispatches(d):
+    """dispatch id -> {"name", "ns", counter: value} from a rocprofv3 --pmc csv folder."""
+    disp = {}
+    for r in _csv_rows(d, "counter_collection.csv"):
+        k = int(r["Dispatch_Id"])
+        e = disp.setdefault(k, {"name": r["Kernel_Name"], "ns": int(r["End_Timestamp"]) - int(r["Start_Timestamp"]),
+                                "vgpr": r.get("Arch_VGPR_Count") or r.get("VGPR_Count"),
+                                "lds": r.get("LDS_Block_Size") or r.get("Group_Segment_Size")})
+        e[r["Counter_Name"]] = e.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return disp
+
+
+def _last_plain(disp, n):
+    ks = sorted(k for k, e in disp.items() if "plain_scan_kernel" in e["name"])
+    if len(ks) < n:
+        raise SystemExit(f"expected at least {n} plain_scan_kernel dispatches, found {len(ks)}")
+    return [disp[k] for k in ks[-n:]]
+
+
+def _copy_factor(disp, counter, nbytes):
+    """bytes per counter unit: the known size of the calibration copy over what the counter reads for it."""
+    ks = [k for k, e in disp.items() if counter in e and ("copy" in e["name"].lower() or "elementwise" in e["name"].lower())]
+    if not ks:
+        raise SystemExit(f"no calibration copy kernel with {counter} found")
+    return nbytes / disp[max(ks, key=lambda k: disp[k][counter])][counter]
+
+
+def summarize_counters(args):
+    """profiles/plain_scan_counters.json + a markdown table from rocprofv3 runs of this script with
+    --no-overlap --warmup 1 --steps 1 --calibrate-copy --manifest M (one pass per folder)."""
+    with open(args.manifest) as f:
+        man = json.load(f)
+    runs = man["runs"]            # [[name, launches, hits, L, genome_nucl]] in dispatch order
+    n = sum(r[1] for r in runs)
+    names = [r[0] for r in runs for _ in range(r[1])]
+    per = [{"run": nm} for nm in names]
+    out = {"plain_hip_sha256": file_sha256(PLAIN_SRC), "workload": man["workload"], "units": {}}
+    fetch = _dispatches(args.counters_from[0])
+    fdisp = _last_plain(fetch, n)
+    ffac = _copy_factor(fetch, "FETCH_SIZE", man["copy_bytes"])
+    write = _dispatches(args.counters_from[1])
+    wdisp = _last_plain(write, n)
+    wfac = _copy_factor(write, "WRITE_SIZE", man["copy_bytes"])
+    sq = _last_plain(_dispatches(args.counters_from[2]), n)
+    mf = _last_plain(_dispatches(args.counters_from[3]), n)
+    for i, p in enumerate(per):
+        t = sq[i]["ns"] * 1e-9
+        p["ms_pmc"] = round(sq[i]["ns"] / 1e6, 3)
+        p["fetch_gb"] = round(fdisp[i]["FETCH_SIZE"] * ffac / 1e9, 4)
+        p["write_gb"] = round(wdisp[i]["WRITE_SIZE"] * wfac / 1e9, 4)
+        p["valu_issue"] = round(sq[i]["SQ_INSTS_VALU"] / (t * VALU_ISSUE_PEAK), 3)
+        p["valu_per_wave"] = round(sq[i]["SQ_INSTS_VALU"] / sq[i]["SQ_WAVES"], 1)
+        p["lds_per_wave"] = round(sq[i]["SQ_INSTS_LDS"] / sq[i]["SQ_WAVES"], 1)
+        p["bank_conflict_cycles_per_lds"] = round(sq[i]["SQ_LDS_BANK_CONFLICT"] / max(sq[i]["SQ_INSTS_LDS"], 1), 3)
+        p["clock_ghz"] = round(sq[i]["GRBM_GUI_ACTIVE"] / 8 / t / 1e9, 2)
+        mt = mf[i]["ns"] * 1e-9
+        p["mfma_i8_tops"] = round(mf[i]["SQ_INSTS_VALU_MFMA_MOPS_I8"] * 512 / mt / 1e12, 1)
+        clk = mf[i]["GRBM_GUI_ACTIVE"] / 8
+        p["mfma_busy"] = round(mf[i]["SQ_VALU_MFMA_BUSY_CYCLES"] / (clk * 1024), 3)
+        p["vgpr"] = sq[i]["vgpr"]
+        p["lds_bytes"] = sq[i]["lds"]
+    tot = sum(p["ms_pmc"] for p in per)
+    w = lambda k: round(sum(p[k] * p["ms_pmc"] for p in per) / tot, 3)
+    out.update({"runs": per, "traffic_bytes_per_launch": round(sum((p["fetch_gb"] + p["write_gb"]) * 1e9 for p in per) / n),
+                "fetch_factor": round(ffac, 4), "write_factor": round(wfac, 4),
+                "valu_issue_frac": w("valu_issue"), "mfma_busy_frac": w("mfma_busy"),
+                "mfma_i8_tops": round(sum(p["mfma_i8_tops"] * p["ms_pmc"] for p in per) / tot, 1),
+                "bank_conflict_cycles_per_lds": w("bank_conflict_cycles_per_lds")})
+    out["units"] = {"fetch_gb": "FETCH_SIZE x fetch_factor (bytes a 256 MiB copy reads per unit)",
+                    "write_gb": "WRITE_SIZE x write_factor", "valu_issue": "SQ_INSTS_VALU / (time x 614.4 G/s)",
+                    "mfma_busy": "SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs x 1024 SIMDs)",
+                    "mfma_i8_tops": "SQ_INSTS_VALU_MFMA_MOPS_I8 x 512 / time",
+                    "weights": "time-weighted means over the runs"}
+    with open(args.counters_out or COUNTERS, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    cols = ["run", "ms_pmc", "fetch_gb", "write_gb", "valu_per_wave", "valu_issue", "lds_per_wave",
+            "bank_conflict_cycles_per_lds", "mfma_i8_tops", "mfma_busy", "clock_ghz", "vgpr", "lds_bytes"]
+    print("| " + " | ".join(cols) + " |")
+    print("|" + "---|" * len(cols))
+    for p in per:
+        print("| " + " | ".join(str(p[c]) for c in cols) + " |")
+    print(json.dumps({k: v for k, v in out.items() if k != "runs"}))
+
+
+def summarize_bulge_counters(args):
+    """profiles/bulge_counters.json: the bulge workload's filter and traceback kernels summed over their
+    dispatches (rocprofv3 --pmc runs of --workload bulge --schedule sync --calibrate-copy), each placed on the
+    HBM and VALU-issue rooflines."""
+    fetch, write, sq, mf = (_dispatches(d) for d in args.counters_from)
+    ffac = _copy_factor(fetch, "FETCH_SIZE", 256 << 20)
+    wfac = _copy_factor(write, "WRITE_SIZE", 256 << 20)
+    out = {"bulge_hip_sha256": file_sha256(os.path.join(ROOT, "cas-offinder-rust-old_amd", "csrc", "bulge.hip")),
+           "workload": "bench.py --workload bulge (config 3: 3.1 Gbp repeat-rich, 100 guides NRG M4 D2 R2), "
+                       "synchronous rounds", "kernels": {}}
+    for key in ("bulge_filter_kernel", "bulge_verify_kernel"):
+        def tot(disp, c):
+            return sum(e.get(c, 0.0) for e in disp.values() if key in e["name"])
+        ds = [e for e in sq.values() if key in e["name"]]
+        if not ds:
+            continue
+        t = sum(e["ns"] for e in ds) * 1e-9
+        tm = sum(e["ns"] for e in mf.values() if key in e["name"]) * 1e-9
+        fb, wb = tot(fetch, "FETCH_SIZE") * ffac, tot(write, "WRITE_SIZE") * wfac
+        tf = sum(e["ns"] for e in fetch.values() if key in e["name"]) * 1e-9
+        k = {"dispatches": len(ds), "ms_pmc": round(t * 1e3, 2),
+             "fetch_gb": round(fb / 1e9, 3), "write_gb": round(wb / 1e9, 3),
+             "hbm_gb_s": round((fb + wb) / max(tf, 1e-12) / 1e9, 1),
+             "hbm_frac": round((fb + wb) / max(tf, 1e-12) / HBM_PEAK, 4),
+             "valu_issue": round(tot(sq, "SQ_INSTS_VALU") / (t * VALU_ISSUE_PEAK), 3),
+             "salu_per_valu": round(tot(sq, "SQ_INSTS_SALU") / max(tot(sq, "SQ_INSTS_VALU"), 1), 3),
+             "valu_per_wave": round(tot(sq, "SQ_INSTS_VALU") / max(tot(sq, "SQ_WAVES"), 1), 1),
+             "lds_per_valu": round(tot(sq, "SQ_INSTS_LDS") / max(tot(sq, "SQ_INSTS_VALU"), 1), 3),
+             "bank_conflict_cycles_per_lds": round(tot(sq, "SQ_LDS_BANK_CONFLICT") / max(tot(sq, "SQ_INSTS_LDS"), 1), 3),
+             "clock_ghz": round(tot(sq, "GRBM_GUI_ACTIVE") / 8 / t / 1e9, 2),
+             "vgpr": ds[0]["vgpr"], "lds_bytes": ds[0]["lds"]}
+        if tm:
+            k["valu_busy"] = round(tot(mf, "SQ_ACTIVE_INST_VALU") / (tot(mf, "GRBM_GUI_ACTIVE") / 8 * 1024), 3) \
+                if tot(mf, "SQ_ACTIVE_INST_VALU") else None
+        out["kernels"][key] = k
+    out["units"] = {"fetch_gb": "FETCH_SIZE x fetch_factor over all dispatches", "hbm_frac": "(fetch + write) / "
+                    "FETCH run's dispatch time / 8 TB/s", "valu_issue": "SQ_INSTS_VALU / (time x 614.4 G/s)"}
+    out["fetch_factor"], out["write_factor"] = round(ffac, 4), round(wfac, 4)
+    with open(args.counters_out or os.path.join(ROOT, "profiles", "bulge_counters.json"), "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out, indent=1))
 
 
 def chrom_lengths(total, n, seed):
@@ -86,8 +259,21 @@ def cpu_baseline(run, sample_bp, seed, threads):
                       f"search-only time {secs:.2f} s, {n} rows"}
 
 
+def calibration_copy():
+    """A 256 MiB device-to-device copy kernel: the known size the traffic counters are scaled by."""
+    import torch
+    a = torch.empty((256 << 20) // 4, dtype=torch.int32, device="cuda").random_()
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    log("calibration copy of 256 MiB done")
+
+
 def bulge_workload(args):
     """Bulge search: 3.1 Gbp layout with 10% repeat copies, 100 guides, NRG, M4 D2 R2."""
+    if args.calibrate_copy:
+        calibration_copy()
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)
     rng = random.Random(args.seed + 5)
@@ -503,7 +689,21 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="run the autotuner on a 64 Mnt prefix first")
     ap.add_argument("--tune", action="append", default=[], metavar="FIELD=VALUE",
                     help="cof_tuning field for the bulge/batched/costmodel contexts (repeatable)")
+    ap.add_argument("--no-verify", action="store_true",
+                    help="plain workload: skip the check of every run's rows against the oracle digests")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="plain workload: run a 256 MiB device copy first (the traffic counters' calibration)")
+    ap.add_argument("--manifest", default=None,
+                    help="plain workload: write the runs in dispatch order (with --counters-from: read them)")
+    ap.add_argument("--counters-from", nargs=4, default=None, metavar=("FETCH", "WRITE", "SQ", "MFMA"),
+                    help="summarise four rocprofv3 --pmc folders of the plain workload into "
+                         "profiles/plain_scan_counters.json (or, with --workload bulge, of the bulge workload "
+                         "into profiles/bulge_counters.json)")
+    ap.add_argument("--counters-out", default=None,
+                    help="with --counters-from: where the JSON goes (default: the file under profiles/)")
     args = ap.parse_args()
+    if args.counters_from:
+        return summarize_bulge_counters(args) if args.workload == "bulge" else summarize_counters(args)
     if args.gpus != 1:
         print("bench.py measures one GPU only: the search is single-GPU by design "
               "(guide or genome sharding across GPUs needs only a host-side gather of hits)", file=sys.stderr)
@@ -521,6 +721,9 @@ def main():
     if args.workload == "cli":
         return cli_workload(args)
 
+    copy_bytes = 256 << 20
+    if args.calibrate_copy:
+        calibration_copy()
     total = int(args.genome_bp)
     lens = chrom_lengths(total, 24, args.seed)
     runs = make_queries(args.seed)
@@ -580,6 +783,26 @@ def main():
         log(f"step {k + 1}/{args.steps}: {step_s[-1] * 1e3:.1f} ms")
 
     genome_nucl = ctxs[0].stats().genome_nucl
+    default_size = abs(genome_nucl - 3.1e9) < 1e6 and args.seed == 2024
+    if args.manifest:
+        with open(args.manifest, "w") as f:
+            json.dump({"workload": "bench.py plain workload, 3.1 Gbp, 12 runs, --no-overlap", "copy_bytes": copy_bytes,
+                       "runs": [[r["name"], launches[i], hits[i], r["L"], genome_nucl] for i, r in enumerate(runs)]}, f)
+    # bit-exact check at full size: every run's sorted rows against the oracle's digests
+    # (tests/golden/make_headline_digest.py), after the timed steps
+    verify = None
+    if not args.no_verify and default_size and os.path.exists(DIGESTS):
+        with open(DIGESTS) as f:
+            want = json.load(f)["runs"]
+        verify = {}
+        with tempfile.TemporaryDirectory() as d:
+            for i, r in enumerate(runs):
+                path = os.path.join(d, "rows.tsv")        # the hits of the last timed step
+                ctxs[i].write_tsv(path)
+                n, sha = rows_digest(path)
+                os.remove(path)
+                verify[r["name"]] = {"rows": n, "match": n == want[r["name"]]["rows"] and sha == want[r["name"]]["sha256"]}
+        log(f"oracle digests: {sum(v['match'] for v in verify.values())}/{len(verify)} runs identical")
     units = sum(len(r["pats"]) for r in runs) * genome_nucl          # guide-bp per step
     step = sum(step_s) / len(step_s)
     value = units / step / 1e9
@@ -593,22 +816,16 @@ def main():
     alg_bytes = sum(launches[i] * genome_nucl / 2 + 16 * hits[i] for i in range(len(runs))) / n_launch
     launch_s = sum(kms) / 1e3 / n_launch
     hbm_achieved = alg_bytes / launch_s
-    # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
-    # this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a 256 MiB
-    # device copy reads on the same counters), mean of the 12 runs: the packed
-    # genome (1.55 GB) + the scan image's two bit planes (0.78 GB) + halos and
-    # pattern fragments; only valid at the default size
-    default_size = abs(genome_nucl - 3.1e9) < 1e6
-    traffic = 2.578e9 + 1.40e7 if default_size else None
-    # issue rates of the scan from rocprofv3 --pmc on this workload, time-weighted
-    # over the 12 runs: VALU = SQ_INSTS_VALU / (kernel time x 256 CU x 4 SIMD x
-    # 2.4 GHz / 4 cycles per wave64 instruction); matrix cores = i8 ops
-    # (SQ_INSTS_VALU_MFMA_MOPS_I8 x 512) / kernel time against 5.0 POP/s, and
-    # SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs)
-    valu_issue = 0.733 if default_size else None
-    mfma_tops = 1596 if default_size else None
-    mfma_busy = 0.336 if default_size else None
+    # HBM traffic per launch and issue rates: rocprofv3 --pmc figures of this
+    # workload, read from the committed summary (bench.py --counters-from), and
+    # null unless that summary was measured on the current plain.hip
+    cdoc, csrc = measured_counters() if default_size else (None, "not the default workload")
+    traffic = cdoc["traffic_bytes_per_launch"] if cdoc else None
+    valu_issue = cdoc["valu_issue_frac"] if cdoc else None
+    mfma_tops = cdoc["mfma_i8_tops"] if cdoc else None
+    mfma_busy = cdoc["mfma_busy_frac"] if cdoc else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
+                "oracle_rows_match": verify[r["name"]]["match"] if verify else None,
                 "search_ms": round(search_ms[i] / args.steps, 3), "convert_ms": round(convert_ms[i] / args.steps, 3),
                 "guide_gbp_s_kernel": round(len(r["pats"]) * genome_nucl / (kms[i] / 1e3) / 1e9, 1)}
                for i, r in enumerate(runs)]
@@ -650,12 +867,17 @@ def main():
                            "note": "achieved = brute-force int32 ops 2*(3*ceil(L/8)+1) per guide-bp / plain-scan "
                                    "kernel time; the kernel tests the PAM first and so executes fewer ops"},
         "counters": {"valu_issue_frac": valu_issue, "mfma_i8_tops": mfma_tops,
-                     "mfma_i8_frac": round(mfma_tops / 5000, 3) if mfma_tops else None,
+                     "mfma_i8_frac": round(mfma_tops * 1e12 / MFMA_I8_PEAK, 3) if mfma_tops else None,
                      "mfma_busy_frac": mfma_busy,
+                     "bank_conflict_cycles_per_lds": cdoc["bank_conflict_cycles_per_lds"] if cdoc else None,
+                     "source": csrc,
                      "note": "measured issue of plain_scan_kernel (rocprofv3 --pmc, default workload only): VALU "
                              "instructions against the gfx950 issue peak; i8 matrix-core ops against the dense "
                              "int8 peak of 5.0 POP/s and matrix-core busy cycles"},
         "cpu_baseline": baseline,
+        "bit_exact": (all(v["match"] for v in verify.values()) if verify is not None else None),
+        "bit_exact_check": (f"sorted rows of all {len(verify)} runs vs the oracle's row counts and SHA-256 digests "
+                            "(tests/golden/headline_3.1Gbp_digest.json)" if verify is not None else "not run"),
         "kernel_guide_gbp_s": round(units / (sum(kms) / 1e3) / 1e9, 1),
         "per_run": per_run,
     }

@@ -46,6 +46,32 @@ def _case(name, seed, lens, pam, three, glen, n_guides, M, D=0, R=0, line_width=
             "filter": filt, "patterns": pats, "M": M, "D": D, "R": R, "batch_nucl": batch_nucl}
 
 
+def _mixed_case(name, seed, lens, n_guides, max_bulge_edits=6, repeats=(0.10, 20, 300, 0.10), batch_nucl=1 << 22):
+    """One query mixing what one reference run cannot (cli_utils.rs:97-108: one filter, length and M per run):
+    the batched workload's guide mix (length U{17..30}, PAM in {NRG, NGG, NNGRRT, TTTN}, M U{0..6}, 20% with
+    D, R in U{1,2}), except that a bulge guide draws M from 0..max_bulge_edits - D - R: a 17-nt guide at
+    M6 D2 R2 aligns ~3e7 ways per Mbp of this genome, too many rows for a fixture.  The expected rows are the
+    union of one oracle run per (filter, M, D, R) set."""
+    rng = random.Random(seed)
+    guides = []
+    while len(guides) < n_guides:
+        glen = rng.randint(17, 30)
+        pam = rng.choice(["NRG", "NGG", "NNGRRT", "TTTN"])
+        g = "".join(rng.choice("ACGT") for _ in range(glen))
+        mm = rng.randint(0, 6)
+        d, r = (rng.randint(1, 2), rng.randint(1, 2)) if rng.random() < 0.2 else (0, 0)
+        if d + r:
+            mm = min(mm, max_bulge_edits - d - r)
+        if pam == "TTTN":
+            guides.append(("NNNN" + g, pam + "N" * glen, mm, d, r, (pam + g, 4, glen, min(mm, 3) or 1, 4)))
+        else:
+            guides.append((g + "N" * len(pam), "N" * glen + pam, mm, d, r, (g + pam, 0, glen, min(mm, 3) or 1, 4)))
+    plants = [x[5] for x in guides]
+    return {"name": name, "genome": {"seed": seed, "lens": lens, "line_width": 60, "n_frac": 0.02, "gap": (50, 3000),
+                                     "repeats": repeats, "iupac": 0.0, "plants": plants},
+            "mixed": [x[:5] for x in guides], "batch_nucl": batch_nucl}
+
+
 CASES = [
     _case("plain_nrg20_m4", 501, [300_000, 170_001], "NRG", True, 20, 4, 4, iupac=0.0005),
     _case("plain_tttn20_m3_w61", 502, [250_000, 90_000], "TTTN", False, 20, 4, 3, line_width=61),
@@ -56,7 +82,18 @@ CASES = [
     _case("bulge_tttn20_m2_d1r1", 505, [120_000, 50_000], "TTTN", False, 20, 2, 2, 1, 1, iupac=0.0005),
     _case("bulge_nrg20_m4_d2r2", 506, [40_000, 20_001], "NRG", True, 20, 2, 4, 2, 2, plants_per_guide=2),
     _case("bulge_nrg28_m5_d2r1_w33", 507, [90_000, 30_007], "NRG", True, 28, 2, 5, 2, 1),
+    _mixed_case("mixed_len17_30_4pams_m0_6_bulge20pct", 509, [200_000, 150_001, 65520 + 4000], 64, max_bulge_edits=7),
 ]
+
+
+def run_sets(case):
+    """(filter, M, D, R, patterns) per reference run of a case: one run, or one per set of a mixed case."""
+    if "mixed" not in case:
+        return [(case["filter"], case["M"], case["D"], case["R"], case["patterns"])]
+    sets = {}
+    for pat, filt, mm, d, r in case["mixed"]:
+        sets.setdefault((filt, mm, d, r), []).append(pat)
+    return [k + (v,) for k, v in sets.items()]
 
 
 def make_genome(spec):
@@ -66,9 +103,10 @@ def make_genome(spec):
                              iupac_frac=g["iupac"], plants=[tuple(p) for p in g["plants"]])
 
 
-def input_text(case, fasta):
-    line2 = case["filter"] + (f" {case['D']} {case['R']}" if case["D"] or case["R"] else "")
-    return f"{fasta}\n{line2}\n" + "".join(f"{p} {case['M']}\n" for p in case["patterns"])
+def input_text(run, fasta):
+    filt, M, D, R, pats = run
+    line2 = filt + (f" {D} {R}" if D or R else "")
+    return f"{fasta}\n{line2}\n" + "".join(f"{p} {M}\n" for p in pats)
 
 
 def main(out_dir=HERE):
@@ -79,8 +117,11 @@ def main(out_dir=HERE):
             fa = os.path.join(d, case["name"] + ".fa")
             make_genome(case["genome"]).write_fasta(fa)
             tsv = os.path.join(d, case["name"] + ".tsv")
-            n, _ = oracle.run(input_text(case, fa), tsv, batch_nucl=case["batch_nucl"])
-            rows = cofpy.read_rows(tsv)
+            rows = []
+            for run in run_sets(case):
+                oracle.run(input_text(run, fa), tsv, batch_nucl=case["batch_nucl"])
+                rows += cofpy.read_rows(tsv)
+            rows.sort()
             data = ("\n".join(rows) + "\n").encode() if rows else b""
             with open(os.path.join(out_dir, case["name"] + ".tsv.gz"), "wb") as f:
                 f.write(gzip.compress(data, compresslevel=9, mtime=0))
