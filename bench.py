@@ -533,7 +533,7 @@ def batched_workload(args):
             if si % 100 == 0:
                 log(f"plain schedule: run {si}/{len(sets)}, {t_sum:.2f} s so far")
             s.set_query([g[0] for g in gs], [key[0]] * len(gs), [key[1]] * len(gs), [key[2]] * len(gs),
-                        [key[3]] * len(gs), count_only=True)
+                        [key[3]] * len(gs), count_only=True, flags=schedule_flags(args))
             t0 = time.perf_counter()
             hp += s.search()
             t_sum += time.perf_counter() - t0

@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <tuple>
 
@@ -1123,13 +1125,20 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         }
         return nq;
     };
-    // synchronous rounds, halving a range whose queue overflowed
+    // synchronous rounds, halving a range whose queue overflowed.  With
+    // COF_LOG_ROUNDS set, each completed round is reported on stderr (tile
+    // range, candidates, time), so a search of many minutes shows progress.
+    static const bool log_rounds = getenv("COF_LOG_ROUNDS") != nullptr;
     auto run_sync = [&](size_t t0, size_t t1, uint32_t q_cap) {
         std::vector<std::pair<size_t, size_t>> stack{{t0, t1}};
         while (!stack.empty()) {
             auto [a0, a1] = stack.back();
             stack.pop_back();
+            const auto r0 = std::chrono::steady_clock::now();
             uint32_t nq = sync_round(a0, a1, q_cap);
+            if (log_rounds)
+                fprintf(stderr, "cof: bulge round tiles [%zu, %zu) of %zu: %u candidates, %.1f ms\n", a0, a1, (size_t)n_tiles,
+                        nq, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count());
             if (nq > q_cap) {
                 if (a1 - a0 == 1) throw Error(COF_E_NOMEM, "one filter tile produced more candidates than the queue holds");
                 if (stats) stats->n_redo_rounds++;
