@@ -973,16 +973,23 @@ void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uin
 // memory and skips a round whose queue overflowed; such rounds are redone
 // synchronously in halves (the reference bisects the same way,
 // search.rs:1140-1152).  `order` (optional) is the tile processing order.
-void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-               const uint32_t* d_genome, hipStream_t st, RawView<BulgeRaw>& out, cof_stats* stats,
-               const BulgeTune& tune) {
+namespace {
+
+// Myers columns (L + 1) up to which a pattern goes to the short 32-bit pass.
+constexpr uint32_t kShortCols = 24;
+
+// Adds src to dst (n words): per-tile candidate counts of two passes.
+__global__ void add_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+// The search for the plan's groups, all of one window class (32- or 64-bit
+// windows).  The first pass of a search clears the record count, sums and
+// per-tile step counts; a later pass adds to them.
+void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
+                const uint32_t* d_genome, hipStream_t st, cof_stats* stats, const BulgeTune& tune, bool first_pass) {
     const uint32_t flags = tune.flags;
-    out = RawView<BulgeRaw>{};
-    ws.tile_j0.clear();
-    ws.tile_n.clear();
-    ws.tile_chunk.clear();
-    if (!bp || bp->pats.empty()) return;
-    ws.ensure_events();
     std::vector<BTile> tiles;
     for (const Batch& b : batches) {
         uint64_t floor = g.chunks[b.first].nib_begin();
@@ -993,6 +1000,9 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     }
     if (tiles.empty()) return;
     const uint32_t n_tiles = (uint32_t)tiles.size();
+    ws.tile_j0.clear();
+    ws.tile_n.clear();
+    ws.tile_chunk.clear();
     ws.last_tiles = n_tiles;
     ws.tile_j0.resize(n_tiles);
     ws.tile_n.resize(n_tiles);
@@ -1042,9 +1052,11 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     HIPCHECK(hipMemcpyAsync(ws.tiles, tiles.data(), n_tiles * sizeof(BTile), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ws.tile_order, ord.data(), n_tiles * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemsetAsync(ws.tile_cand, 0, n_tiles * sizeof(uint32_t), st));
-    HIPCHECK(hipMemsetAsync(ws.tile_steps, 0, n_tiles * sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(ws.sums, 0, 8 * sizeof(unsigned long long), st));
-    HIPCHECK(hipMemsetAsync(ws.counters, 0, 4 * sizeof(uint32_t), st));
+    if (first_pass) {
+        HIPCHECK(hipMemsetAsync(ws.tile_steps, 0, n_tiles * sizeof(unsigned long long), st));
+        HIPCHECK(hipMemsetAsync(ws.sums, 0, 8 * sizeof(unsigned long long), st));
+        HIPCHECK(hipMemsetAsync(ws.counters, 0, 4 * sizeof(uint32_t), st));
+    }
     if (bp->count_only) {
         std::vector<DChunk> dc(g.chunks.size());
         for (size_t c = 0; c < g.chunks.size(); c++)
@@ -1309,6 +1321,79 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     }
     if (stats) {
         stats->bulge_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+}
+
+// The plan restricted to the groups of one window class whose Myers columns
+// (L + 1) lie in [min_cols, max_cols): the same pattern table, and the class's
+// device group table from the first kept group on.  Groups are ordered by L
+// first (bulge_plan_create), so the kept groups are one contiguous range of it.
+BulgePlan groups_view(const BulgePlan& bp, bool wide, uint32_t min_cols, uint32_t max_cols) {
+    BulgePlan v;
+    v.d_pats = bp.d_pats;
+    v.count_only = bp.count_only;
+    const std::vector<BGroup>& all = wide ? bp.groups64 : bp.groups32;
+    std::vector<BGroup>& kept = wide ? v.groups64 : v.groups32;
+    size_t first = all.size();
+    for (size_t k = 0; k < all.size(); k++) {
+        const BGroup& G = all[k];
+        if (G.L + 1 < min_cols || G.L + 1 > max_cols) continue;
+        if (kept.empty()) first = k;
+        else if (k != first + kept.size()) throw Error(COF_E_HIP, "bulge pattern classes are not contiguous");
+        kept.push_back(G);
+        v.n_cols = std::max(v.n_cols, G.L + 1);
+    }
+    if (!kept.empty()) (wide ? v.d_g64 : v.d_g32) = (wide ? bp.d_g64 : bp.d_g32) + first;
+    return v;
+}
+
+}  // namespace
+
+// Runs the filter over every virtual batch's end range and the traceback over
+// the queued candidates (see bulge_pass), once per class of patterns present:
+// 64-bit windows (L + D > 32), 32-bit windows with more than kShortCols
+// Myers columns, and the rest.  A query mixing them runs each class with its
+// own word and with LDS for its own longest pattern, rather than every
+// candidate with the widest word and the longest pattern's columns (which
+// cuts the traceback's resident workgroups per CU).
+void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
+               const uint32_t* d_genome, hipStream_t st, RawView<BulgeRaw>& out, cof_stats* stats,
+               const BulgeTune& tune) {
+    out = RawView<BulgeRaw>{};
+    ws.tile_j0.clear();
+    ws.tile_n.clear();
+    ws.tile_chunk.clear();
+    if (!bp || bp->pats.empty()) return;
+    ws.ensure_events();
+    std::vector<BulgePlan> views;
+    for (auto [wide, lo, hi] : {std::make_tuple(false, 0u, kShortCols), std::make_tuple(false, kShortCols + 1, 0xFFFFu),
+                                std::make_tuple(true, 0u, 0xFFFFu)}) {
+        BulgePlan v = groups_view(*bp, wide, lo, hi);
+        if (!v.groups32.empty() || !v.groups64.empty()) views.push_back(std::move(v));
+    }
+    if (views.size() == 1) {
+        bulge_pass(bp, ws, g, batches, d_genome, st, stats, tune, true);
+    } else {
+        // per-tile candidate counts add up over the passes (each pass clears
+        // and refills them)
+        for (size_t k = 0; k < views.size(); k++) {
+            bulge_pass(&views[k], ws, g, batches, d_genome, st, stats, tune, k == 0);
+            const uint32_t nt = ws.last_tiles;
+            if (!nt) continue;
+            if (k == 0) {
+                ws_grow((void**)&ws.tile_cand_prev, &ws.tile_cand_prev_cap, nt, sizeof(uint32_t));
+                HIPCHECK(hipMemcpyAsync(ws.tile_cand_prev, ws.tile_cand, nt * sizeof(uint32_t),
+                                        hipMemcpyDeviceToDevice, st));
+            } else {
+                hipLaunchKernelGGL(add_u32_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, ws.tile_cand_prev,
+                                   (const uint32_t*)ws.tile_cand, nt);
+                HIPCHECK(hipGetLastError());
+            }
+            if (k + 1 == views.size())
+                HIPCHECK(hipMemcpyAsync(ws.tile_cand, ws.tile_cand_prev, nt * sizeof(uint32_t),
+                                        hipMemcpyDeviceToDevice, st));
+        }
+        HIPCHECK(hipStreamSynchronize(st));
     }
     uint32_t err = 0, no = 0;
     HIPCHECK(hipMemcpy(&err, ws.counters + 3, sizeof err, hipMemcpyDeviceToHost));

@@ -58,6 +58,8 @@ struct Workspace {
     size_t tile_order_cap = 0;
     uint32_t* tile_cand = nullptr;             // candidates per tile (last search)
     size_t tile_cand_cap = 0;
+    uint32_t* tile_cand_prev = nullptr;        // the first window class's counts while the second runs
+    size_t tile_cand_prev_cap = 0;
     unsigned long long* tile_steps = nullptr;  // DFS transitions per tile (last search)
     size_t tile_steps_cap = 0;
     uint32_t last_tiles = 0;

@@ -1005,7 +1005,7 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
 Workspace::~Workspace() {
     for (void* p : {plain_out, plain_out2, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
                     qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
-                    (void*)tile_order, (void*)tile_cand, (void*)tile_steps, (void*)rounds,
+                    (void*)tile_order, (void*)tile_cand, (void*)tile_cand_prev, (void*)tile_steps, (void*)rounds,
                     (void*)round_ctr, (void*)img_hi, (void*)img_lo, (void*)img_flags})
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
