@@ -154,6 +154,20 @@ __device__ __forceinline__ void tile_sync() {
     }
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active): DPP
+// row shifts by 1, 2, 4 and 8 sum within each row of 16 lanes (a lane without
+// a source adds 0), then row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1
+// and 3, and row_bcast:31 adds lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return v;
+}
+
 // bit 0 of nibble j -> bit j (other bits zero)
 __device__ __forceinline__ uint32_t gather_nibble_bit0(uint32_t m) {
     m = (m | (m >> 3)) & 0x03030303u;
@@ -231,11 +245,29 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
         const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
         bf[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
     }
+    // The fragments arrive here (an empty asm that reads them): otherwise the
+    // loop waits for them on every iteration, and so also for the hit records
+    // stored by the previous one (stores count in vmcnt).
+#pragma unroll
+    for (int b = 0; b < NB; b++) asm volatile("" ::"v"(bf[b]));
     // the list is padded to whole blocks; rows past the list are dropped in
     // the hit path
-    for (uint32_t cb = wave * 16u; cb < nc; cb += 16u * (THREADS / 64)) {
-        const uint32_t p = (uint32_t)cand[cb + r] + start;
-        const uint32_t B = __builtin_amdgcn_alignbit(gw[(p >> 3) + 1], gw[p >> 3], p << 2);
+    // Software pipelined over the blocks of 16 list entries: an iteration
+    // reads the bit4 words of the next block and the list entry of the one
+    // after it, so no LDS latency lies between the compares of two blocks.
+    // Reads past the list take entry cb instead (their values go unused).
+    constexpr uint32_t kStep = 16u * (THREADS / 64);
+    uint32_t cb = wave * 16u;
+    if (cb >= nc) return;
+    uint32_t p1 = (uint32_t)cand[cb + r] + start;   // start of this block's window
+    uint32_t x0 = gw[p1 >> 3], x1 = gw[(p1 >> 3) + 1];
+    uint32_t p2 = (uint32_t)cand[(cb + kStep < nc ? cb + kStep : cb) + r] + start;
+    for (; cb < nc; cb += kStep) {
+        const uint32_t B = __builtin_amdgcn_alignbit(x1, x0, p1 << 2);
+        p1 = p2;
+        x0 = gw[p1 >> 3];
+        x1 = gw[(p1 >> 3) + 1];
+        p2 = (uint32_t)cand[(cb + 2u * kStep < nc ? cb + 2u * kStep : cb) + r] + start;
         const uint32_t n = __builtin_amdgcn_ubfe(B, xsh, 4u);
         const mf_v4i a = {(int32_t)(B & 0x01010101u), (int32_t)(B & 0x02020202u), (int32_t)(B & 0x04040404u),
                           (int32_t)(((n * 0x010101u) & 0x040201u) | bias_one)};
@@ -477,67 +509,81 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 m8[w] = m;
             }
         }
-        // block-wide exclusive scan of per-thread counts (each <= 64): within
-        // the wave bit by bit, a ballot per bit and its lane count below
-        // (mbcnt), so no cross-lane permute waits on LDS; nc = the total
-        auto block_scan = [&](uint32_t cnt, uint32_t& nc, uint32_t& pos) {
-            uint32_t excl = 0, wtot = 0;
-#pragma unroll
-            for (int b = 0; b < 7; b++) {
-                const uint64_t bb = __ballot((cnt >> b) & 1u);
-                excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bb, 0u)) << b;
-                wtot += (uint32_t)__popcll(bb) << b;
-            }
+        // block-wide exclusive scan of per-thread values: a DPP prefix sum in
+        // the wave, the wave totals through LDS; tot = the block total.  The
+        // sums stay below 2^16, so a value may pack two counts.
+        auto block_scan = [&](uint32_t v, uint32_t& tot, uint32_t& excl) {
+            const uint32_t inc = wave_prefix_sum(v);
+            const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
             uint32_t off = 0;
-            nc = wtot;
+            tot = wtot;
             if (kThreads > 64) {
                 if (lane == 0) wave_sum[wave] = wtot;
                 __syncthreads();
-                nc = 0;
+                tot = 0;
 #pragma unroll
                 for (int x = 0; x < kThreads / 64; x++) {
                     uint32_t ws = wave_sum[x];
                     off += (uint32_t)x < wave ? ws : 0u;
-                    nc += ws;
+                    tot += ws;
                 }
                 __syncthreads();   // wave_sum is written again by the next scan
             }
-            pos = off + excl;
+            excl = off + inc - v;
         };
-        uint32_t cnt_all = 0;
-#pragma unroll
-        for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
-        uint32_t nc_all, pos_all;
-        block_scan(cnt_all, nc_all, pos_all);
-        // the window list holds half a tile: the tile's windows are compared
+        // The window list holds half a tile: the tile's windows are compared
         // in one pass when they fit, else in two passes over the halves of
-        // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3)
+        // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3).
+        // Plain-base tiles list their windows in start order (plane word 0 of
+        // every thread, then word 1), so 16 consecutive entries lie in a few
+        // consecutive LDS words: one scan of both words' counts, packed.
+        uint32_t nc_all, pos_all = 0, tot0 = 0, ex0 = 0, ex1 = 0;
+        if (plain_tile) {
+            uint32_t tot, ex;
+            block_scan((uint32_t)__popc(m8[0]) | ((uint32_t)__popc(m8[1]) << 16), tot, ex);
+            tot0 = tot & 0xFFFFu;
+            nc_all = tot0 + (tot >> 16);
+            ex0 = ex & 0xFFFFu;
+            ex1 = ex >> 16;
+        } else {
+            uint32_t cnt_all = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
+            block_scan(cnt_all, nc_all, pos_all);
+        }
         const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
         for (uint32_t h = 0; h < npass; h++) {
-            uint32_t mh[8];
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                const uint32_t half = plain_tile ? (uint32_t)w : (uint32_t)w >> 2;
-                mh[w] = (npass == 1 || half == h) ? m8[w] : 0u;
-            }
-            uint32_t nc = nc_all, pos = pos_all;
-            if (npass == 2) {
-                uint32_t cnt = 0;
-#pragma unroll
-                for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
-                block_scan(cnt, nc, pos);
-            }
+            uint32_t nc;
             if (plain_tile) {
-#pragma unroll
-                for (int w = 0; w < 2; w++) {
-                    uint32_t m = mh[w];
-                    while (m) {
-                        uint32_t j = (uint32_t)__builtin_ctz(m);
-                        cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 32u + j);
-                        m &= m - 1u;
-                    }
+                // word 0's windows from 0, word 1's after them (one pass) or
+                // from 0 (second pass)
+                nc = npass == 1 ? nc_all : h == 0 ? tot0 : nc_all - tot0;
+                uint32_t m = (npass == 1 || h == 0) ? m8[0] : 0u;
+                uint32_t pos = ex0;
+                while (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    cand[pos++] = (uint16_t)(tid * 32u + j);
+                    m &= m - 1u;
+                }
+                m = (npass == 1 || h == 1) ? m8[1] : 0u;
+                pos = (npass == 1 ? tot0 : 0u) + ex1;
+                while (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    cand[pos++] = (uint16_t)(((uint32_t)kThreads + tid) * 32u + j);
+                    m &= m - 1u;
                 }
             } else {
+                uint32_t mh[8];
+#pragma unroll
+                for (int w = 0; w < 8; w++) mh[w] = (npass == 1 || ((uint32_t)w >> 2) == h) ? m8[w] : 0u;
+                uint32_t pos = pos_all;
+                nc = nc_all;
+                if (npass == 2) {
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
+                    block_scan(cnt, nc, pos);
+                }
 #pragma unroll
                 for (int w = 0; w < 8; w++) {
                     uint32_t m = mh[w];
