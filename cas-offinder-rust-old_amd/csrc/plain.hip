@@ -11,12 +11,13 @@
 //
 // Design (one workgroup = one tile of 64 x THREADS consecutive start
 // positions, optionally one slice of a group's patterns per grid row):
-//   1. the tile's genome words (+ halo) are loaded with 16-byte loads, all
-//      issued before any is used, and stored into LDS, N-masked (0xF -> 0,
-//      search.rs:512-531) on tiles that hold zero nibbles.  The genome's scan
-//      image (built once per loaded genome) gives the tile's flags (IUPAC
-//      genome letters, zero nibbles) and, for tiles of plain bases only, its
-//      two 32-position bit planes (base bit 1, base bit 0);
+//   1. the genome's scan image (built once per loaded genome) gives the
+//      tile's flags (IUPAC genome letters, zero nibbles) and its two
+//      32-position bit planes (base bit 1, base bit 0), which are all that
+//      tiles of plain bases need.  Flagged tiles (and every tile of a launch
+//      with a group that compares bit4 words) also load their bit4 words
+//      (+ halo) with 16-byte loads into LDS, N-masked (0xF -> 0,
+//      search.rs:512-531) where the tile holds zero nibbles;
 //   2. per pattern group (patterns sharing one effective filter), every thread
 //      tests the PAM of its 64 starts and the passing starts are compacted
 //      into an LDS list of half a tile by a block-wide scan (two passes over
@@ -24,13 +25,14 @@
 //      cell on 32 starts with three v_bitop3 selects over the planes and an
 //      AND; other tiles test 8 starts per word (SWAR over nibbles);
 //   3. groups whose patterns are plain bases on every position that is not N
-//      in all of them score plain-base tiles' windows on the matrix cores (16
-//      windows x 16 patterns per v_mfma_i32_16x16x64_i8, see mfma_chunk), or,
-//      for windows wider than 20 positions where that filter would pass too
-//      many pairs, compare the planes (xor + bitop3 + v_bcnt per pattern, 16
-//      patterns per batch of scalar loads).  Other tiles and groups extract
-//      each window with v_alignbit funnel shifts, AND the bit4 words and
-//      v_bcnt them (W words per pattern);
+//      in all of them (windows of at most 32 positions) score plain-base
+//      tiles' windows on the matrix cores: 16 windows x 16 patterns per fp4
+//      v_mfma_f32_16x16x128_f8f6f4 built from the planes, an exact
+//      mismatch count (see mfma_chunk); with pattern slices that are not
+//      whole blocks they compare the planes (xor + bitop3 + v_bcnt per
+//      pattern, 16 patterns per batch of scalar loads).  Other tiles and
+//      groups extract each window with v_alignbit funnel shifts, AND the bit4
+//      words and v_bcnt them (W words per pattern);
 //   4. hits (with the writer's mismatch recount) go to one of 64 record
 //      segments, each with its own counter.
 // Without IUPAC letters in the window, genome single bases make the writer's
@@ -90,14 +92,14 @@ struct GroupDev {
 
 // One pattern as two bit planes over the group's plane window (base T C A G =
 // planes (0,0) (0,1) (1,0) (1,1), positions outside the window 0) and
-// K = 64 - M: a window hits when popcount(mismatch) + K <= 64.  A group's
-// records are padded to a multiple of 16 with records that never hit
-// (K = 200), so the compare runs in whole batches of 16 or 8; `plan` is the
-// pattern's index in the plan (its bit4 record and full words).
+// K = 64 - M: a window hits when popcount(mismatch) + K <= 64, and that
+// popcount is the writer's recount.  A group's records are padded to a
+// multiple of 16 with records that never hit (K = 200), so the compare runs in
+// whole batches of 16 or 8; `pid` is the query pattern index.
 struct __attribute__((aligned(16))) PlaneRec {
     uint32_t ph, pl;
     uint32_t K;
-    uint32_t plan;
+    uint32_t pid;
 };
 
 // One pattern of a group: its compare window (the pattern's nibbles from
@@ -200,112 +202,117 @@ __device__ __forceinline__ uint32_t recount_swar(const uint32_t* gw, uint32_t i,
     return amm;
 }
 
-// Matrix-core filter of plain-base windows (groups whose patterns are plain
-// bases on the window; tiles of plain bases only).  Mismatches on window
-// positions 0-19 are a product of 0/1 genome features and signed pattern
-// weights: per position with genome nibble n (T C A G = 1 2 4 8) the
-// features are the bytes n & 1, n & 2, n & 4 (one AND of a bit4 word gives
-// four of them), and a pattern base b weighs them T (4,0,0), C (0,2,0),
-// A (0,0,1), G (-4,-2,-1), so feature . weight = 4 [n == b] - 4 [b == G].
-// With a bias feature (genome 1, pattern 4 (#G - Lw + M), #G and Lw counted
-// over window positions 0-19) the score of a window and a pattern is
-// 4 (M - mismatches on positions 0-19): >= 0 for every hit, and exactly the
-// hits when the window has at most 20 positions.  Windows that score >= 0
-// are checked on the 2-bit planes before they are recorded.
+// Matrix-core score of plain-base windows (groups whose patterns are plain
+// bases on every position of a window of at most 32 positions; tiles of plain
+// bases only).  Per window position with planes (h, l) (T C A G = (0,0)
+// (0,1) (1,0) (1,1)) and pattern base b, [genome == b] is linear in the
+// features h, l and h AND l: T = 1 - h - l + hl, C = l - hl, A = h - hl,
+// G = hl.  Features are fp4 (e2m1) 0.5 where the bit is set (nibble 0x1),
+// weights are -1, 0 or +1 (nibbles 0xA, 0x0, 0x2), so a window scores
+// 0.5 (matches - #T) over the window, and 32 bias slots (feature 0.5, fp4
+// weights chosen on the host) add 0.5 (#T - Lw + M) + 0.25: the score is
+// 0.5 (M - mismatches) + 0.25, exact in fp32, never zero, positive iff the
+// window hits, and the hit's mismatch count (the writer's recount on a tile
+// of plain bases) is M + 0.5 - 2 score.
 //
-// One v_mfma_i32_16x16x64_i8 scores 16 windows (rows) against 16 patterns
-// (columns) over 64 K bytes.  Lane l holds row/column l % 16 and K bytes
-// 16 (l / 16) .. +15 of both operands; output v of lane l is row
-// 4 (l / 16) + v, column l % 16.  Lane group c = l / 16 reads one bit4 word
-// at window position s_c = 0, 3, 10, 13: bytes 0-11 are the features of its
-// even nibbles (positions s_c + 2e, e = 0..3; three ANDs), bytes 12-14 those
-// of nibble x_c = 1, 5, 1, 5 (position s_c + x_c) and byte 15 is the bias
-// (c = 3), so the four groups cover positions 0-19 once each.  Pattern
-// fragments (fillers: weights 0, bias -128) are built on the host, 16 bytes
+// One v_mfma_f32_16x16x128_f8f6f4 (fp4 x fp4) scores 16
+// windows (rows) against 16 patterns (columns) over 128 K nibbles.  Lane l
+// holds row/column l % 16 and K nibbles 32 (l / 16) + 8 d + j (dword d,
+// nibble j) of both operands; output v of lane l is row 4 (l / 16) + v,
+// column l % 16.  Lane group c = l / 16 takes window positions c + 4 j:
+// dwords 0-2 are the h, l and hl features ((plane >> c) & 0x11111111: one
+// AND each after one funnel shift), dword 3 the bias feature.  Pattern
+// fragments (fillers: weights 0, bias -96) are built on the host, 16 bytes
 // per lane per block of 16 patterns.
 typedef int32_t mf_v4i __attribute__((ext_vector_type(4)));
+typedef int32_t mf_v8i __attribute__((ext_vector_type(8)));
+typedef float mf_v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf_v4f mfma_fp4(const mf_v8i& a, const mf_v8i& b) {
+    // formats 4/4: fp4 e2m1 operands (4 dwords each).  With zero scale
+    // operands the compiler emits the unscaled v_mfma_f32_16x16x128_f8f6f4
+    // (no scale registers, and the i8 MFMA's result latency; the scaled form
+    // with scales of 2^0 took ~12 cycles longer per dependent use).
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, mf_v4f{0.f, 0.f, 0.f, 0.f}, 4, 4, 0, 0, 0, 0);
+}
 
 // One chunk of NB pattern blocks (held in registers) against every window of
-// the list: per 16 windows, NB independent MFMAs and an AND of their scores;
-// the hit path runs only when some score is >= 0.
+// the list: per 16 windows, NB independent MFMAs and an AND of their score
+// signs; the hit path runs only when some score is positive.
 template <int NB, int THREADS>
-__device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* ghi, const uint32_t* glo,
-                                           const uint16_t* cand, uint32_t nc, const GroupDev& G, uint32_t bc,
+__device__ __forceinline__ void mfma_chunk(const uint32_t* ghi, const uint32_t* glo, const uint16_t* cand,
+                                           uint32_t nc, const GroupDev& G, uint32_t bc,
                                            const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
-                                           const PatRec* __restrict__ pats, const uint32_t* __restrict__ full_words,
                                            uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
                                            uint32_t lane, uint32_t wave) {
     const uint32_t lg = lane >> 4, r = lane & 15u;
-    const uint32_t start = G.pk_lo + (lg == 0 ? 0u : lg == 1 ? 3u : lg == 2 ? 10u : 13u);
-    const uint32_t xsh = (lg & 1u) ? 20u : 4u;   // nibble x_c of the word
-    const uint32_t bias_one = lg == 3 ? 0x01000000u : 0u;
-    mf_v4i bf[NB];
+    const uint32_t off = G.pk_lo + lg;   // plane position of this lane group's first window position
+    mf_v4i bv[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
-        bf[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
+        bv[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
     }
     // The fragments arrive here (an empty asm that reads them): otherwise the
     // loop waits for them on every iteration, and so also for the hit records
     // stored by the previous one (stores count in vmcnt).
 #pragma unroll
-    for (int b = 0; b < NB; b++) asm volatile("" ::"v"(bf[b]));
+    for (int b = 0; b < NB; b++) asm volatile("" ::"v"(bv[b]));
     // the list is padded to whole blocks; rows past the list are dropped in
     // the hit path
     // Software pipelined over the blocks of 16 list entries: an iteration
-    // reads the bit4 words of the next block and the list entry of the one
+    // reads the plane words of the next block and the list entry of the one
     // after it, so no LDS latency lies between the compares of two blocks.
     // Reads past the list take entry cb instead (their values go unused).
     constexpr uint32_t kStep = 16u * (THREADS / 64);
     uint32_t cb = wave * 16u;
     if (cb >= nc) return;
-    uint32_t p1 = (uint32_t)cand[cb + r] + start;   // start of this block's window
-    uint32_t x0 = gw[p1 >> 3], x1 = gw[(p1 >> 3) + 1];
-    uint32_t p2 = (uint32_t)cand[(cb + kStep < nc ? cb + kStep : cb) + r] + start;
+    uint32_t q1 = (uint32_t)cand[cb + r] + off;   // plane position of this block's window
+    uint32_t h0 = ghi[q1 >> 5], h1 = ghi[(q1 >> 5) + 1], l0 = glo[q1 >> 5], l1 = glo[(q1 >> 5) + 1];
+    uint32_t q2 = (uint32_t)cand[(cb + kStep < nc ? cb + kStep : cb) + r] + off;
     for (; cb < nc; cb += kStep) {
-        const uint32_t B = __builtin_amdgcn_alignbit(x1, x0, p1 << 2);
-        p1 = p2;
-        x0 = gw[p1 >> 3];
-        x1 = gw[(p1 >> 3) + 1];
-        p2 = (uint32_t)cand[(cb + 2u * kStep < nc ? cb + 2u * kStep : cb) + r] + start;
-        const uint32_t n = __builtin_amdgcn_ubfe(B, xsh, 4u);
-        const mf_v4i a = {(int32_t)(B & 0x01010101u), (int32_t)(B & 0x02020202u), (int32_t)(B & 0x04040404u),
-                          (int32_t)(((n * 0x010101u) & 0x040201u) | bias_one)};
-        // sign bit of the AND of all scores: clear iff some score >= 0
+        const uint32_t wh = __builtin_amdgcn_alignbit(h1, h0, q1 & 31u);
+        const uint32_t wl = __builtin_amdgcn_alignbit(l1, l0, q1 & 31u);
+        q1 = q2;
+        h0 = ghi[q1 >> 5];
+        h1 = ghi[(q1 >> 5) + 1];
+        l0 = glo[q1 >> 5];
+        l1 = glo[(q1 >> 5) + 1];
+        q2 = (uint32_t)cand[(cb + 2u * kStep < nc ? cb + 2u * kStep : cb) + r] + off;
+        const mf_v4i af = {(int32_t)(wh & 0x11111111u), (int32_t)(wl & 0x11111111u),
+                           (int32_t)__builtin_amdgcn_bitop3_b32(wh, wl, 0x11111111u, 0x80), 0x11111111};
+        const mf_v8i a = {af.x, af.y, af.z, af.w, 0, 0, 0, 0};
+        // sign bit of the AND of all scores: clear iff some score is positive
         uint32_t red = 0xFFFFFFFFu;
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const mf_v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
-            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[0], (uint32_t)d[1], 0x80);
-            red = __builtin_amdgcn_bitop3_b32(red, (uint32_t)d[2], (uint32_t)d[3], 0x80);
+            const mf_v8i bf = {bv[b].x, bv[b].y, bv[b].z, bv[b].w, 0, 0, 0, 0};
+            const mf_v4f d = mfma_fp4(a, bf);
+            red = __builtin_amdgcn_bitop3_b32(red, __float_as_uint(d[0]), __float_as_uint(d[1]), 0x80);
+            red = __builtin_amdgcn_bitop3_b32(red, __float_as_uint(d[2]), __float_as_uint(d[3]), 0x80);
         }
         if (!__ballot((int32_t)red >= 0)) continue;
         // window row 4 lg + v against pattern column r of block bc + b (scores
         // computed again, so they need no registers in the common case; blocks
-        // without a passing score are skipped as a whole): on windows wider
-        // than 20 positions the plane compare decides, then the hit is recorded
-        const bool exact = (G.wmask >> 20) == 0u;   // uniform
+        // without a positive score are skipped as a whole).  The features pass
+        // through an empty asm so the compiler cannot reuse the scores above
+        // (and keep all of them live across the loop).
+        mf_v4i af2 = af;
+        asm volatile("" : "+v"(af2));
+        const mf_v8i a2 = {af2.x, af2.y, af2.z, af2.w, 0, 0, 0, 0};
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const mf_v4i d = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bf[b], mf_v4i{0, 0, 0, 0}, 0, 0, 0);
-            if (!__ballot(max(max(d[0], d[1]), max(d[2], d[3])) >= 0)) continue;
+            const mf_v8i bf = {bv[b].x, bv[b].y, bv[b].z, bv[b].w, 0, 0, 0, 0};
+            const mf_v4f d = mfma_fp4(a2, bf);
+            if (!__ballot(max(max(d[0], d[1]), max(d[2], d[3])) > 0.f)) continue;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const uint32_t c = cb + 4u * lg + (uint32_t)v;
-                if (d[v] < 0 || c >= nc) continue;
+                if (!(d[v] > 0.f) || c >= nc) continue;
                 const PlaneRec& rec = prec[G.prec_begin + (bc + b) * 16u + r];
-                const uint32_t i = cand[c];
-                if (!exact) {
-                    const uint32_t q = i + G.pk_lo;
-                    const uint32_t gh =
-                        __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
-                    const uint32_t gl =
-                        __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
-                    if ((uint32_t)__popc((gh ^ rec.ph) | (gl ^ rec.pl)) + rec.K > 64u) continue;
-                }
-                const uint32_t amm = recount_swar(gw, i, full_words + (size_t)rec.plan * 8, G.L);
+                const uint32_t amm = (uint32_t)((float)(64 - (int32_t)rec.K) + 0.5f - 2.f * d[v]);
                 const uint32_t idx = atomicAdd(out_count, 1u);
-                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[rec.plan].pid, amm};
+                if (idx < out_cap) out[idx] = PlainRaw{s0 + cand[c], rec.pid, amm};
             }
         }
     }
@@ -313,20 +320,18 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* gw, const uint32_t* g
 
 // The group's pattern blocks [pb, pe) in chunks of up to 8 blocks.
 template <int THREADS>
-__device__ __forceinline__ void mfma_compare(const uint32_t* gw, const uint32_t* ghi, const uint32_t* glo,
-                                             const uint16_t* cand, uint32_t nc, const GroupDev& G, uint32_t pb,
-                                             uint32_t pe, const uint4* __restrict__ mfrag,
-                                             const PlaneRec* __restrict__ prec, const PatRec* __restrict__ pats,
-                                             const uint32_t* __restrict__ full_words, uint64_t s0, PlainRaw* out,
-                                             uint32_t* out_count, uint32_t out_cap, uint32_t lane, uint32_t wave) {
+__device__ __forceinline__ void mfma_compare(const uint32_t* ghi, const uint32_t* glo, const uint16_t* cand,
+                                             uint32_t nc, const GroupDev& G, uint32_t pb, uint32_t pe,
+                                             const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
+                                             uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
+                                             uint32_t lane, uint32_t wave) {
     const uint32_t blk_lo = (pb - G.pat_begin) >> 4;
     const uint32_t blk_hi = (pe - G.pat_begin + 15u) >> 4;
     for (uint32_t bc = blk_lo; bc < blk_hi; bc += 8) {
         const uint32_t nb = min(8u, blk_hi - bc);   // uniform
-#define COF_MFMA_CHUNK(N)                                                                                        \
-    case N:                                                                                                      \
-        mfma_chunk<N, THREADS>(gw, ghi, glo, cand, nc, G, bc, mfrag, prec, pats, full_words, s0, out, out_count, \
-                               out_cap, lane, wave);                                                             \
+#define COF_MFMA_CHUNK(N)                                                                                         \
+    case N:                                                                                                       \
+        mfma_chunk<N, THREADS>(ghi, glo, cand, nc, G, bc, mfrag, prec, s0, out, out_count, out_cap, lane, wave); \
         break;
         switch (nb) {
             COF_MFMA_CHUNK(1) COF_MFMA_CHUNK(2) COF_MFMA_CHUNK(3) COF_MFMA_CHUNK(4)
@@ -348,7 +353,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t* __restrict__ img_hi, const uint32_t* __restrict__ img_lo,   // scan image planes
     const uint8_t* __restrict__ img_flags,    // scan image flags per 4096-start unit
     PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
-    uint32_t pat_chunk) {                     // patterns per grid row (0: whole group)
+    uint32_t pat_chunk,                       // patterns per grid row (0: whole group)
+    uint32_t bit4_all) {                      // a group compares bit4 words: every tile loads them
     constexpr int kTile = THREADS * 64;
     constexpr int kTileWords = kTile / 8 + kHaloWords;
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
@@ -377,53 +383,56 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
     const uint32_t out_cap = seg_cap;
 
-    // 1. genome tile -> LDS.  The genome's scan image gives the tile's flags
-    //    (IUPAC letters; zero nibbles: masked N, padding) as the OR of its
-    //    4096-start units, and its two 32-position bit planes (base bit 1 =
-    //    A/G, base bit 0 = C/G), which serve tiles of plain bases only.  The
-    //    bit4 words are N-masked (0xF -> 0, search.rs:512-531) where the tile
-    //    has zero nibbles; a tile without them holds no N.  Every load of a
-    //    thread is issued before any is used.
+    // 1. genome tile -> LDS.  The genome's scan image gives the tile's two
+    //    32-position bit planes (base bit 1 = A/G, base bit 0 = C/G), which
+    //    serve tiles of plain bases only, and its flags (IUPAC letters; zero
+    //    nibbles: masked N, padding) as the OR of its 4096-start units.  The
+    //    plane loads are issued before the flags arrive.  Flagged tiles, and
+    //    every tile when a group of the launch compares bit4 words
+    //    (bit4_all), also load their bit4 words, N-masked (0xF -> 0,
+    //    search.rs:512-531) where the tile has zero nibbles; a tile without
+    //    them holds no N.  Every load of a thread is issued before any is
+    //    used.
+    constexpr int kPlaneIt = (kPlaneWords + kThreads - 1) / kThreads;
+    uint32_t ph[kPlaneIt], pl[kPlaneIt];
+#pragma unroll
+    for (int it = 0; it < kPlaneIt; it++) {
+        const uint32_t j = tid + (uint32_t)it * kThreads;
+        const bool in = j < (uint32_t)kPlaneWords;
+        ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
+        pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
+    }
     int flags = 0;
 #pragma unroll
     for (int u = 0; u < kThreads / 64; u++) flags |= img_flags[(s0 >> 12) + u];   // uniform
-    constexpr int kGroups4 = (kTileWords + 3) / 4;
-    constexpr int kLoadIt = (kGroups4 + kThreads - 1) / kThreads;
-    constexpr int kPlaneIt = (kPlaneWords + kThreads - 1) / kThreads;
-    const uint4* const g4 = reinterpret_cast<const uint4*>(genome + w0);
-    uint4 lv[kLoadIt];
+    if (bit4_all || flags) {
+        constexpr int kGroups4 = (kTileWords + 3) / 4;
+        constexpr int kLoadIt = (kGroups4 + kThreads - 1) / kThreads;
+        const uint4* const g4 = reinterpret_cast<const uint4*>(genome + w0);
+        uint4 lv[kLoadIt];
 #pragma unroll
-    for (int it = 0; it < kLoadIt; it++) {
-        const uint32_t j = tid + (uint32_t)it * kThreads;
-        lv[it] = j < (uint32_t)kGroups4 ? g4[j] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    uint32_t ph[kPlaneIt], pl[kPlaneIt];
-    if (flags == 0) {
-#pragma unroll
-        for (int it = 0; it < kPlaneIt; it++) {
+        for (int it = 0; it < kLoadIt; it++) {
             const uint32_t j = tid + (uint32_t)it * kThreads;
-            const bool in = j < (uint32_t)kPlaneWords;
-            ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
-            pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
+            lv[it] = j < (uint32_t)kGroups4 ? g4[j] : make_uint4(0u, 0u, 0u, 0u);
         }
-    }
 #pragma unroll
-    for (int it = 0; it < kLoadIt; it++) {
-        const uint32_t j = tid + (uint32_t)it * kThreads;
-        uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
-        if (flags) {
+        for (int it = 0; it < kLoadIt; it++) {
+            const uint32_t j = tid + (uint32_t)it * kThreads;
+            uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
+            if (flags) {
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint32_t f = raw[b] & (raw[b] >> 1) & (raw[b] >> 2) & (raw[b] >> 3) & 0x11111111u;   // nibble == 0xF
-                raw[b] &= ~((f << 4) - f);
+                for (int b = 0; b < 4; b++) {
+                    const uint32_t f = raw[b] & (raw[b] >> 1) & (raw[b] >> 2) & (raw[b] >> 3) & 0x11111111u;   // nibble == 0xF
+                    raw[b] &= ~((f << 4) - f);
+                }
             }
-        }
-        if (4 * j + 3 < (uint32_t)kTileWords) {   // a whole 16-byte group: one LDS store
-            *reinterpret_cast<uint4*>(&gw[4 * j]) = make_uint4(raw[0], raw[1], raw[2], raw[3]);
-        } else {
+            if (4 * j + 3 < (uint32_t)kTileWords) {   // a whole 16-byte group: one LDS store
+                *reinterpret_cast<uint4*>(&gw[4 * j]) = make_uint4(raw[0], raw[1], raw[2], raw[3]);
+            } else {
 #pragma unroll
-            for (int b = 0; b < 4; b++)
-                if (4 * j + b < (uint32_t)kTileWords) gw[4 * j + b] = raw[b];
+                for (int b = 0; b < 4; b++)
+                    if (4 * j + b < (uint32_t)kTileWords) gw[4 * j + b] = raw[b];
+            }
         }
     }
     if (flags == 0) {
@@ -602,8 +611,8 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
             tile_sync<kThreads>();
             // 3. compare windows against the group's patterns
             if (mfma) {
-                mfma_compare<kThreads>(gw, ghi, glo, cand, nc, G, pb, pe, mfrag, prec, pats, full_words, s0, out,
-                                       out_count, out_cap, lane, wave);
+                mfma_compare<kThreads>(ghi, glo, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
+                                       wave);
                 tile_sync<kThreads>();
                 continue;
             }
@@ -682,12 +691,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                         }
                         if (__ballot(mn <= 64u) & act_mask) {
                             for (int u = 0; u < NB; u++) {
-                                const uint32_t d = (gh ^ rp[u].ph) | (gl ^ rp[u].pl);
-                                if (!(act && (uint32_t)__popc(d) + rp[u].K <= 64u)) continue;
-                                const uint32_t gp = rp[u].plan;
-                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)gp * 8, G.L);
+                                const uint32_t amm = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl));
+                                if (!(act && amm + rp[u].K <= 64u)) continue;
                                 uint32_t idx = atomicAdd(out_count, 1u);
-                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[gp].pid, amm};
+                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
                             }
                         }
                         p += NB;
@@ -700,11 +707,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                     if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
                     for (; p < qe; p++) {
                         const PlaneRec& r = prec[p];
-                        const uint32_t d = (gh ^ r.ph) | (gl ^ r.pl);
-                        if (act && (uint32_t)__popc(d) + r.K <= 64u) {
-                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)r.plan * 8, G.L);
+                        const uint32_t amm = (uint32_t)__popc((gh ^ r.ph) | (gl ^ r.pl));
+                        if (act && amm + r.K <= 64u) {
                             uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[r.plan].pid, amm};
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                         }
                     }
                 } else if (!iupac) {
@@ -808,6 +814,7 @@ struct PlainPlan {
         int W;
         uint32_t group_begin, n_groups;
         uint32_t max_pats;         // largest group of the launch
+        uint32_t bit4_all;         // a group compares bit4 words on plain-base tiles
     };
     std::vector<GroupDev> groups;
     std::vector<Launch> launches;
@@ -822,64 +829,71 @@ struct PlainPlan {
 namespace {
 
 // Matrix-core operand fragments of a plane group (layout and weights as in
-// mfma_chunk): for each block of 16 plane records, 64 lanes x 16 bytes.
-// Returns false (the group stays on the plane compare) when a bias does not
-// fit a signed byte.
+// mfma_chunk): for each block of 16 plane records, 64 lanes x 16 bytes of fp4
+// nibbles.  Every plane group with a window position is scored exactly.
 bool build_mfma_frags(GroupDev& G, const std::vector<const Pattern*>& gp, std::vector<uint4>& frags) {
     if (!G.planes || G.wmask == 0) return false;
-    static const uint32_t kStart[4] = {0, 3, 10, 13}, kExtra[4] = {1, 5, 1, 5};
-    static const int kWeight[4][3] = {{4, 0, 0}, {0, 2, 0}, {0, 0, 1}, {-4, -2, -1}};   // T C A G
-    const uint32_t m20 = G.wmask & 0xFFFFFu;   // the filter's positions
-    auto weight = [&](const Pattern& p, uint32_t pos, int k) -> int {
-        if (!((m20 >> pos) & 1u)) return 0;
-        const uint32_t b = p.b4[G.pk_lo + pos];
-        return kWeight[b == 1 ? 0 : b == 2 ? 1 : b == 4 ? 2 : 3][k];
+    // fp4 (e2m1) codes of +1 / -1 and of the bias weights by value in units of
+    // 0.25 (bias feature 0.5 x weight): 12 = 6.0, 8 = 4.0, 6 = 3.0, 4 = 2.0,
+    // 3 = 1.5, 2 = 1.0, 1 = 0.5
+    constexpr uint8_t kPlus = 0x2, kMinus = 0xA;
+    static const int kUnits[7] = {12, 8, 6, 4, 3, 2, 1};
+    static const uint8_t kCode[7] = {0x7, 0x6, 0x5, 0x4, 0x3, 0x2, 0x1};
+    // weights on the (h, l, hl) features: [genome == b] - (b == T)
+    auto weights = [](uint32_t b, uint8_t w[3]) {
+        w[0] = w[1] = w[2] = 0;
+        if (b == 1) { w[0] = kMinus; w[1] = kMinus; w[2] = kPlus; }        // T: -h - l + hl (+1 in the bias)
+        else if (b == 2) { w[1] = kPlus; w[2] = kMinus; }                  // C: l - hl
+        else if (b == 4) { w[0] = kPlus; w[2] = kMinus; }                  // A: h - hl
+        else { w[2] = kPlus; }                                             // G: hl
     };
-    const int Lw = __builtin_popcount(m20);
-    std::vector<int> bias(gp.size());
-    for (size_t i = 0; i < gp.size(); i++) {
-        int nG = 0;
-        for (uint32_t pos = 0; pos < 20; pos++)
-            if (((m20 >> pos) & 1u) && gp[i]->b4[G.pk_lo + pos] == 8) nG++;
-        bias[i] = 4 * (nG - Lw + (int)std::min<uint32_t>(gp[i]->M, 64u));
-        if (bias[i] < -128 || bias[i] > 127) return false;
-    }
-    if (G.wmask >> 20) {
-        // wider windows: the filter passes pairs with <= M mismatches on
-        // positions 0-19, each checked on the planes.  Expected passes per 16
-        // windows on uniform bases (binomial tail, P(match) = 1/4) must stay
-        // small, or the plane compare alone is cheaper.
-        double passes = 0;
-        for (const Pattern* p : gp) {
-            double tail = 0, c = 1;   // c = C(Lw, k)
-            for (int k = 0; k <= std::min<int>((int)p->M, Lw); k++) {
-                tail += c * std::pow(0.75, k) * std::pow(0.25, Lw - k);
-                c = c * (Lw - k) / (k + 1);
-            }
-            passes += 16.0 * tail;
-        }
-        if (passes > 0.25) return false;
-    }
+    const int Lw = __builtin_popcount(G.wmask);
     G.mfma = 1;
     G.frag_begin = (uint32_t)frags.size();
     const uint32_t nblk = G.prec_padded / 16;
-    for (uint32_t blk = 0; blk < nblk; blk++)
+    for (uint32_t blk = 0; blk < nblk; blk++) {
         for (uint32_t lane = 0; lane < 64; lane++) {
             const uint32_t c = lane >> 4, slot = blk * 16 + (lane & 15u);
-            int8_t by[16] = {0};
+            uint32_t dw[4] = {0, 0, 0, 0};
+            // bias slots: 32 per pattern, 8 in dword 3 of each lane group
+            int units_left;
             if (slot < gp.size()) {
                 const Pattern& p = *gp[slot];
-                for (int d = 0; d < 3; d++)
-                    for (uint32_t e = 0; e < 4; e++) by[4 * d + e] = (int8_t)weight(p, kStart[c] + 2 * e, d);
-                for (int e = 0; e < 3; e++) by[12 + e] = (int8_t)weight(p, kStart[c] + kExtra[c], e);
-                if (c == 3) by[15] = (int8_t)bias[slot];
-            } else if (c == 3) {
-                by[15] = -128;   // filler: never passes
+                int nT = 0;
+                for (uint32_t j = 0; j < 8; j++) {
+                    const uint32_t pos = c + 4 * j;
+                    if (!((G.wmask >> pos) & 1u)) continue;
+                    uint8_t w[3];
+                    weights(p.b4[G.pk_lo + pos], w);
+                    for (int f = 0; f < 3; f++) dw[f] |= (uint32_t)w[f] << (4 * j);
+                }
+                for (uint32_t pos = 0; pos < 32; pos++)
+                    if (((G.wmask >> pos) & 1u) && p.b4[G.pk_lo + pos] == 1) nT++;
+                // 0.5 (nT - Lw + M) + 0.25 in units of 0.25
+                units_left = 2 * (nT - Lw + (int)std::min<uint32_t>(p.M, 64u)) + 1;
+            } else {
+                units_left = -12 * 32;   // filler: -96, never passes
             }
-            uint4 v;
-            std::memcpy(&v, by, 16);
-            frags.push_back(v);
+            // the pattern's bias split greedily over its 32 slots (lane group
+            // 0 first); every lane of the column computes the same split
+            const int sign = units_left < 0 ? -1 : 1;
+            int mag = sign * units_left;
+            for (uint32_t g = 0; g < 4; g++)
+                for (uint32_t j = 0; j < 8; j++) {
+                    uint8_t code = 0;
+                    for (int k = 0; k < 7; k++)
+                        if (mag >= kUnits[k]) {
+                            code = kCode[k];
+                            mag -= kUnits[k];
+                            break;
+                        }
+                    if (code && sign < 0) code |= 0x8;
+                    if (g == c) dw[3] |= (uint32_t)code << (4 * j);
+                }
+            if (mag != 0) throw Error(COF_E_UNSUPPORTED, "matrix-core bias does not fit its slots");
+            frags.push_back(make_uint4(dw[0], dw[1], dw[2], dw[3]));
         }
+    }
     return true;
 }
 
@@ -968,7 +982,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             r.M = p.M;
             r.pid = pi;
             recs.push_back(r);
-            PlaneRec pr{0, 0, 64u - std::min<uint32_t>(p.M, 64u), pat_index};
+            PlaneRec pr{0, 0, 64u - std::min<uint32_t>(p.M, 64u), pi};
             if (planes)
                 for (uint32_t k = pk_lo; k < pk_hi; k++) {
                     if (all_n[k]) continue;
@@ -980,7 +994,7 @@ PlainPlan* plain_plan_create(const Query& q) {
             full.insert(full.end(), fw, fw + 8);
             pat_index++;
         }
-        while ((precs.size() - G.prec_begin) % 16) precs.push_back(PlaneRec{0, 0, 200u, G.pat_begin});
+        while ((precs.size() - G.prec_begin) % 16) precs.push_back(PlaneRec{0, 0, 200u, 0u});
         G.prec_padded = (uint32_t)(precs.size() - G.prec_begin);
         std::vector<const Pattern*> gpats;
         for (uint32_t pi : kv.second) gpats.push_back(&q.pats[pi]);
@@ -988,9 +1002,12 @@ PlainPlan* plain_plan_create(const Query& q) {
         by_w[(int)G.W].push_back(G);
     }
     for (auto& kv : by_w) {
-        uint32_t mp = 0;
-        for (const GroupDev& G : kv.second) mp = std::max(mp, G.pat_count);
-        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp};
+        uint32_t mp = 0, bit4 = 0;
+        for (const GroupDev& G : kv.second) {
+            mp = std::max(mp, G.pat_count);
+            bit4 |= G.planes ? 0u : 1u;
+        }
+        PlainPlan::Launch l{kv.first, (uint32_t)pp->groups.size(), (uint32_t)kv.second.size(), mp, bit4};
         pp->groups.insert(pp->groups.end(), kv.second.begin(), kv.second.end());
         pp->launches.push_back(l);
     }
@@ -1028,14 +1045,14 @@ namespace {
 
 template <int W>
 void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t* d_genome, uint64_t scan_end,
-                  uint64_t first, const GroupDev* g, uint32_t n_groups, const PatRec* pats, const PlaneRec* prec,
-                  const uint32_t* full, const uint4* frag, const Workspace& ws, PlainRaw* out, uint32_t* count,
-                  uint32_t cap) {
+                  uint64_t first, const GroupDev* g, uint32_t n_groups, uint32_t bit4_all, const PatRec* pats,
+                  const PlaneRec* prec, const uint32_t* full, const uint4* frag, const Workspace& ws, PlainRaw* out,
+                  uint32_t* count, uint32_t cap) {
 #define COF_PLAIN_SHAPE(T)                                                                                  \
     case T:                                                                                                 \
         hipLaunchKernelGGL((plain_scan_kernel<W, T>), grid, dim3(T), 0, st, d_genome, scan_end, first, g, n_groups, \
                            pats, prec, full, frag, ws.img_hi, ws.img_lo, ws.img_flags, out, count, cap,      \
-                           t.pat_chunk);                                                                    \
+                           t.pat_chunk, bit4_all);                                                          \
         break;
     switch (t.threads) {
         COF_PLAIN_SHAPE(64) COF_PLAIN_SHAPE(128) COF_PLAIN_SHAPE(256) COF_PLAIN_SHAPE(512)
@@ -1185,8 +1202,8 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
             dim3 grid(nb, std::max(1u, ny));
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
-        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, pp->d_frag, ws, d_out, ws.plain_count, seg_cap);                         \
+        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, l.bit4_all, pp->d_pats,     \
+                         pp->d_prec, pp->d_full, pp->d_frag, ws, d_out, ws.plain_count, seg_cap);          \
         break;
             switch (l.W) {
                 COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
@@ -1321,8 +1338,8 @@ void plain_search_static(PlainPlan* pp, const Workspace& ws, const uint32_t* d_g
             dim3 grid(nb, std::max(1u, ny));
 #define COF_PLAIN_LAUNCH(WW)                                                                                   \
     case WW:                                                                                                   \
-        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, pp->d_pats, pp->d_prec,       \
-                         pp->d_full, pp->d_frag, ws, segs, counts, kStaticSegCap);                            \
+        launch_plain<WW>(tune, grid, st, d_genome, scan_end, first, g, l.n_groups, l.bit4_all, pp->d_pats,     \
+                         pp->d_prec, pp->d_full, pp->d_frag, ws, segs, counts, kStaticSegCap);             \
         break;
             switch (l.W) {
                 COF_PLAIN_LAUNCH(1) COF_PLAIN_LAUNCH(2) COF_PLAIN_LAUNCH(3) COF_PLAIN_LAUNCH(4)
