@@ -35,7 +35,8 @@ import cofpy  # noqa: E402
 INT32_PEAK = 256 * 64 * 2.4e9       # gfx950: 256 CUs x 64 lanes x 2.4 GHz int32 VALU ops/s
 HBM_PEAK = 8.0e12
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions/s: 4 SIMDs per CU, 4 cycles each
-MFMA_I8_PEAK = 5.0e15                  # dense int8 matrix-core ops/s
+MFMA_FP4_PEAK = 10.1e15                # dense fp4 matrix-core ops/s
+MFMA_FP4_OPS = 2 * 16 * 16 * 128      # ops of one v_mfma_f32_16x16x128_f8f6f4
 DIGESTS = os.path.join(ROOT, "tests", "golden", "headline_3.1Gbp_digest.json")
 COUNTERS = os.path.join(ROOT, "profiles", "plain_scan_counters.json")
 PLAIN_SRC = os.path.join(ROOT, "cas-offinder-rust-old_amd", "csrc", "plain.hip")
@@ -138,7 +139,9 @@ def summarize_counters(args):
         p["bank_conflict_cycles_per_lds"] = round(sq[i]["SQ_LDS_BANK_CONFLICT"] / max(sq[i]["SQ_INSTS_LDS"], 1), 3)
         p["clock_ghz"] = round(sq[i]["GRBM_GUI_ACTIVE"] / 8 / t / 1e9, 2)
         mt = mf[i]["ns"] * 1e-9
-        p["mfma_i8_tops"] = round(mf[i]["SQ_INSTS_VALU_MFMA_MOPS_I8"] * 512 / mt / 1e12, 1)
+        # every matrix-core instruction of the kernel is one fp4 16x16x128 MFMA
+        p["mfma_fp4_tops"] = round(mf[i]["SQ_INSTS_MFMA"] * MFMA_FP4_OPS / mt / 1e12, 1)
+        p["mfma_per_wave"] = round(mf[i]["SQ_INSTS_MFMA"] / max(sq[i]["SQ_WAVES"], 1), 1)
         clk = mf[i]["GRBM_GUI_ACTIVE"] / 8
         p["mfma_busy"] = round(mf[i]["SQ_VALU_MFMA_BUSY_CYCLES"] / (clk * 1024), 3)
         p["vgpr"] = sq[i]["vgpr"]
@@ -148,18 +151,19 @@ def summarize_counters(args):
     out.update({"runs": per, "traffic_bytes_per_launch": round(sum((p["fetch_gb"] + p["write_gb"]) * 1e9 for p in per) / n),
                 "fetch_factor": round(ffac, 4), "write_factor": round(wfac, 4),
                 "valu_issue_frac": w("valu_issue"), "mfma_busy_frac": w("mfma_busy"),
-                "mfma_i8_tops": round(sum(p["mfma_i8_tops"] * p["ms_pmc"] for p in per) / tot, 1),
+                "mfma_fp4_tops": round(sum(p["mfma_fp4_tops"] * p["ms_pmc"] for p in per) / tot, 1),
                 "bank_conflict_cycles_per_lds": w("bank_conflict_cycles_per_lds")})
     out["units"] = {"fetch_gb": "FETCH_SIZE x fetch_factor (bytes a 256 MiB copy reads per unit)",
                     "write_gb": "WRITE_SIZE x write_factor", "valu_issue": "SQ_INSTS_VALU / (time x 614.4 G/s)",
                     "mfma_busy": "SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs x 1024 SIMDs)",
-                    "mfma_i8_tops": "SQ_INSTS_VALU_MFMA_MOPS_I8 x 512 / time",
+                    "mfma_fp4_tops": "SQ_INSTS_MFMA x 65,536 (ops of one fp4 16x16x128 MFMA) / time",
                     "weights": "time-weighted means over the runs"}
     with open(args.counters_out or COUNTERS, "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")
     cols = ["run", "ms_pmc", "fetch_gb", "write_gb", "valu_per_wave", "valu_issue", "lds_per_wave",
-            "bank_conflict_cycles_per_lds", "mfma_i8_tops", "mfma_busy", "clock_ghz", "vgpr", "lds_bytes"]
+            "bank_conflict_cycles_per_lds", "mfma_per_wave", "mfma_fp4_tops", "mfma_busy", "clock_ghz", "vgpr",
+            "lds_bytes"]
     print("| " + " | ".join(cols) + " |")
     print("|" + "---|" * len(cols))
     for p in per:
@@ -828,7 +832,7 @@ def main():
     cdoc, csrc = measured_counters() if default_size else (None, "not the default workload")
     traffic = cdoc["traffic_bytes_per_launch"] if cdoc else None
     valu_issue = cdoc["valu_issue_frac"] if cdoc else None
-    mfma_tops = cdoc["mfma_i8_tops"] if cdoc else None
+    mfma_tops = cdoc["mfma_fp4_tops"] if cdoc else None
     mfma_busy = cdoc["mfma_busy_frac"] if cdoc else None
     per_run = [{"run": r["name"], "hits": hits[i], "kernel_ms": round(kms[i], 3),
                 "oracle_rows_match": verify[r["name"]]["match"] if verify else None,
@@ -855,7 +859,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8 (matrix-core window scores) + u32 (bit4/plane ops)",
+        "dtype": "fp4 (matrix-core window scores: exact small integers in fp32) + u32 (bit4/plane ops)",
         "data": "synthetic",
         "config": {"workload": f"plain GPU mismatch scan: {genome_nucl / 1e9:.2f} Gbp synthetic genome (24 chroms, "
                                "41% GC, 5% N), 100 guides x len{20,25} x PAM{NRG,NNGRRT,TTTN} x M{3,6}, both "
@@ -872,14 +876,14 @@ def main():
                            "frac": round(achieved / INT32_PEAK, 3),
                            "note": "achieved = brute-force int32 ops 2*(3*ceil(L/8)+1) per guide-bp / plain-scan "
                                    "kernel time; the kernel tests the PAM first and so executes fewer ops"},
-        "counters": {"valu_issue_frac": valu_issue, "mfma_i8_tops": mfma_tops,
-                     "mfma_i8_frac": round(mfma_tops * 1e12 / MFMA_I8_PEAK, 3) if mfma_tops else None,
+        "counters": {"valu_issue_frac": valu_issue, "mfma_fp4_tops": mfma_tops,
+                     "mfma_fp4_frac": round(mfma_tops * 1e12 / MFMA_FP4_PEAK, 3) if mfma_tops else None,
                      "mfma_busy_frac": mfma_busy,
                      "bank_conflict_cycles_per_lds": cdoc["bank_conflict_cycles_per_lds"] if cdoc else None,
                      "source": csrc,
                      "note": "measured issue of plain_scan_kernel (rocprofv3 --pmc, default workload only): VALU "
-                             "instructions against the gfx950 issue peak; i8 matrix-core ops against the dense "
-                             "int8 peak of 5.0 POP/s and matrix-core busy cycles"},
+                             "instructions against the gfx950 issue peak; fp4 matrix-core ops against the dense "
+                             "fp4 peak of 10.1 POP/s and matrix-core busy cycles"},
         "cpu_baseline": baseline,
         "bit_exact": (all(v["match"] for v in verify.values()) if verify is not None else None),
         "bit_exact_check": (f"sorted rows of all {len(verify)} runs vs the oracle's row counts and SHA-256 digests "

@@ -342,6 +342,18 @@ __device__ __forceinline__ void mfma_compare(const uint32_t* ghi, const uint32_t
     }
 }
 
+// Workgroups go to the 8 XCDs round-robin (workgroup b on XCD b mod 8), and
+// each XCD has its own L2.  Tile order: within every run of 512 workgroups,
+// XCD x takes 64 consecutive tiles, so a tile's halo (the next tile's first
+// words and plane words) is normally in its own XCD's L2 already.  A
+// bijection on whole runs; the last partial run keeps the identity order.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    constexpr uint32_t kRun = 64, kGroup = 8 * kRun;
+    if (b >= n / kGroup * kGroup) return b;
+    const uint32_t g = b / kGroup, r = b % kGroup;
+    return g * kGroup + (r & 7u) * kRun + (r >> 3);
+}
+
 template <int W, int THREADS>
 __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
@@ -376,7 +388,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: scalar loops over waves
-    const uint64_t s0 = first_tile_nib + (uint64_t)blockIdx.x * kTile;
+    const uint64_t s0 = first_tile_nib + (uint64_t)xcd_tile(blockIdx.x, gridDim.x) * kTile;
     const uint64_t w0 = s0 >> 3;
     // this workgroup's hit segment
     uint32_t* const out_count = counts + (blockIdx.x & (kHitSegs - 1)) * kSegStride;
@@ -447,298 +459,309 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     }
     tile_sync<kThreads>();
     const int iupac = flags & 1;
-    // the planes serve tiles of plain bases only
-    const bool plain_tile = flags == 0;
 
     // Bit4 PAM test: thread tid owns tile words w * kThreads + tid (w < 8), 8
     // starts each; plane PAM test: plane words w * kThreads + tid (w < 2), 32
     // starts each.  Consecutive lanes read consecutive LDS words (no bank
     // conflicts).
 
-    for (uint32_t gi = 0; gi < n_groups; gi++) {
-        const GroupDev& G = groups[gi];
-        uint32_t pb = G.pat_begin, pe = G.pat_begin + G.pat_count;
-        if (pat_chunk) {
-            const uint32_t lo = blockIdx.y * pat_chunk;   // uniform over the workgroup
-            if (lo >= G.pat_count) continue;
-            pb += lo;
-            pe = min(pe, pb + pat_chunk);
-        }
-        // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
-        uint32_t m8[8];
-        if (plain_tile) {
-            // planes: per PAM cell and 32 starts, two bitop3 selects by the l
-            // plane, one by the h plane and an AND (pam_cell2)
-            uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
-            for (uint32_t k = 0; k < G.n_pam; k++) {
-                const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
-                const uint32_t qo = pk >> 5, sh = pk & 31u;
-                const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
-                const uint32_t h0 = __builtin_amdgcn_alignbit(ghi[q0 + 1], ghi[q0], sh);
-                const uint32_t l0 = __builtin_amdgcn_alignbit(glo[q0 + 1], glo[q0], sh);
-                const uint32_t h1 = __builtin_amdgcn_alignbit(ghi[q1 + 1], ghi[q1], sh);
-                const uint32_t l1 = __builtin_amdgcn_alignbit(glo[q1 + 1], glo[q1], sh);
-                pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
+    // The group loop, compiled once for tiles of plain bases (plane PAM test,
+    // matrix-core or plane compare) and once for flagged tiles (bit4 words),
+    // so neither copy carries the other's set-up.  Only the tile holding
+    // scan_end masks starts past it.
+    const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
+    auto run_groups = [&](auto plain_tag) {
+        constexpr bool kPlainTile = decltype(plain_tag)::value;
+        for (uint32_t gi = 0; gi < n_groups; gi++) {
+            const GroupDev& G = groups[gi];
+            uint32_t pb = G.pat_begin, pe = G.pat_begin + G.pat_count;
+            if (pat_chunk) {
+                const uint32_t lo = blockIdx.y * pat_chunk;   // uniform over the workgroup
+                if (lo >= G.pat_count) continue;
+                pb += lo;
+                pe = min(pe, pb + pat_chunk);
             }
-#pragma unroll
-            for (int w = 0; w < 2; w++) {
-                uint32_t m = ok[w];
-                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
-                if (scan_end <= first) m = 0;
-                else if (scan_end < first + 32) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
-                m8[w] = m;
-            }
-#pragma unroll
-            for (int w = 2; w < 8; w++) m8[w] = 0;
-        } else {
-            // bit4, 8 positions per word: nibble j of X_k is the genome at
-            // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
-            // is in the filter's plain-base set iff it overlaps the filter code.
-            uint32_t ok[8];
-#pragma unroll
-            for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
-            for (uint32_t k = 0; k < G.n_pam; k++) {
-                const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
-                const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
-#pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    const uint32_t q = (uint32_t)w * kThreads + tid + qo;
-                    const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
-                    const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
-                    ok[w] &= nz_nibbles(X & pmask) & ~bad;
+            // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
+            uint32_t m8[8];
+            if constexpr (kPlainTile) {
+                // planes: per PAM cell and 32 starts, two bitop3 selects by the l
+                // plane, one by the h plane and an AND (pam_cell2)
+                uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+                for (uint32_t k = 0; k < G.n_pam; k++) {
+                    const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
+                    const uint32_t qo = pk >> 5, sh = pk & 31u;
+                    const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
+                    const uint32_t h0 = __builtin_amdgcn_alignbit(ghi[q0 + 1], ghi[q0], sh);
+                    const uint32_t l0 = __builtin_amdgcn_alignbit(glo[q0 + 1], glo[q0], sh);
+                    const uint32_t h1 = __builtin_amdgcn_alignbit(ghi[q1 + 1], ghi[q1], sh);
+                    const uint32_t l1 = __builtin_amdgcn_alignbit(glo[q1 + 1], glo[q1], sh);
+                    pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
                 }
-            }
+                    m8[0] = ok[0];
+                m8[1] = ok[1];
+                if (tail) {
 #pragma unroll
-            for (int w = 0; w < 8; w++) {
-                uint32_t m = gather_nibble_flags(ok[w]);
-                // the last tile ends at scan_end
-                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
-                if (scan_end <= first) m = 0;
-                else if (scan_end < first + 8) m &= (1u << (uint32_t)(scan_end - first)) - 1u;
-                m8[w] = m;
-            }
-        }
-        // block-wide exclusive scan of per-thread values: a DPP prefix sum in
-        // the wave, the wave totals through LDS; tot = the block total.  The
-        // sums stay below 2^16, so a value may pack two counts.
-        auto block_scan = [&](uint32_t v, uint32_t& tot, uint32_t& excl) {
-            const uint32_t inc = wave_prefix_sum(v);
-            const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-            uint32_t off = 0;
-            tot = wtot;
-            if (kThreads > 64) {
-                if (lane == 0) wave_sum[wave] = wtot;
-                __syncthreads();
-                tot = 0;
-#pragma unroll
-                for (int x = 0; x < kThreads / 64; x++) {
-                    uint32_t ws = wave_sum[x];
-                    off += (uint32_t)x < wave ? ws : 0u;
-                    tot += ws;
+                    for (int w = 0; w < 2; w++) {
+                        const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
+                        if (scan_end <= first) m8[w] = 0;
+                        else if (scan_end < first + 32) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
+                    }
                 }
-                __syncthreads();   // wave_sum is written again by the next scan
-            }
-            excl = off + inc - v;
-        };
-        // The window list holds half a tile: the tile's windows are compared
-        // in one pass when they fit, else in two passes over the halves of
-        // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3).
-        // Plain-base tiles list their windows in start order (plane word 0 of
-        // every thread, then word 1), so 16 consecutive entries lie in a few
-        // consecutive LDS words: one scan of both words' counts, packed.
-        uint32_t nc_all, pos_all = 0, tot0 = 0, ex0 = 0, ex1 = 0;
-        if (plain_tile) {
-            uint32_t tot, ex;
-            block_scan((uint32_t)__popc(m8[0]) | ((uint32_t)__popc(m8[1]) << 16), tot, ex);
-            tot0 = tot & 0xFFFFu;
-            nc_all = tot0 + (tot >> 16);
-            ex0 = ex & 0xFFFFu;
-            ex1 = ex >> 16;
-        } else {
-            uint32_t cnt_all = 0;
-#pragma unroll
-            for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
-            block_scan(cnt_all, nc_all, pos_all);
-        }
-        const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
-        for (uint32_t h = 0; h < npass; h++) {
-            uint32_t nc;
-            if (plain_tile) {
-                // word 0's windows from 0, word 1's after them (one pass) or
-                // from 0 (second pass)
-                nc = npass == 1 ? nc_all : h == 0 ? tot0 : nc_all - tot0;
-                uint32_t m = (npass == 1 || h == 0) ? m8[0] : 0u;
-                uint32_t pos = ex0;
-                while (m) {
-                    const uint32_t j = (uint32_t)__builtin_ctz(m);
-                    cand[pos++] = (uint16_t)(tid * 32u + j);
-                    m &= m - 1u;
-                }
-                m = (npass == 1 || h == 1) ? m8[1] : 0u;
-                pos = (npass == 1 ? tot0 : 0u) + ex1;
-                while (m) {
-                    const uint32_t j = (uint32_t)__builtin_ctz(m);
-                    cand[pos++] = (uint16_t)(((uint32_t)kThreads + tid) * 32u + j);
-                    m &= m - 1u;
-                }
+    #pragma unroll
+                for (int w = 2; w < 8; w++) m8[w] = 0;
             } else {
-                uint32_t mh[8];
-#pragma unroll
-                for (int w = 0; w < 8; w++) mh[w] = (npass == 1 || ((uint32_t)w >> 2) == h) ? m8[w] : 0u;
-                uint32_t pos = pos_all;
-                nc = nc_all;
-                if (npass == 2) {
-                    uint32_t cnt = 0;
-#pragma unroll
-                    for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
-                    block_scan(cnt, nc, pos);
+                // bit4, 8 positions per word: nibble j of X_k is the genome at
+                // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
+                // is in the filter's plain-base set iff it overlaps the filter code.
+                uint32_t ok[8];
+    #pragma unroll
+                for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
+                for (uint32_t k = 0; k < G.n_pam; k++) {
+                    const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
+                    const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
+    #pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        const uint32_t q = (uint32_t)w * kThreads + tid + qo;
+                        const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
+                        const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
+                        ok[w] &= nz_nibbles(X & pmask) & ~bad;
+                    }
                 }
+    #pragma unroll
+                for (int w = 0; w < 8; w++) m8[w] = gather_nibble_flags(ok[w]);
+                if (tail) {
 #pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    uint32_t m = mh[w];
-                    while (m) {
-                        uint32_t j = (uint32_t)__builtin_ctz(m);
-                        cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
-                        m &= m - 1u;
+                    for (int w = 0; w < 8; w++) {
+                        const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
+                        if (scan_end <= first) m8[w] = 0;
+                        else if (scan_end < first + 8) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
                     }
                 }
             }
-            const bool planes = plain_tile && G.planes;   // uniform
-            const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
-            // the matrix-core compare reads whole blocks of 16 list entries: pad
-            // the list with start 0 (outputs of rows past the list are dropped)
-            if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
-            tile_sync<kThreads>();
-            // 3. compare windows against the group's patterns
-            if (mfma) {
-                mfma_compare<kThreads>(ghi, glo, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
-                                       wave);
-                tile_sync<kThreads>();
-                continue;
+            // block-wide exclusive scan of per-thread values: a DPP prefix sum in
+            // the wave, the wave totals through LDS; tot = the block total.  The
+            // sums stay below 2^16, so a value may pack two counts.
+            auto block_scan = [&](uint32_t v, uint32_t& tot, uint32_t& excl) {
+                const uint32_t inc = wave_prefix_sum(v);
+                const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+                uint32_t off = 0;
+                tot = wtot;
+                if (kThreads > 64) {
+                    if (lane == 0) wave_sum[wave] = wtot;
+                    __syncthreads();
+                    tot = 0;
+    #pragma unroll
+                    for (int x = 0; x < kThreads / 64; x++) {
+                        uint32_t ws = wave_sum[x];
+                        off += (uint32_t)x < wave ? ws : 0u;
+                        tot += ws;
+                    }
+                    __syncthreads();   // wave_sum is written again by the next scan
+                }
+                excl = off + inc - v;
+            };
+            // The window list holds half a tile: the tile's windows are compared
+            // in one pass when they fit, else in two passes over the halves of
+            // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3).
+            // Plain-base tiles list their windows in start order (plane word 0 of
+            // every thread, then word 1), so 16 consecutive entries lie in a few
+            // consecutive LDS words: one scan of both words' counts, packed.
+            uint32_t nc_all, pos_all = 0, tot0 = 0, ex0 = 0, ex1 = 0;
+            if constexpr (kPlainTile) {
+                uint32_t tot, ex;
+                block_scan((uint32_t)__popc(m8[0]) | ((uint32_t)__popc(m8[1]) << 16), tot, ex);
+                tot0 = tot & 0xFFFFu;
+                nc_all = tot0 + (tot >> 16);
+                ex0 = ex & 0xFFFFu;
+                ex1 = ex >> 16;
+            } else {
+                uint32_t cnt_all = 0;
+    #pragma unroll
+                for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
+                block_scan(cnt_all, nc_all, pos_all);
             }
-            for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
-                uint32_t c = c0 + lane;
-                bool active = c < nc;
-                uint32_t i = active ? cand[c] : 0;
-                // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
-                auto bit4_compare = [&](bool act) {
-                    uint32_t q = i + G.k_lo;
-                    uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
-                    uint32_t win[W];
-#pragma unroll
-                    for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                    // four patterns per iteration: their records are fetched with
-                    // one batch of scalar loads (pointer offsets, no per-pattern
-                    // address arithmetic), and one branch covers the four hits
-                    uint32_t p = pb;
-                    for (; p + 4 <= pe; p += 4) {
-                        const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
-                        uint32_t hit = 0;
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            uint32_t sum = 0;
-#pragma unroll
-                            for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
-                            hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
+            const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
+            for (uint32_t h = 0; h < npass; h++) {
+                uint32_t nc;
+                if constexpr (kPlainTile) {
+                    // word 0's windows from 0, word 1's after them (one pass) or
+                    // from 0 (second pass)
+                    nc = npass == 1 ? nc_all : h == 0 ? tot0 : nc_all - tot0;
+                    uint32_t m = (npass == 1 || h == 0) ? m8[0] : 0u;
+                    uint32_t pos = ex0;
+                    while (m) {
+                        const uint32_t j = (uint32_t)__builtin_ctz(m);
+                        cand[pos++] = (uint16_t)(tid * 32u + j);
+                        m &= m - 1u;
+                    }
+                    m = (npass == 1 || h == 1) ? m8[1] : 0u;
+                    pos = (npass == 1 ? tot0 : 0u) + ex1;
+                    while (m) {
+                        const uint32_t j = (uint32_t)__builtin_ctz(m);
+                        cand[pos++] = (uint16_t)(((uint32_t)kThreads + tid) * 32u + j);
+                        m &= m - 1u;
+                    }
+                } else {
+                    uint32_t mh[8];
+    #pragma unroll
+                    for (int w = 0; w < 8; w++) mh[w] = (npass == 1 || ((uint32_t)w >> 2) == h) ? m8[w] : 0u;
+                    uint32_t pos = pos_all;
+                    nc = nc_all;
+                    if (npass == 2) {
+                        uint32_t cnt = 0;
+    #pragma unroll
+                        for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
+                        block_scan(cnt, nc, pos);
+                    }
+    #pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        uint32_t m = mh[w];
+                        while (m) {
+                            uint32_t j = (uint32_t)__builtin_ctz(m);
+                            cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
+                            m &= m - 1u;
                         }
-                        if (act && hit) {
+                    }
+                }
+                const bool planes = kPlainTile && G.planes;   // uniform
+                const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
+                // the matrix-core compare reads whole blocks of 16 list entries: pad
+                // the list with start 0 (outputs of rows past the list are dropped)
+                if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
+                tile_sync<kThreads>();
+                // 3. compare windows against the group's patterns
+                if (mfma) {
+                    mfma_compare<kThreads>(ghi, glo, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
+                                           wave);
+                    tile_sync<kThreads>();
+                    continue;
+                }
+                for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
+                    uint32_t c = c0 + lane;
+                    bool active = c < nc;
+                    uint32_t i = active ? cand[c] : 0;
+                    // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
+                    auto bit4_compare = [&](bool act) {
+                        uint32_t q = i + G.k_lo;
+                        uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
+                        uint32_t win[W];
+    #pragma unroll
+                        for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
+                        // four patterns per iteration: their records are fetched with
+                        // one batch of scalar loads (pointer offsets, no per-pattern
+                        // address arithmetic), and one branch covers the four hits
+                        uint32_t p = pb;
+                        for (; p + 4 <= pe; p += 4) {
+                            const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
+                            uint32_t hit = 0;
+    #pragma unroll
                             for (int u = 0; u < 4; u++) {
-                                if (!((hit >> u) & 1u)) continue;
-                                const PatRec& r = pats[p + u];
-                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                                uint32_t sum = 0;
+    #pragma unroll
+                                for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
+                                hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
+                            }
+                            if (act && hit) {
+                                for (int u = 0; u < 4; u++) {
+                                    if (!((hit >> u) & 1u)) continue;
+                                    const PatRec& r = pats[p + u];
+                                    uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                                    uint32_t idx = atomicAdd(out_count, 1u);
+                                    if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                                }
+                            }
+                        }
+                        for (; p < pe; p++) {
+                            const PatRec& r = pats[p];
+                            uint32_t sum = 0;
+    #pragma unroll
+                            for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
+                            if (act && (int32_t)sum >= r.thr) {
+                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
                                 uint32_t idx = atomicAdd(out_count, 1u);
                                 if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                             }
                         }
-                    }
-                    for (; p < pe; p++) {
-                        const PatRec& r = pats[p];
-                        uint32_t sum = 0;
-#pragma unroll
-                        for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
-                        if (act && (int32_t)sum >= r.thr) {
-                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
-                            uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                        }
-                    }
-                };
-                if (planes) {
-                    // plain bases only: kmm = window positions whose planes differ
-                    // (positions that are N in every pattern match any base)
-                    const uint32_t q = i + G.pk_lo;
-                    const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
-                    const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
-                    const bool act = active;
-                    const uint64_t act_mask = __ballot(act);
-                    // plane records of patterns [pb, pe); a whole group runs on
-                    // its padded record count
-                    uint32_t p = G.prec_begin + (pb - G.pat_begin);
-                    const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
-                                                  : G.prec_begin + ((G.pat_count + 7u) & ~7u);
-                    // NB patterns per batch: per pattern xor, bitop3, v_bcnt
-                    // accumulating K and half a v_min3, one compare per batch; all
-                    // NB records are loaded before the batch computes
-                    auto batch = [&](auto nb_tag) {
-                        constexpr int NB = decltype(nb_tag)::value;
-                        const PlaneRec* const rp = prec + p;   // uniform base
-                        uint32_t mn = 0xFFFFFFFFu;
-#pragma unroll
-                        for (int u = 0; u < NB; u += 2) {
-                            const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
-                            const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
-                            mn = min(mn, min(t0, t1));
-                        }
-                        if (__ballot(mn <= 64u) & act_mask) {
-                            for (int u = 0; u < NB; u++) {
-                                const uint32_t amm = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl));
-                                if (!(act && amm + rp[u].K <= 64u)) continue;
+                    };
+                    if (planes) {
+                        // plain bases only: kmm = window positions whose planes differ
+                        // (positions that are N in every pattern match any base)
+                        const uint32_t q = i + G.pk_lo;
+                        const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
+                        const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                        const bool act = active;
+                        const uint64_t act_mask = __ballot(act);
+                        // plane records of patterns [pb, pe); a whole group runs on
+                        // its padded record count
+                        uint32_t p = G.prec_begin + (pb - G.pat_begin);
+                        const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
+                                                      : G.prec_begin + ((G.pat_count + 7u) & ~7u);
+                        // NB patterns per batch: per pattern xor, bitop3, v_bcnt
+                        // accumulating K and half a v_min3, one compare per batch; all
+                        // NB records are loaded before the batch computes
+                        auto batch = [&](auto nb_tag) {
+                            constexpr int NB = decltype(nb_tag)::value;
+                            const PlaneRec* const rp = prec + p;   // uniform base
+                            uint32_t mn = 0xFFFFFFFFu;
+    #pragma unroll
+                            for (int u = 0; u < NB; u += 2) {
+                                const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
+                                const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
+                                mn = min(mn, min(t0, t1));
+                            }
+                            if (__ballot(mn <= 64u) & act_mask) {
+                                for (int u = 0; u < NB; u++) {
+                                    const uint32_t amm = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl));
+                                    if (!(act && amm + rp[u].K <= 64u)) continue;
+                                    uint32_t idx = atomicAdd(out_count, 1u);
+                                    if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
+                                }
+                            }
+                            p += NB;
+                        };
+                        // batches of 16 while more than 8 records remain (a whole
+                        // group may run into its fillers, a pattern slice may not),
+                        // then at most one batch of 8, then single records
+                        const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
+                        while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
+                        if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
+                        for (; p < qe; p++) {
+                            const PlaneRec& r = prec[p];
+                            const uint32_t amm = (uint32_t)__popc((gh ^ r.ph) | (gl ^ r.pl));
+                            if (act && amm + r.K <= 64u) {
                                 uint32_t idx = atomicAdd(out_count, 1u);
-                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
+                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
                             }
                         }
-                        p += NB;
-                    };
-                    // batches of 16 while more than 8 records remain (a whole
-                    // group may run into its fillers, a pattern slice may not),
-                    // then at most one batch of 8, then single records
-                    const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
-                    while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
-                    if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
-                    for (; p < qe; p++) {
-                        const PlaneRec& r = prec[p];
-                        const uint32_t amm = (uint32_t)__popc((gh ^ r.ph) | (gl ^ r.pl));
-                        if (act && amm + r.K <= 64u) {
-                            uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                        }
-                    }
-                } else if (!iupac) {
-                    bit4_compare(active);
-                } else {
-                    for (uint32_t p = pb; p < pe; p++) {
-                        const uint32_t* fw = full_words + (size_t)p * 8;
-                        int32_t pop = 0;
-                        uint32_t amm = 0;
-                        for (uint32_t k = 0; k < G.L; k++) {
-                            uint32_t q = i + k;
-                            uint32_t g = nib_of(gw[q >> 3], q & 7u);
-                            uint32_t pb = nib_of(fw[k >> 3], k & 7u);
-                            pop += __popc(pb & g);
-                            bool single = g != 0 && (g & (g - 1)) == 0;
-                            amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
-                        }
-                        uint32_t M = pats[p].M;
-                        if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
-                            uint32_t idx = atomicAdd(out_count, 1u);
-                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
+                    } else if (!iupac) {
+                        bit4_compare(active);
+                    } else {
+                        for (uint32_t p = pb; p < pe; p++) {
+                            const uint32_t* fw = full_words + (size_t)p * 8;
+                            int32_t pop = 0;
+                            uint32_t amm = 0;
+                            for (uint32_t k = 0; k < G.L; k++) {
+                                uint32_t q = i + k;
+                                uint32_t g = nib_of(gw[q >> 3], q & 7u);
+                                uint32_t pb = nib_of(fw[k >> 3], k & 7u);
+                                pop += __popc(pb & g);
+                                bool single = g != 0 && (g & (g - 1)) == 0;
+                                amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
+                            }
+                            uint32_t M = pats[p].M;
+                            if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
+                                uint32_t idx = atomicAdd(out_count, 1u);
+                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
+                            }
                         }
                     }
                 }
+                tile_sync<kThreads>();
             }
-            tile_sync<kThreads>();
         }
-    }
+    };
+    if (flags == 0) run_groups(std::true_type{});
+    else run_groups(std::false_type{});
 }
 
 // Scan image of the genome, one wave per 4096-start unit u: the flags of
