@@ -240,7 +240,7 @@ __device__ __forceinline__ mf_v4f mfma_fp4(const mf_v8i& a, const mf_v8i& b) {
 // the list: per 16 windows, NB independent MFMAs and an AND of their score
 // signs; the hit path runs only when some score is positive.
 template <int NB, int THREADS>
-__device__ __forceinline__ void mfma_chunk(const uint32_t* ghi, const uint32_t* glo, const uint16_t* cand,
+__device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand,
                                            uint32_t nc, const GroupDev& G, uint32_t bc,
                                            const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
                                            uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
@@ -268,16 +268,14 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* ghi, const uint32_t* 
     uint32_t cb = wave * 16u;
     if (cb >= nc) return;
     uint32_t q1 = (uint32_t)cand[cb + r] + off;   // plane position of this block's window
-    uint32_t h0 = ghi[q1 >> 5], h1 = ghi[(q1 >> 5) + 1], l0 = glo[q1 >> 5], l1 = glo[(q1 >> 5) + 1];
+    uint2 p0 = gp[q1 >> 5], p1 = gp[(q1 >> 5) + 1];   // (hi, lo) plane words
     uint32_t q2 = (uint32_t)cand[(cb + kStep < nc ? cb + kStep : cb) + r] + off;
     for (; cb < nc; cb += kStep) {
-        const uint32_t wh = __builtin_amdgcn_alignbit(h1, h0, q1 & 31u);
-        const uint32_t wl = __builtin_amdgcn_alignbit(l1, l0, q1 & 31u);
+        const uint32_t wh = __builtin_amdgcn_alignbit(p1.x, p0.x, q1 & 31u);
+        const uint32_t wl = __builtin_amdgcn_alignbit(p1.y, p0.y, q1 & 31u);
         q1 = q2;
-        h0 = ghi[q1 >> 5];
-        h1 = ghi[(q1 >> 5) + 1];
-        l0 = glo[q1 >> 5];
-        l1 = glo[(q1 >> 5) + 1];
+        p0 = gp[q1 >> 5];
+        p1 = gp[(q1 >> 5) + 1];
         q2 = (uint32_t)cand[(cb + 2u * kStep < nc ? cb + 2u * kStep : cb) + r] + off;
         const mf_v4i af = {(int32_t)(wh & 0x11111111u), (int32_t)(wl & 0x11111111u),
                            (int32_t)__builtin_amdgcn_bitop3_b32(wh, wl, 0x11111111u, 0x80), 0x11111111};
@@ -320,7 +318,7 @@ __device__ __forceinline__ void mfma_chunk(const uint32_t* ghi, const uint32_t* 
 
 // The group's pattern blocks [pb, pe) in chunks of up to 8 blocks.
 template <int THREADS>
-__device__ __forceinline__ void mfma_compare(const uint32_t* ghi, const uint32_t* glo, const uint16_t* cand,
+__device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* cand,
                                              uint32_t nc, const GroupDev& G, uint32_t pb, uint32_t pe,
                                              const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
                                              uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
@@ -331,7 +329,7 @@ __device__ __forceinline__ void mfma_compare(const uint32_t* ghi, const uint32_t
         const uint32_t nb = min(8u, blk_hi - bc);   // uniform
 #define COF_MFMA_CHUNK(N)                                                                                         \
     case N:                                                                                                       \
-        mfma_chunk<N, THREADS>(ghi, glo, cand, nc, G, bc, mfrag, prec, s0, out, out_count, out_cap, lane, wave); \
+        mfma_chunk<N, THREADS>(gp, cand, nc, G, bc, mfrag, prec, s0, out, out_count, out_cap, lane, wave);       \
         break;
         switch (nb) {
             COF_MFMA_CHUNK(1) COF_MFMA_CHUNK(2) COF_MFMA_CHUNK(3) COF_MFMA_CHUNK(4)
@@ -371,17 +369,17 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     constexpr int kTileWords = kTile / 8 + kHaloWords;
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
     constexpr int kThreads = THREADS;
-    // one LDS block, the bit4 words at offset 0 (LDS addresses need no base)
+    // one LDS block, the plane words at offset 0: a window's or PAM cell's
+    // four plane words are one ds_read2_b64 with no base to add
     struct alignas(16) TileLds {
-        uint32_t gw[kTileWords];
-        uint32_t ghi[kPlaneWords], glo[kPlaneWords];
+        uint2 gp[kPlaneWords];      // (hi, lo) plane words
+        uint32_t gw[kTileWords];    // bit4 words (flagged tiles, bit4 groups)
         uint16_t cand[kTile / 2];   // one half tile's windows
         uint32_t wave_sum[kThreads / 64];
     };
     __shared__ TileLds lds;
     uint32_t* const gw = lds.gw;
-    uint32_t* const ghi = lds.ghi;
-    uint32_t* const glo = lds.glo;
+    uint2* const gp = lds.gp;
     uint16_t* const cand = lds.cand;
     uint32_t* const wave_sum = lds.wave_sum;
 
@@ -452,8 +450,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         for (int it = 0; it < kPlaneIt; it++) {
             const uint32_t j = tid + (uint32_t)it * kThreads;
             if (j < (uint32_t)kPlaneWords) {
-                ghi[j] = ph[it];
-                glo[j] = pl[it];
+                gp[j] = make_uint2(ph[it], pl[it]);
             }
         }
     }
@@ -491,10 +488,11 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                     const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
                     const uint32_t qo = pk >> 5, sh = pk & 31u;
                     const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
-                    const uint32_t h0 = __builtin_amdgcn_alignbit(ghi[q0 + 1], ghi[q0], sh);
-                    const uint32_t l0 = __builtin_amdgcn_alignbit(glo[q0 + 1], glo[q0], sh);
-                    const uint32_t h1 = __builtin_amdgcn_alignbit(ghi[q1 + 1], ghi[q1], sh);
-                    const uint32_t l1 = __builtin_amdgcn_alignbit(glo[q1 + 1], glo[q1], sh);
+                    const uint2 a0 = gp[q0], a1 = gp[q0 + 1], b0 = gp[q1], b1 = gp[q1 + 1];
+                    const uint32_t h0 = __builtin_amdgcn_alignbit(a1.x, a0.x, sh);
+                    const uint32_t l0 = __builtin_amdgcn_alignbit(a1.y, a0.y, sh);
+                    const uint32_t h1 = __builtin_amdgcn_alignbit(b1.x, b0.x, sh);
+                    const uint32_t l1 = __builtin_amdgcn_alignbit(b1.y, b0.y, sh);
                     pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
                 }
                     m8[0] = ok[0];
@@ -631,7 +629,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                 tile_sync<kThreads>();
                 // 3. compare windows against the group's patterns
                 if (mfma) {
-                    mfma_compare<kThreads>(ghi, glo, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
+                    mfma_compare<kThreads>(gp, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
                                            wave);
                     tile_sync<kThreads>();
                     continue;
@@ -687,8 +685,9 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
                         // plain bases only: kmm = window positions whose planes differ
                         // (positions that are N in every pattern match any base)
                         const uint32_t q = i + G.pk_lo;
-                        const uint32_t gh = __builtin_amdgcn_alignbit(ghi[(q >> 5) + 1], ghi[q >> 5], q & 31u) & G.wmask;
-                        const uint32_t gl = __builtin_amdgcn_alignbit(glo[(q >> 5) + 1], glo[q >> 5], q & 31u) & G.wmask;
+                        const uint2 w0 = gp[q >> 5], w1 = gp[(q >> 5) + 1];
+                        const uint32_t gh = __builtin_amdgcn_alignbit(w1.x, w0.x, q & 31u) & G.wmask;
+                        const uint32_t gl = __builtin_amdgcn_alignbit(w1.y, w0.y, q & 31u) & G.wmask;
                         const bool act = active;
                         const uint64_t act_mask = __ballot(act);
                         // plane records of patterns [pb, pe); a whole group runs on
