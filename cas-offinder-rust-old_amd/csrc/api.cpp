@@ -484,9 +484,9 @@ void search_end(cof_ctx* ctx) {
     cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, ctx->batches, ctx->d_genome, ctx->stream, braw, &st, bt);
     st.n_raw_hits = praw.size() + braw.size();
     auto tc = std::chrono::steady_clock::now();
-    cof::convert_plain(ctx->g, ctx->q, ctx->batches, praw, ctx->hits, ctx->aux);
+    cof::convert_plain(ctx->g, ctx->q, ctx->batches, ctx->bidx, praw, ctx->hits, ctx->aux);
     size_t first_bulge = ctx->hits.size();
-    cof::convert_bulge(ctx->g, ctx->q, ctx->batches, braw, ctx->hits, ctx->aux);
+    cof::convert_bulge(ctx->g, ctx->q, ctx->batches, ctx->bidx, braw, ctx->hits, ctx->aux);
     if (!(ctx->q.flags & COF_FLAG_COUNT_ONLY)) {
         st.bulge_hits = ctx->hits.size() - first_bulge;
         for (size_t h = first_bulge; h < ctx->hits.size(); h++) {

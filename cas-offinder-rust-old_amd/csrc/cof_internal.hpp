@@ -212,9 +212,9 @@ bool place_plain_hit(const Genome& g, const Query& q, const std::vector<Batch>& 
 // returns the number of final hits.  Allocates nothing.
 uint64_t convert_plain_into(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
                             RawView<PlainRaw> raw, cof_hit* out);
-void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& batches,
+void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& batches, const BatchIndex& bi,
                    RawView<PlainRaw> raw, HitVec& out, AuxVec& aux);
-void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches,
+void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& batches, const BatchIndex& bi,
                    RawView<BulgeRaw> raw, HitVec& out, AuxVec& aux);
 void write_tsv(const Genome& g, const Query& q, const HitVec& hits, const AuxVec& aux, FILE* f);
 // Match.dna_seq / Match.rna_seq of one final hit, before the writer's case

@@ -225,9 +225,8 @@ bool place_plain_hit(const Genome& g, const Query& q, const std::vector<Batch>& 
     return true;
 }
 
-void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<PlainRaw> raw,
-                   HitVec& out_all, AuxVec& aux_all) {
-    BatchIndex bi(g, bs);
+void convert_plain(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                   RawView<PlainRaw> raw, HitVec& out_all, AuxVec& aux_all) {
     parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, cof_hit* out, HitAux* aux) -> size_t {
         size_t k = 0;
         cof_hit h;
@@ -257,9 +256,8 @@ uint64_t convert_plain_into(const Genome& g, const Query& q, const std::vector<B
     return k;
 }
 
-void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs, RawView<BulgeRaw> raw,
-                   HitVec& out_all, AuxVec& aux_all) {
-    BatchIndex bi(g, bs);
+void convert_bulge(const Genome& g, const Query& q, const std::vector<Batch>& bs, const BatchIndex& bi,
+                   RawView<BulgeRaw> raw, HitVec& out_all, AuxVec& aux_all) {
     parallel_convert(raw.size(), out_all, aux_all, [&](size_t b0, size_t b1, cof_hit* out, HitAux* aux) -> size_t {
     size_t k = 0;
     for (size_t ri = b0; ri < b1; ri++) {
