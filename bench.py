@@ -321,6 +321,7 @@ def bulge_workload(args):
                       "genome_bp": st.genome_nucl},
            "candidates": st.n_candidates, "bulge_hits": st.bulge_hits, "checksum": st.bulge_checksum,
            "schedule": args.schedule, "tuning": s.tuning(), "pipeline_rounds": st.n_pipeline_rounds,
+           "graph_instantiations_last_step": st.graph_instantiations, "graph_updates_last_step": st.graph_updates,
            "kernel_ms_filter": round(filt / args.steps, 1), "kernel_ms_verify": round(ver / args.steps, 1)}
     s.close()
     if not args.no_cpu_baseline:

@@ -79,7 +79,8 @@ class CofStats(C.Structure):
                 ("verify_steps", C.c_uint64), ("verify_slots", C.c_uint64), ("n_pipeline_rounds", C.c_uint64), ("bulge_ms", C.c_double),
                 ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64),
                 ("convert_ms", C.c_double), ("load_ms", C.c_double), ("load_device", C.c_uint64),
-                ("write_ms", C.c_double), ("write_device", C.c_uint64)]
+                ("write_ms", C.c_double), ("write_device", C.c_uint64),
+                ("graph_instantiations", C.c_uint64), ("graph_updates", C.c_uint64)]
 
 
 _lib = None

@@ -988,7 +988,8 @@ __global__ void add_u32_kernel(uint32_t* __restrict__ dst, const uint32_t* __res
 // windows).  The first pass of a search clears the record count, sums and
 // per-tile step counts; a later pass adds to them.
 void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<Batch>& batches,
-                const uint32_t* d_genome, hipStream_t st, cof_stats* stats, const BulgeTune& tune, bool first_pass) {
+                const uint32_t* d_genome, hipStream_t st, cof_stats* stats, const BulgeTune& tune, bool first_pass,
+                uint32_t window_class) {
     const uint32_t flags = tune.flags;
     std::vector<BTile> tiles;
     for (const Batch& b : batches) {
@@ -1294,12 +1295,28 @@ void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector
             }
             for (uint32_t r = R >= 2 ? R - 2 : 0; r < R; r++) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * r + 1], 0));
             if (use_graph) {
+                // The window class's executable graph from the last search is
+                // updated in place from this capture when the pipeline has the
+                // same shape (kernel arguments may differ) and replayed;
+                // otherwise it is instantiated again.
                 HIPCHECK(hipStreamEndCapture(st, &graph));
-                hipGraphExec_t exec = nullptr;
-                HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                hipGraphExec_t& exec = ws.bulge_exec[window_class];
+                bool updated = false;
+                if (exec) {
+                    hipGraphNode_t err_node = nullptr;
+                    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                    updated = hipGraphExecUpdate(exec, graph, &err_node, &res) == hipSuccess &&
+                              res == hipGraphExecUpdateSuccess;
+                    if (!updated) {
+                        (void)hipGetLastError();
+                        HIPCHECK(hipGraphExecDestroy(exec));
+                        exec = nullptr;
+                    }
+                }
+                if (!exec) HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                if (stats) (updated ? stats->graph_updates : stats->graph_instantiations)++;
                 HIPCHECK(hipGraphLaunch(exec, st));
                 HIPCHECK(hipStreamSynchronize(st));
-                HIPCHECK(hipGraphExecDestroy(exec));
                 HIPCHECK(hipGraphDestroy(graph));
             }
             HIPCHECK(hipStreamSynchronize(st));
@@ -1366,18 +1383,24 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
     if (!bp || bp->pats.empty()) return;
     ws.ensure_events();
     std::vector<BulgePlan> views;
+    std::vector<uint32_t> view_class;
+    uint32_t cls = 0;
     for (auto [wide, lo, hi] : {std::make_tuple(false, 0u, kShortCols), std::make_tuple(false, kShortCols + 1, 0xFFFFu),
                                 std::make_tuple(true, 0u, 0xFFFFu)}) {
         BulgePlan v = groups_view(*bp, wide, lo, hi);
-        if (!v.groups32.empty() || !v.groups64.empty()) views.push_back(std::move(v));
+        if (!v.groups32.empty() || !v.groups64.empty()) {
+            views.push_back(std::move(v));
+            view_class.push_back(cls);
+        }
+        cls++;
     }
     if (views.size() == 1) {
-        bulge_pass(bp, ws, g, batches, d_genome, st, stats, tune, true);
+        bulge_pass(bp, ws, g, batches, d_genome, st, stats, tune, true, view_class[0]);
     } else {
         // per-tile candidate counts add up over the passes (each pass clears
         // and refills them)
         for (size_t k = 0; k < views.size(); k++) {
-            bulge_pass(&views[k], ws, g, batches, d_genome, st, stats, tune, k == 0);
+            bulge_pass(&views[k], ws, g, batches, d_genome, st, stats, tune, k == 0, view_class[k]);
             const uint32_t nt = ws.last_tiles;
             if (!nt) continue;
             if (k == 0) {

@@ -71,6 +71,7 @@ struct Workspace {
     uint32_t* round_ctr = nullptr;             // synchronous round: cursor, overflow
     std::vector<hipEvent_t> rev;               // per round: filtered, verified
     hipStream_t st2 = nullptr, st3 = nullptr;  // traceback streams of the pipeline
+    hipGraphExec_t bulge_exec[3] = {};         // per window class: the last pipeline graph, updated in place
     ~Workspace();
     void ensure_events();
     void ensure_round_events(uint32_t rounds);

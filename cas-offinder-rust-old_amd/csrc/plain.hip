@@ -1095,6 +1095,7 @@ Workspace::~Workspace() {
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : rev) if (e) (void)hipEventDestroy(e);
+    for (auto& x : bulge_exec) if (x) (void)hipGraphExecDestroy(x);
     if (st2) (void)hipStreamDestroy(st2);
     if (st3) (void)hipStreamDestroy(st3);
     if (h_plain) (void)hipHostFree(h_plain);

@@ -131,6 +131,9 @@ typedef struct {
     uint64_t load_device;      /* 1 if that load encoded the FASTA on the device */
     double write_ms;           /* wall time of the last cof_write_tsv{,_ex} */
     uint64_t write_device;     /* 1 if those rows were formatted on the device */
+    uint64_t graph_instantiations; /* bulge pipeline graphs instantiated by the last search */
+    uint64_t graph_updates;    /* bulge pipeline graphs replayed after an in-place update (same
+                                  pipeline shape as the previous search of this context) */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes

@@ -83,8 +83,8 @@ def test_bulge_schedules_match_oracle(tmp_path, flags):
 
 
 def test_graph_replays_interleaved_with_eager(tmp_path):
-    # each search builds and launches a fresh graph; eager searches in between
-    # share the context's workspace, counters and queues
+    # each search captures the pipeline and replays the context's graph;
+    # synchronous searches in between share the workspace, counters and queues
     fa, pats = _bulge_genome(tmp_path)
     want = _oracle(tmp_path, fa, pats, 3, 1, 1)
     n = len(pats)
@@ -102,6 +102,28 @@ def test_graph_replays_interleaved_with_eager(tmp_path):
             st = s.stats()
             sums.add((st.bulge_hits, st.bulge_checksum))
     assert len(sums) == 1
+    s.close()
+
+
+def test_graph_updated_in_place_and_replayed(tmp_path):
+    # the first graph search instantiates the pipeline graph; the next ones,
+    # with the same pipeline shape, update it in place and replay it, and an
+    # eager search in between leaves it valid
+    fa, pats = _bulge_genome(tmp_path)
+    want = _oracle(tmp_path, fa, pats, 3, 1, 1)
+    n = len(pats)
+    s = _bulge_search(fa, pats, 0, bulge_first_tiles=8, bulge_round_tiles=16)
+    assert _rows(s, tmp_path) == want
+    st = s.stats()
+    assert st.graph_instantiations >= 1 and st.graph_updates == 0
+    for flags in (0, cofpy.COF_FLAG_NO_GRAPH, 0):
+        s.set_query(pats, [FILT] * n, [3] * n, [1] * n, [1] * n, batch_nucl=1 << 30, flags=flags)
+        assert _rows(s, tmp_path) == want, flags
+        st = s.stats()
+        if flags == 0:
+            assert st.graph_instantiations == 0 and st.graph_updates >= 1
+        else:
+            assert st.graph_instantiations == 0 and st.graph_updates == 0
     s.close()
 
 
