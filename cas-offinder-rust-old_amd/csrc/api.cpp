@@ -235,8 +235,9 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
 
 // Times each candidate shape on a genome prefix and keeps the fastest:
 // plain-scan workgroup size x patterns per workgroup (kernel time of a scan
-// of the prefix), then traceback workgroups per CU (wall time of the bulge
-// path over the prefix's batches).
+// of the prefix), then, on the wall time of the bulge path over the prefix's
+// batches, traceback workgroups per CU, the pipeline's round size and the
+// traceback's lane-refill threshold.
 void autotune(cof_ctx* ctx, uint64_t sample) {
     const uint64_t n = std::min<uint64_t>(sample, ctx->g.n_nibbles());
     if (!cof::plain_plan_empty(ctx->plain)) {
@@ -263,17 +264,37 @@ void autotune(cof_ctx* ctx, uint64_t sample) {
         size_t nc = 0;
         while (nc < ctx->g.chunks.size() && ctx->g.chunks[nc].nib_begin() < n) nc++;
         std::vector<cof::Batch> sb = cof::make_batches(std::max<size_t>(nc, 1), ctx->q.batch_nucl);
+        // coordinate descent: traceback workgroups per CU, then the pipeline's
+        // round size (filter tiles of 4 Ki ends per round; 0 sizes rounds from
+        // the first round's candidate density), then the lane-refill threshold
         cof::RawView<cof::BulgeRaw> raw;
-        double best = 1e300;
         cof::BulgeTune best_t = ctx->bulge_tune;
-        for (uint32_t v : {4u, 8u, 12u, 16u, 24u}) {
-            cof::BulgeTune t = ctx->bulge_tune;
-            t.verify_per_cu = v;
-            t.flags = ctx->q.flags;
+        best_t.flags = ctx->q.flags;
+        auto time_of = [&](const cof::BulgeTune& t) {
             cof_stats st{};
             cof::bulge_run(ctx->bulge, ctx->ws, ctx->g, sb, ctx->d_genome, ctx->stream, raw, &st, t);
-            if (st.bulge_ms < best) { best = st.bulge_ms; best_t.verify_per_cu = v; }
+            return st.bulge_ms;
+        };
+        double best = 1e300;
+        for (uint32_t v : {4u, 8u, 12u, 16u, 24u}) {
+            cof::BulgeTune t = best_t;
+            t.verify_per_cu = v;
+            const double ms = time_of(t);
+            if (ms < best) { best = ms; best_t = t; }
         }
+        for (uint32_t rt : {256u, 1024u, 4096u}) {
+            cof::BulgeTune t = best_t;
+            t.round_tiles = rt;
+            const double ms = time_of(t);
+            if (ms < best) { best = ms; best_t = t; }
+        }
+        for (uint32_t rf : {4u, 16u}) {
+            cof::BulgeTune t = best_t;
+            t.refill = rf;
+            const double ms = time_of(t);
+            if (ms < best) { best = ms; best_t = t; }
+        }
+        best_t.flags = 0;
         ctx->bulge_tune = best_t;
     }
     ctx->cached = false;

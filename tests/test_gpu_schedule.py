@@ -189,7 +189,10 @@ def test_autotune_keeps_results(tmp_path):
     bwant = _oracle(tmp_path, fa, bpats, 3, 1, 1)
     s = _bulge_search(fa, bpats)
     t = s.autotune(1 << 20)
-    assert 1 <= t["verify_blocks_per_cu"] <= 64
+    assert t["verify_blocks_per_cu"] in (4, 8, 12, 16, 24)
+    assert t["bulge_round_tiles"] in (0, 256, 1024, 4096)
+    assert t["verify_refill"] in (4, 8, 16)
+    assert s.tuning() == t
     assert _rows(s, tmp_path) == bwant
     s.close()
 
