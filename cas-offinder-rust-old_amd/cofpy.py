@@ -37,7 +37,8 @@ COF_MLP_EAGER = 1
 class CofTuning(C.Structure):
     _fields_ = [("plain_threads", C.c_uint32), ("plain_pat_chunk", C.c_uint32), ("bulge_first_tiles", C.c_uint32),
                 ("bulge_round_tiles", C.c_uint32), ("verify_blocks_per_cu", C.c_uint32),
-                ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32), ("plain_hit_cap", C.c_uint32)]
+                ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32), ("plain_hit_cap", C.c_uint32),
+                ("plain_queue", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

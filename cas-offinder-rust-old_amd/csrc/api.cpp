@@ -199,6 +199,7 @@ cof_tuning get_tuning(const cof_ctx* ctx) {
     t.bulge_queue_cap = ctx->bulge_tune.queue_cap;
     t.verify_refill = ctx->bulge_tune.refill;
     t.plain_hit_cap = ctx->plain_tune.hit_cap;
+    t.plain_queue = ctx->plain_tune.queue;
     return t;
 }
 
@@ -213,6 +214,8 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
     pt.pat_chunk = t.plain_pat_chunk;
     if (t.plain_hit_cap % 64) throw cof::Error(COF_E_INVALID, "plain_hit_cap must be a multiple of 64");
     pt.hit_cap = t.plain_hit_cap;
+    if (t.plain_queue > 1) throw cof::Error(COF_E_INVALID, "plain_queue must be 0 or 1");
+    pt.queue = t.plain_queue;
     if (t.bulge_first_tiles) bt.first_tiles = t.bulge_first_tiles;
     bt.round_tiles = t.bulge_round_tiles;
     if (t.verify_blocks_per_cu) {
@@ -234,8 +237,8 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
 }
 
 // Times each candidate shape on a genome prefix and keeps the fastest:
-// plain-scan workgroup size x patterns per workgroup (kernel time of a scan
-// of the prefix), then, on the wall time of the bulge path over the prefix's
+// plain-scan workgroup size x patterns per workgroup, and the persistent
+// tile-queue scan (kernel time of a scan of the prefix), then, on the wall time of the bulge path over the prefix's
 // batches, traceback workgroups per CU, the pipeline's round size and the
 // traceback's lane-refill threshold.
 void autotune(cof_ctx* ctx, uint64_t sample) {
@@ -245,10 +248,13 @@ void autotune(cof_ctx* ctx, uint64_t sample) {
         double best = 1e300;
         cof::PlainTune best_t = ctx->plain_tune;
         for (uint32_t th : {64u, 128u, 256u, 512u}) {
-            for (uint32_t pc : {0u, 16u, 64u}) {
+            for (uint32_t pc : {0u, 16u, 64u, 1000u}) {
+                // pc 1000: the persistent tile-queue scan (64-thread tiles, whole groups)
+                if (pc == 1000 && th != 64) continue;
                 cof::PlainTune t;
                 t.threads = th;
-                t.pat_chunk = pc;
+                t.pat_chunk = pc == 1000 ? 0u : pc;
+                t.queue = pc == 1000 ? 1u : 0u;
                 double ms = 0, warm = 0;
                 uint64_t nl = 0;
                 cof::plain_run(ctx->plain, ctx->ws, ctx->d_genome, n, ctx->stream, raw, &warm, &nl, t);

@@ -16,6 +16,7 @@ struct Workspace {
     uint32_t plain_cap = 0;
     uint32_t plain_cap_tuned = 0;  // the PlainTune::hit_cap the buffer was last sized to
     uint32_t* plain_count = nullptr;   // per-segment counters, 128 bytes apart
+    uint32_t* plain_heads = nullptr;   // per-XCD tile queue heads of the persistent scan, 128 bytes apart
     void* plain_out2 = nullptr;    // the segments packed together
     uint32_t plain_cap2 = 0;
     uint32_t* h_counts = nullptr;  // pinned copy of the segment counters
@@ -90,6 +91,7 @@ struct PlainTune {
     uint32_t threads = 64;      // workgroup size; a workgroup scans 64 starts per thread
     uint32_t pat_chunk = 0;     // patterns per workgroup (grid.y slices a group); 0 = whole group
     uint32_t hit_cap = 0;       // hit records the device buffer starts with (0: 2^20, grown on overflow)
+    uint32_t queue = 0;         // 1: persistent workgroups draining per-XCD tile queues (64 threads, whole groups)
 };
 
 // Schedule of the bulge path (results do not depend on it).

@@ -239,10 +239,11 @@ __device__ __forceinline__ mf_v4f mfma_fp4(const mf_v8i& a, const mf_v8i& b) {
 // One chunk of NB pattern blocks (held in registers) against every window of
 // the list: per 16 windows, NB independent MFMAs and an AND of their score
 // signs; the hit path runs only when some score is positive.
-template <int NB, int THREADS>
+template <int NB, int THREADS, bool kPre>
 __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand,
                                            uint32_t nc, const GroupDev& G, uint32_t bc,
-                                           const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
+                                           const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8],
+                                           const PlaneRec* __restrict__ prec,
                                            uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
                                            uint32_t lane, uint32_t wave) {
     const uint32_t lg = lane >> 4, r = lane & 15u;
@@ -250,8 +251,12 @@ __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand
     mf_v4i bv[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
-        bv[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
+        if constexpr (kPre) {
+            bv[b] = pre[b];   // the group's blocks, held by the caller (bc == 0)
+        } else {
+            const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
+            bv[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
+        }
     }
     // The fragments arrive here (an empty asm that reads them): otherwise the
     // loop waits for them on every iteration, and so also for the hit records
@@ -316,11 +321,13 @@ __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand
     }
 }
 
-// The group's pattern blocks [pb, pe) in chunks of up to 8 blocks.
-template <int THREADS>
+// The group's pattern blocks [pb, pe) in chunks of up to 8 blocks (kPre: the
+// group has at most 8 blocks, all in pre).
+template <int THREADS, bool kPre>
 __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* cand,
                                              uint32_t nc, const GroupDev& G, uint32_t pb, uint32_t pe,
-                                             const uint4* __restrict__ mfrag, const PlaneRec* __restrict__ prec,
+                                             const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8],
+                                             const PlaneRec* __restrict__ prec,
                                              uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
                                              uint32_t lane, uint32_t wave) {
     const uint32_t blk_lo = (pb - G.pat_begin) >> 4;
@@ -329,7 +336,8 @@ __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* ca
         const uint32_t nb = min(8u, blk_hi - bc);   // uniform
 #define COF_MFMA_CHUNK(N)                                                                                         \
     case N:                                                                                                       \
-        mfma_chunk<N, THREADS>(gp, cand, nc, G, bc, mfrag, prec, s0, out, out_count, out_cap, lane, wave);       \
+        mfma_chunk<N, THREADS, kPre>(gp, cand, nc, G, bc, mfrag, pre, prec, s0, out, out_count, out_cap, lane, \
+                                     wave);                                                                        \
         break;
         switch (nb) {
             COF_MFMA_CHUNK(1) COF_MFMA_CHUNK(2) COF_MFMA_CHUNK(3) COF_MFMA_CHUNK(4)
@@ -337,6 +345,309 @@ __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* ca
             default: break;
         }
 #undef COF_MFMA_CHUNK
+    }
+}
+
+// One pattern group over one tile held in LDS (plane words gp, bit4 words gw
+// for flagged tiles), with a window list of half a tile in cand: the PAM
+// test, the list, and the compare, by a workgroup of THREADS threads
+// (tid < THREADS; wave = tid / 64).  kPlainTile: the tile holds plain bases
+// only (its planes serve); kPre: the group's matrix-core fragments (at most 8
+// blocks) are already in registers (pre).
+template <int W, int THREADS, bool kPlainTile, bool kPre>
+__device__ __forceinline__ void scan_group(
+    const GroupDev& G, uint32_t pat_chunk, uint32_t chunk_row, const uint2* gp, const uint32_t* gw, uint16_t* cand,
+    uint32_t* wave_sum, uint32_t tid, uint32_t lane, uint32_t wave, uint64_t s0, uint64_t scan_end, bool tail,
+    int iupac, const PatRec* __restrict__ pats, const PlaneRec* __restrict__ prec,
+    const uint32_t* __restrict__ full_words, const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8], PlainRaw* out,
+    uint32_t* out_count, uint32_t out_cap) {
+    constexpr int kThreads = THREADS;
+    constexpr int kTile = THREADS * 64;
+    uint32_t pb = G.pat_begin, pe = G.pat_begin + G.pat_count;
+    if (pat_chunk) {
+        const uint32_t lo = chunk_row * pat_chunk;   // uniform over the workgroup
+        if (lo >= G.pat_count) return;
+        pb += lo;
+        pe = min(pe, pb + pat_chunk);
+    }
+    // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
+    uint32_t m8[8];
+    if constexpr (kPlainTile) {
+        // planes: per PAM cell and 32 starts, two bitop3 selects by the l
+        // plane, one by the h plane and an AND (pam_cell2)
+        uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+        for (uint32_t k = 0; k < G.n_pam; k++) {
+            const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
+            const uint32_t qo = pk >> 5, sh = pk & 31u;
+            const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
+            const uint2 a0 = gp[q0], a1 = gp[q0 + 1], b0 = gp[q1], b1 = gp[q1 + 1];
+            const uint32_t h0 = __builtin_amdgcn_alignbit(a1.x, a0.x, sh);
+            const uint32_t l0 = __builtin_amdgcn_alignbit(a1.y, a0.y, sh);
+            const uint32_t h1 = __builtin_amdgcn_alignbit(b1.x, b0.x, sh);
+            const uint32_t l1 = __builtin_amdgcn_alignbit(b1.y, b0.y, sh);
+            pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
+        }
+        m8[0] = ok[0];
+        m8[1] = ok[1];
+        if (tail) {
+#pragma unroll
+            for (int w = 0; w < 2; w++) {
+                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
+                if (scan_end <= first) m8[w] = 0;
+                else if (scan_end < first + 32) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
+            }
+        }
+#pragma unroll
+        for (int w = 2; w < 8; w++) m8[w] = 0;
+    } else {
+        // bit4, 8 positions per word: nibble j of X_k is the genome at
+        // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
+        // is in the filter's plain-base set iff it overlaps the filter code.
+        uint32_t ok[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
+        for (uint32_t k = 0; k < G.n_pam; k++) {
+            const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
+            const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                const uint32_t q = (uint32_t)w * kThreads + tid + qo;
+                const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
+                const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
+                ok[w] &= nz_nibbles(X & pmask) & ~bad;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < 8; w++) m8[w] = gather_nibble_flags(ok[w]);
+        if (tail) {
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
+                if (scan_end <= first) m8[w] = 0;
+                else if (scan_end < first + 8) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
+            }
+        }
+    }
+    // block-wide exclusive scan of per-thread values: a DPP prefix sum in
+    // the wave, the wave totals through LDS; tot = the block total.  The
+    // sums stay below 2^16, so a value may pack two counts.
+    auto block_scan = [&](uint32_t v, uint32_t& tot, uint32_t& excl) {
+        const uint32_t inc = wave_prefix_sum(v);
+        const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        uint32_t off = 0;
+        tot = wtot;
+        if (kThreads > 64) {
+            if (lane == 0) wave_sum[wave] = wtot;
+            __syncthreads();
+            tot = 0;
+#pragma unroll
+            for (int x = 0; x < kThreads / 64; x++) {
+                uint32_t ws = wave_sum[x];
+                off += (uint32_t)x < wave ? ws : 0u;
+                tot += ws;
+            }
+            __syncthreads();   // wave_sum is written again by the next scan
+        }
+        excl = off + inc - v;
+    };
+    // The window list holds half a tile: the tile's windows are compared
+    // in one pass when they fit, else in two passes over the halves of
+    // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3).
+    // Plain-base tiles list their windows in start order (plane word 0 of
+    // every thread, then word 1), so 16 consecutive entries lie in a few
+    // consecutive LDS words: one scan of both words' counts, packed.
+    uint32_t nc_all, pos_all = 0, tot0 = 0, ex0 = 0, ex1 = 0;
+    if constexpr (kPlainTile) {
+        uint32_t tot, ex;
+        block_scan((uint32_t)__popc(m8[0]) | ((uint32_t)__popc(m8[1]) << 16), tot, ex);
+        tot0 = tot & 0xFFFFu;
+        nc_all = tot0 + (tot >> 16);
+        ex0 = ex & 0xFFFFu;
+        ex1 = ex >> 16;
+    } else {
+        uint32_t cnt_all = 0;
+#pragma unroll
+        for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
+        block_scan(cnt_all, nc_all, pos_all);
+    }
+    const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
+    for (uint32_t h = 0; h < npass; h++) {
+        uint32_t nc;
+        if constexpr (kPlainTile) {
+            // word 0's windows from 0, word 1's after them (one pass) or
+            // from 0 (second pass)
+            nc = npass == 1 ? nc_all : h == 0 ? tot0 : nc_all - tot0;
+            uint32_t m = (npass == 1 || h == 0) ? m8[0] : 0u;
+            uint32_t pos = ex0;
+            while (m) {
+                const uint32_t j = (uint32_t)__builtin_ctz(m);
+                cand[pos++] = (uint16_t)(tid * 32u + j);
+                m &= m - 1u;
+            }
+            m = (npass == 1 || h == 1) ? m8[1] : 0u;
+            pos = (npass == 1 ? tot0 : 0u) + ex1;
+            while (m) {
+                const uint32_t j = (uint32_t)__builtin_ctz(m);
+                cand[pos++] = (uint16_t)(((uint32_t)kThreads + tid) * 32u + j);
+                m &= m - 1u;
+            }
+        } else {
+            uint32_t mh[8];
+#pragma unroll
+            for (int w = 0; w < 8; w++) mh[w] = (npass == 1 || ((uint32_t)w >> 2) == h) ? m8[w] : 0u;
+            uint32_t pos = pos_all;
+            nc = nc_all;
+            if (npass == 2) {
+                uint32_t cnt = 0;
+#pragma unroll
+                for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
+                block_scan(cnt, nc, pos);
+            }
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                uint32_t m = mh[w];
+                while (m) {
+                    uint32_t j = (uint32_t)__builtin_ctz(m);
+                    cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
+                    m &= m - 1u;
+                }
+            }
+        }
+        const bool planes = kPlainTile && G.planes;   // uniform
+        const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
+        // the matrix-core compare reads whole blocks of 16 list entries: pad
+        // the list with start 0 (outputs of rows past the list are dropped)
+        if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
+        tile_sync<kThreads>();
+        // 3. compare windows against the group's patterns
+        if (mfma) {
+            mfma_compare<kThreads, kPre>(gp, cand, nc, G, pb, pe, mfrag, pre, prec, s0, out, out_count, out_cap,
+                                         lane, wave);
+            tile_sync<kThreads>();
+            continue;
+        }
+        for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
+            uint32_t c = c0 + lane;
+            bool active = c < nc;
+            uint32_t i = active ? cand[c] : 0;
+            // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
+            auto bit4_compare = [&](bool act) {
+                uint32_t q = i + G.k_lo;
+                uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
+                uint32_t win[W];
+#pragma unroll
+                for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
+                // four patterns per iteration: their records are fetched with
+                // one batch of scalar loads (pointer offsets, no per-pattern
+                // address arithmetic), and one branch covers the four hits
+                uint32_t p = pb;
+                for (; p + 4 <= pe; p += 4) {
+                    const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
+                    uint32_t hit = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        uint32_t sum = 0;
+#pragma unroll
+                        for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
+                        hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
+                    }
+                    if (act && hit) {
+                        for (int u = 0; u < 4; u++) {
+                            if (!((hit >> u) & 1u)) continue;
+                            const PatRec& r = pats[p + u];
+                            uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
+                            uint32_t idx = atomicAdd(out_count, 1u);
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                        }
+                    }
+                }
+                for (; p < pe; p++) {
+                    const PatRec& r = pats[p];
+                    uint32_t sum = 0;
+#pragma unroll
+                    for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
+                    if (act && (int32_t)sum >= r.thr) {
+                        uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
+                        uint32_t idx = atomicAdd(out_count, 1u);
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                    }
+                }
+            };
+            if (planes) {
+                // plain bases only: kmm = window positions whose planes differ
+                // (positions that are N in every pattern match any base)
+                const uint32_t q = i + G.pk_lo;
+                const uint2 w0 = gp[q >> 5], w1 = gp[(q >> 5) + 1];
+                const uint32_t gh = __builtin_amdgcn_alignbit(w1.x, w0.x, q & 31u) & G.wmask;
+                const uint32_t gl = __builtin_amdgcn_alignbit(w1.y, w0.y, q & 31u) & G.wmask;
+                const bool act = active;
+                const uint64_t act_mask = __ballot(act);
+                // plane records of patterns [pb, pe); a whole group runs on
+                // its padded record count
+                uint32_t p = G.prec_begin + (pb - G.pat_begin);
+                const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
+                                              : G.prec_begin + ((G.pat_count + 7u) & ~7u);
+                // NB patterns per batch: per pattern xor, bitop3, v_bcnt
+                // accumulating K and half a v_min3, one compare per batch; all
+                // NB records are loaded before the batch computes
+                auto batch = [&](auto nb_tag) {
+                    constexpr int NB = decltype(nb_tag)::value;
+                    const PlaneRec* const rp = prec + p;   // uniform base
+                    uint32_t mn = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int u = 0; u < NB; u += 2) {
+                        const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
+                        const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
+                        mn = min(mn, min(t0, t1));
+                    }
+                    if (__ballot(mn <= 64u) & act_mask) {
+                        for (int u = 0; u < NB; u++) {
+                            const uint32_t amm = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl));
+                            if (!(act && amm + rp[u].K <= 64u)) continue;
+                            uint32_t idx = atomicAdd(out_count, 1u);
+                            if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
+                        }
+                    }
+                    p += NB;
+                };
+                // batches of 16 while more than 8 records remain (a whole
+                // group may run into its fillers, a pattern slice may not),
+                // then at most one batch of 8, then single records
+                const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
+                while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
+                if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
+                for (; p < qe; p++) {
+                    const PlaneRec& r = prec[p];
+                    const uint32_t amm = (uint32_t)__popc((gh ^ r.ph) | (gl ^ r.pl));
+                    if (act && amm + r.K <= 64u) {
+                        uint32_t idx = atomicAdd(out_count, 1u);
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
+                    }
+                }
+            } else if (!iupac) {
+                bit4_compare(active);
+            } else {
+                for (uint32_t p = pb; p < pe; p++) {
+                    const uint32_t* fw = full_words + (size_t)p * 8;
+                    int32_t pop = 0;
+                    uint32_t amm = 0;
+                    for (uint32_t k = 0; k < G.L; k++) {
+                        uint32_t q = i + k;
+                        uint32_t g = nib_of(gw[q >> 3], q & 7u);
+                        uint32_t pb = nib_of(fw[k >> 3], k & 7u);
+                        pop += __popc(pb & g);
+                        bool single = g != 0 && (g & (g - 1)) == 0;
+                        amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
+                    }
+                    uint32_t M = pats[p].M;
+                    if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
+                        uint32_t idx = atomicAdd(out_count, 1u);
+                        if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
+                    }
+                }
+            }
+        }
+        tile_sync<kThreads>();
     }
 }
 
@@ -462,305 +773,185 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     // starts each.  Consecutive lanes read consecutive LDS words (no bank
     // conflicts).
 
+    // only the tile holding scan_end masks starts past it
+    const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
     // The group loop, compiled once for tiles of plain bases (plane PAM test,
     // matrix-core or plane compare) and once for flagged tiles (bit4 words),
-    // so neither copy carries the other's set-up.  Only the tile holding
-    // scan_end masks starts past it.
-    const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
-    auto run_groups = [&](auto plain_tag) {
-        constexpr bool kPlainTile = decltype(plain_tag)::value;
-        for (uint32_t gi = 0; gi < n_groups; gi++) {
-            const GroupDev& G = groups[gi];
-            uint32_t pb = G.pat_begin, pe = G.pat_begin + G.pat_count;
-            if (pat_chunk) {
-                const uint32_t lo = blockIdx.y * pat_chunk;   // uniform over the workgroup
-                if (lo >= G.pat_count) continue;
-                pb += lo;
-                pe = min(pe, pb + pat_chunk);
-            }
-            // 2. PAM test.  m8[w]: pass bits of this thread's starts in word w.
-            uint32_t m8[8];
-            if constexpr (kPlainTile) {
-                // planes: per PAM cell and 32 starts, two bitop3 selects by the l
-                // plane, one by the h plane and an AND (pam_cell2)
-                uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
-                for (uint32_t k = 0; k < G.n_pam; k++) {
-                    const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
-                    const uint32_t qo = pk >> 5, sh = pk & 31u;
-                    const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
-                    const uint2 a0 = gp[q0], a1 = gp[q0 + 1], b0 = gp[q1], b1 = gp[q1 + 1];
-                    const uint32_t h0 = __builtin_amdgcn_alignbit(a1.x, a0.x, sh);
-                    const uint32_t l0 = __builtin_amdgcn_alignbit(a1.y, a0.y, sh);
-                    const uint32_t h1 = __builtin_amdgcn_alignbit(b1.x, b0.x, sh);
-                    const uint32_t l1 = __builtin_amdgcn_alignbit(b1.y, b0.y, sh);
-                    pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
-                }
-                    m8[0] = ok[0];
-                m8[1] = ok[1];
-                if (tail) {
+    // so neither copy carries the other's set-up.
+    const mf_v4i no_pre[8] = {};
+    if (flags == 0) {
+        for (uint32_t gi = 0; gi < n_groups; gi++)
+            scan_group<W, THREADS, true, false>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid, lane,
+                                                wave, s0, scan_end, tail, iupac, pats, prec, full_words, mfrag,
+                                                no_pre, out, out_count, out_cap);
+    } else {
+        for (uint32_t gi = 0; gi < n_groups; gi++)
+            scan_group<W, THREADS, false, false>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid,
+                                                 lane, wave, s0, scan_end, tail, iupac, pats, prec, full_words, mfrag,
+                                                 no_pre, out, out_count, out_cap);
+    }
+}
+
+// Persistent variant of the scan: workgroups of two waves stay resident and
+// claim runs of 4,096-start tiles from 8 queues, one per XCD, each owning a
+// contiguous eighth of the tiles (a workgroup drains its own XCD's range
+// first, so consecutive tiles and their halos share that XCD's L2, then
+// the others').  The two waves load a tile's planes (and bit4 words when
+// needed) together and then scan the groups alternately, each with its own
+// window list; when the launch has one matrix-core group per wave, each wave
+// holds its group's pattern fragments in registers for the whole kernel.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    // HW_REG_XCC_ID (hardware register 20), bits 3:0: the XCD this wave runs on
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1);
+}
+
+// Tiles a workgroup claims at once (one queue atomic per claim).
+constexpr uint32_t kClaimTiles = 16;
+
+// The workgroup's next run of up to kClaimTiles tiles [first, end), or
+// first == n when every queue is drained.
+__device__ __forceinline__ uint2 claim_tiles(uint32_t* heads, uint32_t n) {
+    const uint32_t x0 = xcc_id();
+    for (uint32_t k = 0; k < kXcds; k++) {
+        const uint32_t x = (x0 + k) & (kXcds - 1);
+        const uint32_t lo = (uint32_t)((uint64_t)n * x / kXcds), hi = (uint32_t)((uint64_t)n * (x + 1) / kXcds);
+        uint32_t* const h = heads + x * kSegStride;
+        if (lo >= hi || __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi - lo) continue;
+        const uint32_t i = atomicAdd(h, kClaimTiles);
+        if (i < hi - lo) return make_uint2(lo + i, lo + min(i + kClaimTiles, hi - lo));
+    }
+    return make_uint2(n, n);
+}
+
+template <int W>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) void plain_queue_kernel(
+    const uint32_t* __restrict__ genome, uint64_t scan_end, uint32_t n_tiles, const GroupDev* __restrict__ groups,
+    uint32_t n_groups, const PatRec* __restrict__ pats, const PlaneRec* __restrict__ prec,
+    const uint32_t* __restrict__ full_words, const uint4* __restrict__ mfrag, const uint32_t* __restrict__ img_hi,
+    const uint32_t* __restrict__ img_lo, const uint8_t* __restrict__ img_flags, PlainRaw* __restrict__ out_all,
+    uint32_t* __restrict__ counts, uint32_t seg_cap, uint32_t bit4_all, uint32_t* __restrict__ heads) {
+    constexpr int kThreads = 128;
+    constexpr int kTile = 64 * 64;
+    constexpr int kTileWords = kTile / 8 + kHaloWords;
+    constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
+    struct alignas(16) QueueLds {
+        uint2 gp[kPlaneWords];          // (hi, lo) plane words at offset 0
+        uint32_t gw[kTileWords];        // bit4 words (flagged tiles, bit4 groups)
+        uint16_t cand[2][kTile / 2];    // each wave's window list
+        uint32_t tile, tile_end;        // the claimed run of tiles
+    };
+    __shared__ QueueLds lds;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    uint32_t* const out_count = counts + (blockIdx.x & (kHitSegs - 1)) * kSegStride;
+    PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
+    uint16_t* const cand = lds.cand[wave];
+    // one matrix-core group per wave, in one pass of at most 8 blocks: its
+    // fragments stay in registers
+    constexpr bool resident = false;
+    mf_v4i pre[8] = {};
+    if (resident) {
+        const GroupDev& G = groups[wave];
+        const uint32_t nblk = (G.pat_count + 15u) >> 4;
 #pragma unroll
-                    for (int w = 0; w < 2; w++) {
-                        const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
-                        if (scan_end <= first) m8[w] = 0;
-                        else if (scan_end < first + 32) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
-                    }
-                }
-    #pragma unroll
-                for (int w = 2; w < 8; w++) m8[w] = 0;
-            } else {
-                // bit4, 8 positions per word: nibble j of X_k is the genome at
-                // start 8t+j+k; with N masked to 0 and no IUPAC letters, a nibble
-                // is in the filter's plain-base set iff it overlaps the filter code.
-                uint32_t ok[8];
-    #pragma unroll
-                for (int w = 0; w < 8; w++) ok[w] = 0x88888888u;
-                for (uint32_t k = 0; k < G.n_pam; k++) {
-                    const uint32_t pk = G.pam_pos[k], pmask = G.pam_mask[k];   // uniform
-                    const uint32_t qo = pk >> 3, sh = (pk & 7u) * 4u;
-    #pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        const uint32_t q = (uint32_t)w * kThreads + tid + qo;
-                        const uint32_t X = __builtin_amdgcn_alignbit(gw[q + 1], gw[q], sh);
-                        const uint32_t bad = iupac ? multi_nibbles(X) : 0u;
-                        ok[w] &= nz_nibbles(X & pmask) & ~bad;
-                    }
-                }
-    #pragma unroll
-                for (int w = 0; w < 8; w++) m8[w] = gather_nibble_flags(ok[w]);
-                if (tail) {
-#pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 8u;
-                        if (scan_end <= first) m8[w] = 0;
-                        else if (scan_end < first + 8) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
-                    }
-                }
-            }
-            // block-wide exclusive scan of per-thread values: a DPP prefix sum in
-            // the wave, the wave totals through LDS; tot = the block total.  The
-            // sums stay below 2^16, so a value may pack two counts.
-            auto block_scan = [&](uint32_t v, uint32_t& tot, uint32_t& excl) {
-                const uint32_t inc = wave_prefix_sum(v);
-                const uint32_t wtot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-                uint32_t off = 0;
-                tot = wtot;
-                if (kThreads > 64) {
-                    if (lane == 0) wave_sum[wave] = wtot;
-                    __syncthreads();
-                    tot = 0;
-    #pragma unroll
-                    for (int x = 0; x < kThreads / 64; x++) {
-                        uint32_t ws = wave_sum[x];
-                        off += (uint32_t)x < wave ? ws : 0u;
-                        tot += ws;
-                    }
-                    __syncthreads();   // wave_sum is written again by the next scan
-                }
-                excl = off + inc - v;
-            };
-            // The window list holds half a tile: the tile's windows are compared
-            // in one pass when they fit, else in two passes over the halves of
-            // its starts (plane word h of each thread, bit4 words 4h .. 4h + 3).
-            // Plain-base tiles list their windows in start order (plane word 0 of
-            // every thread, then word 1), so 16 consecutive entries lie in a few
-            // consecutive LDS words: one scan of both words' counts, packed.
-            uint32_t nc_all, pos_all = 0, tot0 = 0, ex0 = 0, ex1 = 0;
-            if constexpr (kPlainTile) {
-                uint32_t tot, ex;
-                block_scan((uint32_t)__popc(m8[0]) | ((uint32_t)__popc(m8[1]) << 16), tot, ex);
-                tot0 = tot & 0xFFFFu;
-                nc_all = tot0 + (tot >> 16);
-                ex0 = ex & 0xFFFFu;
-                ex1 = ex >> 16;
-            } else {
-                uint32_t cnt_all = 0;
-    #pragma unroll
-                for (int w = 0; w < 8; w++) cnt_all += (uint32_t)__popc(m8[w]);
-                block_scan(cnt_all, nc_all, pos_all);
-            }
-            const uint32_t npass = nc_all <= (uint32_t)kTile / 2 ? 1u : 2u;   // uniform
-            for (uint32_t h = 0; h < npass; h++) {
-                uint32_t nc;
-                if constexpr (kPlainTile) {
-                    // word 0's windows from 0, word 1's after them (one pass) or
-                    // from 0 (second pass)
-                    nc = npass == 1 ? nc_all : h == 0 ? tot0 : nc_all - tot0;
-                    uint32_t m = (npass == 1 || h == 0) ? m8[0] : 0u;
-                    uint32_t pos = ex0;
-                    while (m) {
-                        const uint32_t j = (uint32_t)__builtin_ctz(m);
-                        cand[pos++] = (uint16_t)(tid * 32u + j);
-                        m &= m - 1u;
-                    }
-                    m = (npass == 1 || h == 1) ? m8[1] : 0u;
-                    pos = (npass == 1 ? tot0 : 0u) + ex1;
-                    while (m) {
-                        const uint32_t j = (uint32_t)__builtin_ctz(m);
-                        cand[pos++] = (uint16_t)(((uint32_t)kThreads + tid) * 32u + j);
-                        m &= m - 1u;
-                    }
-                } else {
-                    uint32_t mh[8];
-    #pragma unroll
-                    for (int w = 0; w < 8; w++) mh[w] = (npass == 1 || ((uint32_t)w >> 2) == h) ? m8[w] : 0u;
-                    uint32_t pos = pos_all;
-                    nc = nc_all;
-                    if (npass == 2) {
-                        uint32_t cnt = 0;
-    #pragma unroll
-                        for (int w = 0; w < 8; w++) cnt += (uint32_t)__popc(mh[w]);
-                        block_scan(cnt, nc, pos);
-                    }
-    #pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        uint32_t m = mh[w];
-                        while (m) {
-                            uint32_t j = (uint32_t)__builtin_ctz(m);
-                            cand[pos++] = (uint16_t)(((uint32_t)w * kThreads + tid) * 8u + j);
-                            m &= m - 1u;
-                        }
-                    }
-                }
-                const bool planes = kPlainTile && G.planes;   // uniform
-                const bool mfma = planes && G.mfma && (pat_chunk & 15u) == 0;
-                // the matrix-core compare reads whole blocks of 16 list entries: pad
-                // the list with start 0 (outputs of rows past the list are dropped)
-                if (mfma && tid < ((16u - (nc & 15u)) & 15u)) cand[nc + tid] = 0;
-                tile_sync<kThreads>();
-                // 3. compare windows against the group's patterns
-                if (mfma) {
-                    mfma_compare<kThreads>(gp, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane,
-                                           wave);
-                    tile_sync<kThreads>();
-                    continue;
-                }
-                for (uint32_t c0 = wave * 64; c0 < nc; c0 += kThreads) {
-                    uint32_t c = c0 + lane;
-                    bool active = c < nc;
-                    uint32_t i = active ? cand[c] : 0;
-                    // bit4 compare of the lanes with `act`: W AND + v_bcnt per pattern
-                    auto bit4_compare = [&](bool act) {
-                        uint32_t q = i + G.k_lo;
-                        uint32_t wi = q >> 3, sh = (q & 7u) * 4u;
-                        uint32_t win[W];
-    #pragma unroll
-                        for (int w = 0; w < W; w++) win[w] = __builtin_amdgcn_alignbit(gw[wi + w + 1], gw[wi + w], sh);
-                        // four patterns per iteration: their records are fetched with
-                        // one batch of scalar loads (pointer offsets, no per-pattern
-                        // address arithmetic), and one branch covers the four hits
-                        uint32_t p = pb;
-                        for (; p + 4 <= pe; p += 4) {
-                            const PatRec* const rp = pats + p;   // uniform base, immediate offsets below
-                            uint32_t hit = 0;
-    #pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                uint32_t sum = 0;
-    #pragma unroll
-                                for (int w = 0; w < W; w++) sum += __popc(win[w] & rp[u].w[w]);
-                                hit |= ((int32_t)sum >= rp[u].thr ? 1u : 0u) << u;
-                            }
-                            if (act && hit) {
-                                for (int u = 0; u < 4; u++) {
-                                    if (!((hit >> u) & 1u)) continue;
-                                    const PatRec& r = pats[p + u];
-                                    uint32_t amm = recount_swar(gw, i, full_words + (size_t)(p + u) * 8, G.L);
-                                    uint32_t idx = atomicAdd(out_count, 1u);
-                                    if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                                }
-                            }
-                        }
-                        for (; p < pe; p++) {
-                            const PatRec& r = pats[p];
-                            uint32_t sum = 0;
-    #pragma unroll
-                            for (int w = 0; w < W; w++) sum += __popc(win[w] & r.w[w]);
-                            if (act && (int32_t)sum >= r.thr) {
-                                uint32_t amm = recount_swar(gw, i, full_words + (size_t)p * 8, G.L);
-                                uint32_t idx = atomicAdd(out_count, 1u);
-                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                            }
-                        }
-                    };
-                    if (planes) {
-                        // plain bases only: kmm = window positions whose planes differ
-                        // (positions that are N in every pattern match any base)
-                        const uint32_t q = i + G.pk_lo;
-                        const uint2 w0 = gp[q >> 5], w1 = gp[(q >> 5) + 1];
-                        const uint32_t gh = __builtin_amdgcn_alignbit(w1.x, w0.x, q & 31u) & G.wmask;
-                        const uint32_t gl = __builtin_amdgcn_alignbit(w1.y, w0.y, q & 31u) & G.wmask;
-                        const bool act = active;
-                        const uint64_t act_mask = __ballot(act);
-                        // plane records of patterns [pb, pe); a whole group runs on
-                        // its padded record count
-                        uint32_t p = G.prec_begin + (pb - G.pat_begin);
-                        const uint32_t qe = pat_chunk ? G.prec_begin + (pe - G.pat_begin)
-                                                      : G.prec_begin + ((G.pat_count + 7u) & ~7u);
-                        // NB patterns per batch: per pattern xor, bitop3, v_bcnt
-                        // accumulating K and half a v_min3, one compare per batch; all
-                        // NB records are loaded before the batch computes
-                        auto batch = [&](auto nb_tag) {
-                            constexpr int NB = decltype(nb_tag)::value;
-                            const PlaneRec* const rp = prec + p;   // uniform base
-                            uint32_t mn = 0xFFFFFFFFu;
-    #pragma unroll
-                            for (int u = 0; u < NB; u += 2) {
-                                const uint32_t t0 = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl)) + rp[u].K;
-                                const uint32_t t1 = (uint32_t)__popc((gh ^ rp[u + 1].ph) | (gl ^ rp[u + 1].pl)) + rp[u + 1].K;
-                                mn = min(mn, min(t0, t1));
-                            }
-                            if (__ballot(mn <= 64u) & act_mask) {
-                                for (int u = 0; u < NB; u++) {
-                                    const uint32_t amm = (uint32_t)__popc((gh ^ rp[u].ph) | (gl ^ rp[u].pl));
-                                    if (!(act && amm + rp[u].K <= 64u)) continue;
-                                    uint32_t idx = atomicAdd(out_count, 1u);
-                                    if (idx < out_cap) out[idx] = PlainRaw{s0 + i, rp[u].pid, amm};
-                                }
-                            }
-                            p += NB;
-                        };
-                        // batches of 16 while more than 8 records remain (a whole
-                        // group may run into its fillers, a pattern slice may not),
-                        // then at most one batch of 8, then single records
-                        const uint32_t q16 = pat_chunk ? qe : min(G.prec_begin + G.prec_padded, qe + 7u);
-                        while (p + 16 <= q16) batch(std::integral_constant<int, 16>{});
-                        if (p + 8 <= qe) batch(std::integral_constant<int, 8>{});
-                        for (; p < qe; p++) {
-                            const PlaneRec& r = prec[p];
-                            const uint32_t amm = (uint32_t)__popc((gh ^ r.ph) | (gl ^ r.pl));
-                            if (act && amm + r.K <= 64u) {
-                                uint32_t idx = atomicAdd(out_count, 1u);
-                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, r.pid, amm};
-                            }
-                        }
-                    } else if (!iupac) {
-                        bit4_compare(active);
-                    } else {
-                        for (uint32_t p = pb; p < pe; p++) {
-                            const uint32_t* fw = full_words + (size_t)p * 8;
-                            int32_t pop = 0;
-                            uint32_t amm = 0;
-                            for (uint32_t k = 0; k < G.L; k++) {
-                                uint32_t q = i + k;
-                                uint32_t g = nib_of(gw[q >> 3], q & 7u);
-                                uint32_t pb = nib_of(fw[k >> 3], k & 7u);
-                                pop += __popc(pb & g);
-                                bool single = g != 0 && (g & (g - 1)) == 0;
-                                amm += (pb != 0xFu && !(single && (g & pb))) ? 1u : 0u;
-                            }
-                            uint32_t M = pats[p].M;
-                            if (active && (int32_t)G.L - pop <= (int32_t)M && amm <= M) {
-                                uint32_t idx = atomicAdd(out_count, 1u);
-                                if (idx < out_cap) out[idx] = PlainRaw{s0 + i, pats[p].pid, amm};
-                            }
-                        }
-                    }
-                }
-                tile_sync<kThreads>();
+        for (int b = 0; b < 8; b++) {
+            if ((uint32_t)b < nblk) {
+                const uint4 v = mfrag[G.frag_begin + (uint32_t)b * 64u + lane];
+                pre[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
             }
         }
-    };
-    if (flags == 0) run_groups(std::true_type{});
-    else run_groups(std::false_type{});
+    }
+    uint32_t t = 0, t_end = 0;   // uniform: the tiles claimed and not yet scanned
+    for (;;) {
+        if (t == t_end) {
+            if (tid == 0) {
+                const uint2 c = claim_tiles(heads, n_tiles);
+                lds.tile = c.x;
+                lds.tile_end = c.y;
+            }
+            __syncthreads();
+            t = __builtin_amdgcn_readfirstlane(lds.tile);
+            t_end = __builtin_amdgcn_readfirstlane(lds.tile_end);
+            __syncthreads();   // read before the next claim writes
+        }
+        if (t >= n_tiles) break;
+        const uint64_t s0 = (uint64_t)t * kTile;
+        const uint64_t w0 = s0 >> 3;
+        // the tile's planes and flags (and bit4 words where needed), as in
+        // plain_scan_kernel
+        constexpr int kPlaneIt = (kPlaneWords + kThreads - 1) / kThreads;
+        uint32_t ph[kPlaneIt], pl[kPlaneIt];
+#pragma unroll
+        for (int it = 0; it < kPlaneIt; it++) {
+            const uint32_t j = tid + (uint32_t)it * kThreads;
+            const bool in = j < (uint32_t)kPlaneWords;
+            ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
+            pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
+        }
+        const int flags = img_flags[s0 >> 12];   // uniform
+        if (bit4_all || flags) {
+            constexpr int kGroups4 = (kTileWords + 3) / 4;
+            constexpr int kLoadIt = (kGroups4 + kThreads - 1) / kThreads;
+            const uint4* const g4 = reinterpret_cast<const uint4*>(genome + w0);
+            uint4 lv[kLoadIt];
+#pragma unroll
+            for (int it = 0; it < kLoadIt; it++) {
+                const uint32_t j = tid + (uint32_t)it * kThreads;
+                lv[it] = j < (uint32_t)kGroups4 ? g4[j] : make_uint4(0u, 0u, 0u, 0u);
+            }
+#pragma unroll
+            for (int it = 0; it < kLoadIt; it++) {
+                const uint32_t j = tid + (uint32_t)it * kThreads;
+                uint32_t raw[4] = {lv[it].x, lv[it].y, lv[it].z, lv[it].w};
+                if (flags) {
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const uint32_t f = raw[b] & (raw[b] >> 1) & (raw[b] >> 2) & (raw[b] >> 3) & 0x11111111u;
+                        raw[b] &= ~((f << 4) - f);
+                    }
+                }
+                if (4 * j + 3 < (uint32_t)kTileWords) {
+                    *reinterpret_cast<uint4*>(&lds.gw[4 * j]) = make_uint4(raw[0], raw[1], raw[2], raw[3]);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        if (4 * j + b < (uint32_t)kTileWords) lds.gw[4 * j + b] = raw[b];
+                }
+            }
+        }
+        if (flags == 0) {
+#pragma unroll
+            for (int it = 0; it < kPlaneIt; it++) {
+                const uint32_t j = tid + (uint32_t)it * kThreads;
+                if (j < (uint32_t)kPlaneWords) lds.gp[j] = make_uint2(ph[it], pl[it]);
+            }
+        }
+        __syncthreads();
+        const int iupac = flags & 1;
+        const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
+        if (flags == 0) {
+            if (resident) {
+                scan_group<W, 64, true, true>(groups[wave], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0,
+                                              scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
+                                              out_count, seg_cap);
+            } else {
+                for (uint32_t gi = wave; gi < n_groups; gi += 2)
+                    scan_group<W, 64, true, false>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0,
+                                                   s0, scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
+                                                   out_count, seg_cap);
+            }
+        } else {
+            for (uint32_t gi = wave; gi < n_groups; gi += 2)
+                scan_group<W, 64, false, false>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0,
+                                                scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
+                                                out_count, seg_cap);
+        }
+        __syncthreads();   // both waves are done with the tile before the next one loads
+        t++;
+    }
 }
 
 // Scan image of the genome, one wave per 4096-start unit u: the flags of
@@ -1083,12 +1274,31 @@ void launch_plain(const PlainTune& t, dim3 grid, hipStream_t st, const uint32_t*
 #undef COF_PLAIN_SHAPE
 }
 
+// The persistent scan: as many two-wave workgroups as stay resident on the
+// device, draining the per-XCD tile queues (heads cleared by the caller).
+template <int W>
+void launch_queue(hipStream_t st, const uint32_t* d_genome, uint64_t scan_end, const GroupDev* g, uint32_t n_groups,
+                  uint32_t bit4_all, const PatRec* pats, const PlaneRec* prec, const uint32_t* full, const uint4* frag,
+                  const Workspace& ws, PlainRaw* out, uint32_t* count, uint32_t cap) {
+    static int per_cu = 0;
+    if (!per_cu) {
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, plain_queue_kernel<W>, 128, 0));
+        per_cu = std::max(per_cu, 1);
+    }
+    const uint64_t n_tiles = (scan_end + 4095) / 4096;
+    if (n_tiles > 0xFFFFFFFFull) throw Error(COF_E_UNSUPPORTED, "genome too large for the tile queues");
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(n_tiles, (uint64_t)per_cu * (uint64_t)std::max(ws.cus, 1));
+    hipLaunchKernelGGL((plain_queue_kernel<W>), dim3(grid), dim3(128), 0, st, d_genome, scan_end, (uint32_t)n_tiles,
+                       g, n_groups, pats, prec, full, frag, ws.img_hi, ws.img_lo, ws.img_flags, out, count, cap,
+                       bit4_all, ws.plain_heads);
+}
+
 }  // namespace
 
 // Scans start positions [0, scan_end) of the resident genome.  Device words
 // past the genome must be zero up to plain_pad_words() beyond the last tile.
 Workspace::~Workspace() {
-    for (void* p : {plain_out, plain_out2, (void*)plain_count, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
+    for (void* p : {plain_out, plain_out2, (void*)plain_count, (void*)plain_heads, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
                     qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
                     (void*)tile_order, (void*)tile_cand, (void*)tile_cand_prev, (void*)tile_steps, (void*)rounds,
                     (void*)round_ctr, (void*)img_hi, (void*)img_lo, (void*)img_flags})
@@ -1126,6 +1336,7 @@ void Workspace::ensure_events() {
     HIPCHECK(hipMalloc(&counters, 4 * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&sums, 8 * sizeof(unsigned long long)));
     HIPCHECK(hipMalloc(&plain_count, kHitSegs * kSegStride * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&plain_heads, kXcds * kSegStride * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&round_ctr, 2 * sizeof(uint32_t)));
     int dev = 0;
     HIPCHECK(hipGetDevice(&dev));
@@ -1215,7 +1426,26 @@ void plain_launch(PlainPlan* pp, Workspace& ws, const uint32_t* d_genome, uint64
     ensure_scan_image(ws, d_genome, std::max(scan_end, ws.genome_nib), st);
     HIPCHECK(hipMemsetAsync(ws.plain_count, 0, kHitSegs * kSegStride * sizeof(uint32_t), st));
     HIPCHECK(hipEventRecord(ws.ev[0], st));
+    const bool use_queue = tune.queue && tune.threads == 64 && tune.pat_chunk == 0;
     for (const auto& l : pp->launches) {
+        if (use_queue) {
+            const GroupDev* g = pp->d_groups + l.group_begin;
+            HIPCHECK(hipMemsetAsync(ws.plain_heads, 0, kXcds * kSegStride * sizeof(uint32_t), st));
+#define COF_QUEUE_LAUNCH(WW)                                                                                    \
+    case WW:                                                                                                    \
+        launch_queue<WW>(st, d_genome, scan_end, g, l.n_groups, l.bit4_all, pp->d_pats, pp->d_prec, pp->d_full, \
+                         pp->d_frag, ws, d_out, ws.plain_count, seg_cap);                                       \
+        break;
+            switch (l.W) {
+                COF_QUEUE_LAUNCH(1) COF_QUEUE_LAUNCH(2) COF_QUEUE_LAUNCH(3) COF_QUEUE_LAUNCH(4)
+                COF_QUEUE_LAUNCH(5) COF_QUEUE_LAUNCH(6) COF_QUEUE_LAUNCH(7) COF_QUEUE_LAUNCH(8)
+                default: throw Error(COF_E_UNSUPPORTED, "compare window too wide");
+            }
+#undef COF_QUEUE_LAUNCH
+            HIPCHECK(hipGetLastError());
+            if (n_launches) (*n_launches)++;
+            continue;
+        }
         // grids of at most 2^20 tiles keep the tile index in 32 bits
         for (uint64_t t0 = 0; t0 < n_tiles; t0 += (1u << 20)) {
             uint32_t nb = (uint32_t)std::min<uint64_t>(n_tiles - t0, 1u << 20);

@@ -159,6 +159,9 @@ typedef struct {
     uint32_t plain_hit_cap;        /* plain-scan hit records the device buffer starts with (a
                                       multiple of 64; 0: 2^20).  A scan whose hits do not fit grows
                                       the buffer and runs again */
+    uint32_t plain_queue;          /* 1: the plain scan runs as persistent two-wave workgroups that
+                                      claim 4,096-start tiles from per-XCD device queues (64-thread
+                                      shape, whole groups only); 0: one workgroup per tile */
 } cof_tuning;
 
 /* Per filter tile of the last bulge search (tiles in genome order). */

@@ -173,6 +173,37 @@ def test_plain_launch_shapes(tmp_path, threads, chunk):
     s.close()
 
 
+def test_plain_queue_kernel(tmp_path):
+    # the persistent scan (two-wave workgroups draining per-XCD tile queues):
+    # one matrix-core group per wave, and a query of four groups in one launch
+    # (two filters of one length; the second set's guides hold an IUPAC letter,
+    # so its groups compare bit4 words) that the waves take alternately; rows
+    # equal the oracle's (the union of one run per filter for the mixed query)
+    fa, pats, want = _plain_case(tmp_path)
+    s = cofpy.Search(pats, [FILT] * len(pats), [4] * len(pats), batch_nucl=1 << 22)
+    s.set_tuning(plain_queue=1)
+    s.load_file(fa)
+    assert s.tuning()["plain_queue"] == 1
+    assert _rows(s, tmp_path) == want
+    s.close()
+    rng = random.Random(5)
+    gfilt = "N" * 20 + "NGG"   # same length as FILT: one launch holds all four groups
+    g1 = ["".join(rng.choice("ACGT") for _ in range(20)) + "NNN" for _ in range(12)]
+    g2 = ["".join(rng.choice("ACGT") for _ in range(19)) + "RNNN" for _ in range(9)]
+    want = set()
+    for filt, gs, m in ((FILT, g1, 3), (gfilt, g2, 4)):
+        in_text = f"{fa}\n{filt}\n" + "".join(f"{p} {m}\n" for p in gs)
+        oracle.run(in_text, tmp_path / "ref.tsv", batch_nucl=1 << 22)
+        want |= set(cofpy.read_rows(tmp_path / "ref.tsv"))
+    s = cofpy.Search(g1 + g2, [FILT] * len(g1) + [gfilt] * len(g2), [3] * len(g1) + [4] * len(g2),
+                     batch_nucl=1 << 22)
+    s.set_tuning(plain_queue=1)
+    s.load_file(fa)
+    got = _rows(s, tmp_path)
+    assert sorted(got) == sorted(want) and len(got) == len(want)
+    s.close()
+
+
 def test_autotune_keeps_results(tmp_path):
     fa, pats, want = _plain_case(tmp_path)
     n = len(pats)
@@ -182,6 +213,7 @@ def test_autotune_keeps_results(tmp_path):
     # which shape wins is a timing outcome; only its validity is checked
     assert t["plain_threads"] in (64, 128, 256, 512)
     assert t["plain_pat_chunk"] in (0, 16, 64)
+    assert t["plain_queue"] in (0, 1)
     assert s.tuning() == t
     assert _rows(s, tmp_path) == want
     s.close()
@@ -200,7 +232,7 @@ def test_autotune_keeps_results(tmp_path):
 def test_invalid_tuning_rejected():
     s = cofpy.Search(["A" * 20 + "NNN"], [FILT], [2])
     for kw in ({"plain_threads": 100}, {"verify_blocks_per_cu": 65}, {"bulge_queue_cap": 10},
-               {"plain_hit_cap": 100}):
+               {"plain_hit_cap": 100}, {"plain_queue": 2}):
         with pytest.raises(cofpy.CofError) as e:
             s.set_tuning(**kw)
         assert e.value.status == cofpy.COF_E_INVALID
