@@ -144,7 +144,7 @@ def summarize_counters(args):
         p["mfma_per_wave"] = round(mf[i]["SQ_INSTS_MFMA"] / max(sq[i]["SQ_WAVES"], 1), 1)
         clk = mf[i]["GRBM_GUI_ACTIVE"] / 8
         p["mfma_busy"] = round(mf[i]["SQ_VALU_MFMA_BUSY_CYCLES"] / (clk * 1024), 3)
-        p["vgpr"] = sq[i]["vgpr"]
+        p["rocprof_vgpr_count"] = sq[i]["vgpr"]
         p["lds_bytes"] = sq[i]["lds"]
     tot = sum(p["ms_pmc"] for p in per)
     w = lambda k: round(sum(p[k] * p["ms_pmc"] for p in per) / tot, 3)
@@ -162,7 +162,7 @@ def summarize_counters(args):
         json.dump(out, f, indent=1)
         f.write("\n")
     cols = ["run", "ms_pmc", "fetch_gb", "write_gb", "valu_per_wave", "valu_issue", "lds_per_wave",
-            "bank_conflict_cycles_per_lds", "mfma_per_wave", "mfma_fp4_tops", "mfma_busy", "clock_ghz", "vgpr",
+            "bank_conflict_cycles_per_lds", "mfma_per_wave", "mfma_fp4_tops", "mfma_busy", "clock_ghz", "rocprof_vgpr_count",
             "lds_bytes"]
     print("| " + " | ".join(cols) + " |")
     print("|" + "---|" * len(cols))
@@ -201,7 +201,7 @@ def summarize_bulge_counters(args):
              "lds_per_valu": round(tot(sq, "SQ_INSTS_LDS") / max(tot(sq, "SQ_INSTS_VALU"), 1), 3),
              "bank_conflict_cycles_per_lds": round(tot(sq, "SQ_LDS_BANK_CONFLICT") / max(tot(sq, "SQ_INSTS_LDS"), 1), 3),
              "clock_ghz": round(tot(sq, "GRBM_GUI_ACTIVE") / 8 / t / 1e9, 2),
-             "vgpr": ds[0]["vgpr"], "lds_bytes": ds[0]["lds"]}
+             "rocprof_vgpr_count": ds[0]["vgpr"], "lds_bytes": ds[0]["lds"]}
         if tm:
             k["valu_busy"] = round(tot(mf, "SQ_ACTIVE_INST_VALU") / (tot(mf, "GRBM_GUI_ACTIVE") / 8 * 1024), 3) \
                 if tot(mf, "SQ_ACTIVE_INST_VALU") else None
