@@ -304,10 +304,10 @@ def bulge_workload(args):
         st = s.stats()
         filt += st.kernel_ms_filter
         ver += st.kernel_ms_verify
-        log(f"step {k + 1}: {times[-1]:.3f} s, candidates {st.n_candidates}, hits {st.bulge_hits}, "
+        log(f"step {k + 1}: {times[-1]:.3f} s, candidates {st.n_candidates} (traced {st.n_traced}), hits {st.bulge_hits}, "
             f"bulge path {st.bulge_ms:.1f} ms ({st.n_pipeline_rounds} pipeline rounds, {st.n_redo_rounds} redone; "
             f"synchronous rounds: filter {st.kernel_ms_filter:.1f} ms, verify {st.kernel_ms_verify:.1f} ms), "
-            f"dfs steps {st.verify_steps} ({st.verify_steps / max(st.n_candidates, 1):.1f}/cand), "
+            f"dfs steps {st.verify_steps} ({st.verify_steps / max(st.n_traced, 1):.1f}/traced cand), "
             f"lane use {st.verify_busy / max(st.verify_slots, 1):.3f}")
     st = s.stats()
     step = sum(times) / len(times)
@@ -319,8 +319,8 @@ def bulge_workload(args):
            "config": {"workload": f"bulge search: {total / 1e9:.2f} Gbp synthetic, 10% copies of 20 x 300-bp repeats "
                                   f"at 10% divergence, {n} guides NRG, M4 D2 R2, count+checksum sink",
                       "genome_bp": st.genome_nucl},
-           "candidates": st.n_candidates, "bulge_hits": st.bulge_hits, "checksum": st.bulge_checksum,
-           "schedule": args.schedule, "tuning": s.tuning(), "pipeline_rounds": st.n_pipeline_rounds,
+           "candidates": st.n_candidates, "traced": st.n_traced, "bulge_hits": st.bulge_hits,
+           "checksum": st.bulge_checksum, "dfs_steps": st.verify_steps, "schedule": args.schedule, "tuning": s.tuning(), "pipeline_rounds": st.n_pipeline_rounds,
            "graph_instantiations_last_step": st.graph_instantiations, "graph_updates_last_step": st.graph_updates,
            "kernel_ms_filter": round(filt / args.steps, 1), "kernel_ms_verify": round(ver / args.steps, 1)}
     s.close()

@@ -38,7 +38,7 @@ class CofTuning(C.Structure):
     _fields_ = [("plain_threads", C.c_uint32), ("plain_pat_chunk", C.c_uint32), ("bulge_first_tiles", C.c_uint32),
                 ("bulge_round_tiles", C.c_uint32), ("verify_blocks_per_cu", C.c_uint32),
                 ("bulge_queue_cap", C.c_uint32), ("verify_refill", C.c_uint32), ("plain_hit_cap", C.c_uint32),
-                ("plain_queue", C.c_uint32)]
+                ("plain_queue", C.c_uint32), ("bulge_screen", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -81,7 +81,7 @@ class CofStats(C.Structure):
                 ("n_redo_rounds", C.c_uint64), ("verify_busy", C.c_uint64),
                 ("convert_ms", C.c_double), ("load_ms", C.c_double), ("load_device", C.c_uint64),
                 ("write_ms", C.c_double), ("write_device", C.c_uint64),
-                ("graph_instantiations", C.c_uint64), ("graph_updates", C.c_uint64)]
+                ("graph_instantiations", C.c_uint64), ("graph_updates", C.c_uint64), ("n_traced", C.c_uint64)]
 
 
 _lib = None

@@ -200,6 +200,7 @@ cof_tuning get_tuning(const cof_ctx* ctx) {
     t.verify_refill = ctx->bulge_tune.refill;
     t.plain_hit_cap = ctx->plain_tune.hit_cap;
     t.plain_queue = ctx->plain_tune.queue;
+    t.bulge_screen = ctx->bulge_tune.screen ? 1u : 2u;
     return t;
 }
 
@@ -231,6 +232,8 @@ void set_tuning(cof_ctx* ctx, const cof_tuning& t) {
         if (t.verify_refill > 64) throw cof::Error(COF_E_INVALID, "verify_refill must be in [1, 64]");
         bt.refill = t.verify_refill;
     }
+    if (t.bulge_screen > 2) throw cof::Error(COF_E_INVALID, "bulge_screen must be 0, 1 or 2");
+    bt.screen = t.bulge_screen == 2 ? 0u : 1u;
     ctx->plain_tune = pt;
     ctx->bulge_tune = bt;
     ctx->cached = false;

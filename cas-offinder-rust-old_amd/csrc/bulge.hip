@@ -820,6 +820,144 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     }
 }
 
+// Budget screen between the filter and the traceback.  A Myers survivor has
+// unit-cost edit distance <= E = M + D + R, but the traceback only emits
+// alignments with at most M mismatches, D DNA and R RNA bulges whose diagonal
+// steps pass the filter cells, and most survivors (94% on a repeat-rich
+// genome with M4 D2 R2) have none: their DFS explores the whole tree the
+// E bound admits and emits nothing.  This kernel decides, per candidate and
+// exactly up to the padding rules, whether any admissible alignment exists,
+// and only candidates that have one enter the traceback queue.
+//
+// One lane per candidate, bit-parallel over the text window: S[d][r][k] is
+// the set of window end positions j (bit j - 1) at which the pattern prefix
+// of the current row is alignable to a text suffix ending at j with <= d DNA
+// bulges, <= r RNA bulges and <= k mismatches, under enumerate()'s step rules
+// (traceback.rs:398-468): a diagonal step needs a filter-cell pass
+// (search.rs:634-671) and no mismatch at an N position; an RNA bulge none of
+// the N neighbours; a DNA bulge i < L and not two N neighbours.  Row i of the
+// pattern updates, with the previous row's sets (old) and this row's (new):
+//   S[d][r][k] = (old[d][r][k] << 1 | 1) & match | (old[d][r][k-1] << 1 | 1) & mismatch
+//              | old[d][r-1][k] & rna_ok | (new[d-1][r][k] << 1 | 1) & dna_ok
+// The shifted-in 1 treats every prefix as alignable to the empty text before
+// the window, which relaxes only alignments reaching the window start (the
+// traceback's padding tests are likewise left out): the screen keeps every
+// candidate the traceback could emit from.  A candidate passes when bit
+// n - 1 of S[D][R][M] is set after row L.  Patterns with D or R above 2 or
+// M above KM are passed unchecked.
+template <typename WORD, int KM>
+__global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __restrict__ genome,
+                                                          const BCand* __restrict__ queue,
+                                                          const uint32_t* __restrict__ q_count, uint32_t q_cap,
+                                                          const BPat* __restrict__ pats, BCand* __restrict__ out,
+                                                          uint32_t* __restrict__ out_count,
+                                                          unsigned long long* __restrict__ n_out) {
+    constexpr int DM = 2, RM = 2;
+    const uint32_t n_queue = *q_count;
+    if (n_queue > q_cap) {
+        // overflowed round: hand the count on, the traceback flags it
+        if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = n_queue;
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t kept = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n_queue; base += stride) {
+        const uint32_t qi = base + threadIdx.x;
+        const bool active = qi < n_queue;
+        bool pass = false;
+        BCand C{};
+        if (active) {
+            C = queue[qi];
+            const BPat& P = pats[C.pat];
+            const uint32_t L = P.L, M = P.M, D = P.D, R = P.R;
+            if (D > (uint32_t)DM || R > (uint32_t)RM || M > (uint32_t)KM) {
+                pass = true;
+            } else {
+                const uint32_t W = L + D;
+                const uint64_t ts = C.j + 1 >= C.floor + W ? C.j + 1 - W : C.floor;
+                const uint32_t n = (uint32_t)(C.j - ts + 1);
+                const uint64_t w0 = ts >> 3;
+                const uint32_t sh = (uint32_t)(ts & 7u) * 4u;
+                uint32_t X[sizeof(WORD) * 2];
+                uint32_t prev = genome[w0];
+#pragma unroll
+                for (int k = 0; k < (int)sizeof(WORD) * 2; k++) {
+                    const uint32_t nx = genome[w0 + k + 1];
+                    X[k] = __builtin_amdgcn_alignbit(nx, prev, sh);
+                    prev = nx;
+                }
+                Masks<WORD> mk;
+                WORD z, tN, tA, tC, tG, tT;
+                window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
+                const WORD keep = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
+                const WORD nz = ~z & keep;            // any nibble but padding
+                const WORD real = nz & ~tN;            // a base (not N, not padding)
+                WORD S[DM + 1][RM + 1][KM + 1];
+#pragma unroll
+                for (int d = 0; d <= DM; d++)
+#pragma unroll
+                    for (int r = 0; r <= RM; r++)
+#pragma unroll
+                        for (int k = 0; k <= KM; k++) S[d][r][k] = keep;   // row 0: the empty prefix
+                const uint64_t pn = P.is_n;
+                for (uint32_t i = 1; i <= L; i++) {
+                    const uint32_t p = i - 1;
+                    const uint32_t pa = (uint32_t)(P.pa >> p) & 1u, pc = (uint32_t)(P.pc >> p) & 1u,
+                                   pg = (uint32_t)(P.pg >> p) & 1u, pt = (uint32_t)(P.pt >> p) & 1u;
+                    const uint32_t fa = (uint32_t)(P.fa >> p) & 1u, fc = (uint32_t)(P.fc >> p) & 1u,
+                                   fg = (uint32_t)(P.fg >> p) & 1u, ft = (uint32_t)(P.ft >> p) & 1u;
+                    const uint32_t n_p = (uint32_t)(pn >> p) & 1u;
+                    const uint32_t n_next = i < L ? (uint32_t)(pn >> i) & 1u : 0u;
+                    const uint32_t n_prev = i >= 2 ? (uint32_t)(pn >> (i - 2)) & 1u : 0u;
+                    // cmp_loose_bit4 and pam_cell_ok as window masks
+                    WORD m = (pa ? tA : 0) | (pc ? tC : 0) | (pg ? tG : 0) | (pt ? tT : 0);
+                    m = (pa & pc & pg & pt) ? nz : (m & real);
+                    WORD cell = (fa ? tA : 0) | (fc ? tC : 0) | (fg ? tG : 0) | (ft ? tT : 0);
+                    cell = (fa & fc & fg & ft) ? nz : (cell & real);
+                    const WORD mt = m & cell;
+                    const WORD ms = n_p ? (WORD)0 : (~m & cell);
+                    const WORD rna = (n_p | n_next | n_prev) ? (WORD)0 : ~(WORD)0;
+                    const WORD dna = (i < L && !(n_p & n_next)) ? ~(WORD)0 : (WORD)0;
+#pragma unroll
+                    for (int d = 0; d <= DM; d++)
+#pragma unroll
+                        for (int r = RM; r >= 0; r--)
+#pragma unroll
+                            for (int k = KM; k >= 0; k--) {
+                                WORD v = ((S[d][r][k] << 1) | (WORD)1) & mt;
+                                if (k > 0) v |= ((S[d][r][k - 1] << 1) | (WORD)1) & ms;
+                                if (r > 0) v |= S[d][r - 1][k] & rna;
+                                if (d > 0) v |= ((S[d - 1][r][k] << 1) | (WORD)1) & dna;
+                                S[d][r][k] = v & keep;
+                            }
+                }
+                WORD fin = 0;
+#pragma unroll
+                for (int d = 0; d <= DM; d++)
+#pragma unroll
+                    for (int r = 0; r <= RM; r++)
+#pragma unroll
+                        for (int k = 0; k <= KM; k++)
+                            if ((uint32_t)d == D && (uint32_t)r == R && (uint32_t)k == M) fin = S[d][r][k];
+                pass = ((fin >> (n - 1)) & 1) != 0;
+            }
+        }
+        // wave compaction: one reservation per wave
+        const uint64_t pb = __ballot(pass);
+        if (pb) {
+            uint32_t b0 = 0;
+            if (lane == (uint32_t)__builtin_ctzll(pb)) b0 = atomicAdd(out_count, (uint32_t)__popcll(pb));
+            b0 = __shfl(b0, (int)__builtin_ctzll(pb));
+            const uint32_t off = __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
+            if (pass) out[b0 + off] = C;
+            kept += pass ? 1u : 0u;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_down(kept, off);
+    if (lane == 0 && kept) atomicAdd(n_out, (unsigned long long)kept);
+}
+
 }  // namespace
 
 struct BulgePlan {
@@ -829,7 +967,26 @@ struct BulgePlan {
     BPat* d_pats = nullptr;
     bool count_only = false;
     uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
+    uint32_t screen_km = 0;       // mismatch states of the budget screen (2, 4, 6; 0: no pattern screened)
 };
+
+// Mismatch states the budget screen needs for the patterns of the given
+// groups: the smallest of 2, 4, 6 covering their largest M (patterns with
+// D or R above 2 or M above 6 are passed unchecked); 0 when none is screened.
+uint32_t screen_states(const std::vector<BPat>& pats, const std::vector<BGroup>& g32, const std::vector<BGroup>& g64) {
+    uint32_t m = 0;
+    bool any = false;
+    for (const auto* gs : {&g32, &g64})
+        for (const BGroup& G : *gs)
+            for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
+                const BPat& P = pats[p];
+                if (P.D > 2 || P.R > 2 || P.M > 6) continue;
+                any = true;
+                m = std::max(m, P.M);
+            }
+    if (!any) return 0;
+    return m <= 2 ? 2u : m <= 4 ? 4u : 6u;
+}
 
 BulgePlan* bulge_plan_create(const Query& q) {
     auto* bp = new BulgePlan();
@@ -892,6 +1049,7 @@ BulgePlan* bulge_plan_create(const Query& q) {
     up(bp->d_g64, bp->groups64);
     up(bp->d_pats, bp->pats);
     for (const auto& P : bp->pats) bp->n_cols = std::max(bp->n_cols, P.L + 1);
+    bp->screen_km = screen_states(bp->pats, bp->groups32, bp->groups64);
     return bp;
 }
 
@@ -922,6 +1080,7 @@ struct RoundCtx {
     uint32_t n_tiles;
     uint32_t verify_blocks;
     Sink base_sink;
+    bool screen;              // budget screen before the traceback
 };
 
 // filter of tiles order[t0, t1) into `queue`, count at *q_count
@@ -940,10 +1099,47 @@ void launch_filter(RoundCtx& c, const uint32_t* d_order, size_t t0, size_t t1, B
                            d_order + t0, tile_cand);
 }
 
+// sums[5] += *q_count (candidates traced without the screen)
+__global__ void add_count_kernel(const uint32_t* __restrict__ q_count, uint32_t q_cap, unsigned long long* __restrict__ dst) {
+    const uint32_t n = *q_count;
+    if (threadIdx.x == 0 && n <= q_cap) atomicAdd(dst, (unsigned long long)n);
+}
+
+template <typename WORD>
+void launch_screen_w(RoundCtx& c, uint32_t km, const BCand* queue, const uint32_t* q_count, uint32_t q_cap,
+                     BCand* out, uint32_t* out_count, uint32_t blocks, hipStream_t st) {
+    unsigned long long* n_out = c.ws.sums + 5;
+    if (km <= 2)
+        hipLaunchKernelGGL((bulge_screen_kernel<WORD, 2>), dim3(blocks), dim3(256), 0, st, c.d_genome, queue, q_count,
+                           q_cap, c.bp->d_pats, out, out_count, n_out);
+    else if (km <= 4)
+        hipLaunchKernelGGL((bulge_screen_kernel<WORD, 4>), dim3(blocks), dim3(256), 0, st, c.d_genome, queue, q_count,
+                           q_cap, c.bp->d_pats, out, out_count, n_out);
+    else
+        hipLaunchKernelGGL((bulge_screen_kernel<WORD, 6>), dim3(blocks), dim3(256), 0, st, c.d_genome, queue, q_count,
+                           q_cap, c.bp->d_pats, out, out_count, n_out);
+}
+
 // Traceback of one queue; with `steps` (q_cap entries) it also records the
 // DFS transitions of each entry and adds them to the workspace's tile_steps.
+// With the budget screen on, the queue is first screened into sq[slot] (count
+// in screen_ctr[slot]), which the traceback then reads; `n_hint` (0: unknown)
+// bounds the screen's grid.
 void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uint32_t q_cap, uint32_t* overflow,
-                   uint32_t* q_next, uint32_t blocks, hipStream_t st, uint32_t* steps) {
+                   uint32_t* q_next, uint32_t blocks, hipStream_t st, uint32_t* steps, int slot, uint32_t n_hint) {
+    if (c.screen && c.bp->screen_km) {
+        BCand* sq = (BCand*)c.ws.sq[slot];
+        uint32_t* sc = c.ws.screen_ctr + slot;
+        fill_u32(sc, 1, 0, st);
+        const uint32_t n = n_hint ? n_hint : q_cap;
+        const uint32_t sb = std::max(1u, std::min((n + 255) / 256, (uint32_t)std::max(1, c.ws.cus) * 8u));
+        if (c.bp->groups64.empty()) launch_screen_w<uint32_t>(c, c.bp->screen_km, queue, q_count, q_cap, sq, sc, sb, st);
+        else launch_screen_w<uint64_t>(c, c.bp->screen_km, queue, q_count, q_cap, sq, sc, sb, st);
+        queue = sq;
+        q_count = sc;
+    } else {
+        hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(64), 0, st, q_count, q_cap, c.ws.sums + 5);
+    }
     size_t lds32 = (size_t)c.bp->n_cols * 64 * sizeof(uint32_t) * 2;
     size_t lds64 = (size_t)c.bp->n_cols * 64 * sizeof(uint64_t) * 2;
     Sink sink = c.base_sink;
@@ -1089,7 +1285,11 @@ void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector
     if (!ws.st2) HIPCHECK(hipStreamCreate(&ws.st2));
     if (!ws.st3) HIPCHECK(hipStreamCreate(&ws.st3));
     RoundCtx rc{bp, ws, g, d_genome, n_tiles, (uint32_t)std::max(1, ws.cus) * std::max(1u, tune.verify_per_cu),
-                Sink{}};
+                Sink{}, tune.screen != 0};
+    auto ensure_screen = [&](int k, uint32_t cap) {
+        if (!rc.screen || !bp->screen_km) return;
+        ws_grow(&ws.sq[k], &ws.sq_cap[k], cap, sizeof(BCand));
+    };
     rc.base_sink = Sink{(BulgeRaw*)ws.bulge_out, ws.counters + 1, ws.bulge_cap, bp->count_only ? 1 : 0,
                         (const DChunk*)ws.chunks, (uint32_t)g.chunks.size(), ws.sums, ws.sums + 1, ws.sums + 2,
                         ws.counters + 3, nullptr, ws.chunk_lut, std::max(1u, std::min(64u, tune.refill))};
@@ -1128,8 +1328,10 @@ void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector
         if (stats) stats->n_candidates += nq;
         if (nq) {
             HIPCHECK(hipEventRecord(ws.ev[2], st));
+            ensure_screen(0, q_cap);
             launch_verify(rc, (const BCand*)ws.qs[0], cnt, q_cap, ws.round_ctr + 1, ws.round_ctr,
-                          std::min<uint32_t>((nq + 63) / 64, rc.verify_blocks), st, want_steps ? ws.qsteps[0] : nullptr);
+                          std::min<uint32_t>((nq + 63) / 64, rc.verify_blocks), st, want_steps ? ws.qsteps[0] : nullptr, 0,
+                          nq);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipEventRecord(ws.ev[3], st));
             HIPCHECK(hipEventSynchronize(ws.ev[3]));
@@ -1233,6 +1435,12 @@ void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector
             }
         }
         const uint32_t R = (uint32_t)rounds.size();
+        if (!sync_rounds) {
+            // screened queues are sized before any capture begins
+            const uint32_t scap = std::max(q_cap, have_r0 ? cap0 : 0u);
+            ensure_screen(0, scap);
+            ensure_screen(1, scap);
+        }
         if (R > (have_r0 ? 1u : 0u) || sync_rounds) {
             ensure_queue(0, q_cap);
             ensure_steps(0, q_cap);
@@ -1290,7 +1498,7 @@ void bulge_pass(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector
                     k_buf++;
                 }
                 HIPCHECK(hipStreamWaitEvent(vs, ws.rev[2 * r], 0));
-                launch_verify(rc, qv, qcount, cap_r, rq + 2, rq + 1, rc.verify_blocks, vs, steps);
+                launch_verify(rc, qv, qcount, cap_r, rq + 2, rq + 1, rc.verify_blocks, vs, steps, (int)(r & 1), 0);
                 HIPCHECK(hipEventRecord(ws.rev[2 * r + 1], vs));
             }
             for (uint32_t r = R >= 2 ? R - 2 : 0; r < R; r++) HIPCHECK(hipStreamWaitEvent(st, ws.rev[2 * r + 1], 0));
@@ -1361,6 +1569,7 @@ BulgePlan groups_view(const BulgePlan& bp, bool wide, uint32_t min_cols, uint32_
         v.n_cols = std::max(v.n_cols, G.L + 1);
     }
     if (!kept.empty()) (wide ? v.d_g64 : v.d_g32) = (wide ? bp.d_g64 : bp.d_g32) + first;
+    v.screen_km = screen_states(bp.pats, v.groups32, v.groups64);
     return v;
 }
 
@@ -1442,6 +1651,7 @@ void bulge_run(BulgePlan* bp, Workspace& ws, const Genome& g, const std::vector<
         stats->verify_steps = sums[2];
         stats->verify_slots = sums[3];
         stats->verify_busy = sums[4];
+        stats->n_traced = sums[5];
         if (bp->count_only) {
             stats->bulge_hits = sums[0];
             stats->bulge_checksum = sums[1];

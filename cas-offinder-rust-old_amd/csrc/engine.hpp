@@ -41,7 +41,7 @@ struct Workspace {
     void* chunks = nullptr;        // chunk table for the device drop rule
     size_t chunks_cap = 0;
     uint32_t* counters = nullptr;  // [0] queue count, [1] record count, [2] queue cursor
-    unsigned long long* sums = nullptr;  // [0] kept, [1] checksum, [2..4] diagnostics
+    unsigned long long* sums = nullptr;  // [0] kept, [1] checksum, [2..4] diagnostics, [5] traced candidates
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int cus = 0;
     // bulge pipeline
@@ -70,6 +70,9 @@ struct Workspace {
     uint32_t* rounds = nullptr;                // per round: queue count, cursor, overflow
     size_t rounds_cap = 0;
     uint32_t* round_ctr = nullptr;             // synchronous round: cursor, overflow
+    void* sq[2] = {nullptr, nullptr};          // screened queues (one per traceback stream)
+    size_t sq_cap[2] = {0, 0};
+    uint32_t* screen_ctr = nullptr;            // screened queue counts, one per sq[]
     std::vector<hipEvent_t> rev;               // per round: filtered, verified
     hipStream_t st2 = nullptr, st3 = nullptr;  // traceback streams of the pipeline
     hipGraphExec_t bulge_exec[3] = {};         // per window class: the last pipeline graph, updated in place
@@ -101,6 +104,7 @@ struct BulgeTune {
     uint32_t verify_per_cu = 24;    // persistent traceback workgroups per CU
     uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
     uint32_t refill = 8;            // traceback wave refills once this many lanes are idle
+    uint32_t screen = 1;            // 1: budget screen between the filter and the traceback
     uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
     // predicted traceback work per genome chunk (cost model); null: static
     // genome order and rounds of equal tile counts

@@ -1301,7 +1301,8 @@ Workspace::~Workspace() {
     for (void* p : {plain_out, plain_out2, (void*)plain_count, (void*)plain_heads, queue, bulge_out, tiles, chunks, (void*)counters, (void*)sums,
                     qs[0], qs[1], (void*)qsteps[0], (void*)qsteps[1], q_first, (void*)q_first_steps, (void*)chunk_lut,
                     (void*)tile_order, (void*)tile_cand, (void*)tile_cand_prev, (void*)tile_steps, (void*)rounds,
-                    (void*)round_ctr, (void*)img_hi, (void*)img_lo, (void*)img_flags})
+                    (void*)round_ctr, sq[0], sq[1], (void*)screen_ctr, (void*)img_hi, (void*)img_lo,
+                    (void*)img_flags})
         if (p) (void)hipFree(p);
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : rev) if (e) (void)hipEventDestroy(e);
@@ -1338,6 +1339,7 @@ void Workspace::ensure_events() {
     HIPCHECK(hipMalloc(&plain_count, kHitSegs * kSegStride * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&plain_heads, kXcds * kSegStride * sizeof(uint32_t)));
     HIPCHECK(hipMalloc(&round_ctr, 2 * sizeof(uint32_t)));
+    HIPCHECK(hipMalloc(&screen_ctr, 2 * sizeof(uint32_t)));
     int dev = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

@@ -113,7 +113,7 @@ typedef struct {
                                   bulge path's synchronous rounds (all rounds with COF_FLAG_SYNC_ROUNDS) */
     double kernel_ms_verify;   /* the same for DP traceback launches */
     uint64_t n_plain_launches;
-    uint64_t n_candidates;     /* Myers survivors sent to the DP traceback */
+    uint64_t n_candidates;     /* Myers survivors of the bulge filter */
     uint64_t n_raw_hits;       /* hits before the host-side seam/drop rules */
     uint64_t n_hits;           /* final hits */
     uint64_t genome_nucl;      /* nucleotides in the resident genome (incl. N) */
@@ -134,6 +134,8 @@ typedef struct {
     uint64_t graph_instantiations; /* bulge pipeline graphs instantiated by the last search */
     uint64_t graph_updates;    /* bulge pipeline graphs replayed after an in-place update (same
                                   pipeline shape as the previous search of this context) */
+    uint64_t n_traced;         /* Myers survivors that entered the DP traceback (after the
+                                  budget screen; n_candidates without it) */
 } cof_stats;
 
 /* Launch shapes and schedule.  No reference counterpart: the reference fixes
@@ -162,6 +164,10 @@ typedef struct {
     uint32_t plain_queue;          /* 1: the plain scan runs as persistent two-wave workgroups that
                                       claim 4,096-start tiles from per-XCD device queues (64-thread
                                       shape, whole groups only); 0: one workgroup per tile */
+    uint32_t bulge_screen;         /* 1 (default, also 0): Myers survivors pass an exact budget
+                                      screen (an alignment within the M / D / R caps and filter
+                                      cells exists) before the DP traceback; 2: every survivor is
+                                      traced.  Results are the same either way */
 } cof_tuning;
 
 /* Per filter tile of the last bulge search (tiles in genome order). */

@@ -164,3 +164,79 @@ def test_window_width_edges(tmp_path, glen, pam, D, R):
     got, _ = gpu_rows(tmp_path, fa, filt, [g + "N" * len(pam)], [M], [D], [R], 1 << 22)
     assert len(want) > 0
     assert got == want
+
+
+def test_budget_screen_keeps_every_alignment(tmp_path):
+    # The budget screen between the Myers filter and the traceback drops only
+    # candidates with no alignment inside the M / D / R caps.  A mixed query
+    # covers every screen shape: M 0-8 (M above 6 passes unchecked), D and R
+    # 0-2, a 5' PAM, and 28-nt guides with D = 2 (64-bit windows); rows,
+    # counts and checksums must not change with the screen on or off.  (The
+    # M = 8 guide, passed unchecked, supplies most of this query's candidates.)
+    rng = random.Random(2024)
+    pats, filts, ms, ds, rs, plants = [], [], [], [], [], []
+    for k in range(12):
+        glen = 28 if k % 6 == 5 else 20
+        g = "".join(rng.choice("ACGT") for _ in range(glen))
+        m, d, r = [0, 2, 3, 4, 5, 6, 8, 4, 3, 4, 2, 4][k], k % 3, (k // 3) % 3
+        if k == 7:
+            pats.append("TTTN" + g)
+            filts.append("TTTN" + "N" * glen)
+            plants.append(("TTTA" + g, 4, glen, m, 12))
+        else:
+            pats.append(g + "NNN")
+            filts.append("N" * glen + "NRG")
+            plants.append((g + "AGG", 0, glen, m, 12))
+        ms.append(m)
+        ds.append(d)
+        rs.append(r)
+    sg = cofpy.SynthGenome(4242, [150_000, 70_001], n_frac=0.01, gap=(20, 300), iupac_frac=0.001,
+                           repeats=(0.15, 6, 300, 0.1), plants=plants)
+    res = {}
+    for screen in (1, 2):
+        for count_only in (False, True):
+            s = cofpy.Search(pats, filts, ms, ds, rs, batch_nucl=1 << 17, count_only=count_only)
+            s.set_tuning(bulge_screen=screen)
+            assert s.tuning()["bulge_screen"] == screen
+            s.load_synth(sg)
+            s.search()
+            st = s.stats()
+            rows = None
+            if not count_only:
+                s.write_tsv(tmp_path / "gpu.tsv")
+                rows = cofpy.read_rows(tmp_path / "gpu.tsv")
+            res[(screen, count_only)] = (rows, st.bulge_hits, st.bulge_checksum, st.n_candidates, st.n_traced)
+            s.close()
+    on, off = res[(1, False)], res[(2, False)]
+    assert len(on[0]) > 100
+    assert on[0] == off[0]
+    assert on[1:4] == off[1:4]
+    assert res[(1, True)][1:4] == res[(2, True)][1:4] == on[1:4]
+    assert off[4] == off[3]
+    assert 0 < on[4] < off[3]
+
+
+def test_budget_screen_single_group_vs_oracle(tmp_path):
+    # one reference-format group (the oracle's scope) with the screen on and
+    # off; with M4 D2 R2 on a repeat-rich genome most Myers survivors have no
+    # alignment within the caps and are screened out
+    rng = random.Random(77)
+    guides = ["".join(rng.choice("ACGT") for _ in range(20)) for _ in range(3)]
+    sg = cofpy.SynthGenome(99, [90_000, 30_001], n_frac=0.02, gap=(20, 300), repeats=(0.2, 4, 300, 0.1),
+                           plants=[(g + "TGG", 0, 20, 4, 20) for g in guides])
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    pats = [g + "NNN" for g in guides]
+    filt = "NNNNNNNNNNNNNNNNNNNNNRG"
+    want = oracle_rows(tmp_path, fa, filt + " 2 2", pats, 4, 1 << 30)
+    for screen in (1, 2):
+        s = cofpy.Search(pats, [filt] * 3, [4] * 3, [2] * 3, [2] * 3, batch_nucl=1 << 30)
+        s.set_tuning(bulge_screen=screen)
+        s.load_file(fa)
+        s.search()
+        s.write_tsv(tmp_path / "gpu.tsv")
+        st = s.stats()
+        s.close()
+        assert cofpy.read_rows(tmp_path / "gpu.tsv") == want
+        assert (st.n_traced < st.n_candidates // 2) if screen == 1 else (st.n_traced == st.n_candidates)
+    assert len(want) > 0

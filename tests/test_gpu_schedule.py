@@ -232,7 +232,7 @@ def test_autotune_keeps_results(tmp_path):
 def test_invalid_tuning_rejected():
     s = cofpy.Search(["A" * 20 + "NNN"], [FILT], [2])
     for kw in ({"plain_threads": 100}, {"verify_blocks_per_cu": 65}, {"bulge_queue_cap": 10},
-               {"plain_hit_cap": 100}, {"plain_queue": 2}):
+               {"plain_hit_cap": 100}, {"plain_queue": 2}, {"bulge_screen": 3}):
         with pytest.raises(cofpy.CofError) as e:
             s.set_tuning(**kw)
         assert e.value.status == cofpy.COF_E_INVALID
