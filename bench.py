@@ -181,7 +181,7 @@ def summarize_bulge_counters(args):
     out = {"bulge_hip_sha256": file_sha256(os.path.join(ROOT, "cas-offinder-rust-old_amd", "csrc", "bulge.hip")),
            "workload": "bench.py --workload bulge (config 3: 3.1 Gbp repeat-rich, 100 guides NRG M4 D2 R2), "
                        "synchronous rounds", "kernels": {}}
-    for key in ("bulge_filter_kernel", "bulge_verify_kernel"):
+    for key in ("bulge_filter_kernel", "bulge_screen_kernel", "bulge_verify_kernel"):
         def tot(disp, c):
             return sum(e.get(c, 0.0) for e in disp.values() if key in e["name"])
         ds = [e for e in sq.values() if key in e["name"]]

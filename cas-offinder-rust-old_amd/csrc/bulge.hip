@@ -204,6 +204,56 @@ __device__ __forceinline__ void myers_step(WORD eq, WORD& VP, WORD& VN) {
     VN = Ph & Xv;
 }
 
+// 0aaaa... -> 0a0a0a...: bit k of x to bit 2k
+__device__ __forceinline__ uint64_t spread2(uint32_t x) {
+    uint64_t v = x;
+    v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+    v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+    v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+}
+
+// Ops of a diagonal chain of n steps pushed at depths [d, d + n): step k
+// covers pattern position n - 1 - k and is a mismatch (op 1) where bit
+// n - 1 - k of `mis` is set, else a match (op 0).  2 bits per depth, depth k
+// at bits 2k of the u64 words of `ops`; depths past d + n are left as they are.
+template <typename WORD, int kOpsW>
+__device__ __forceinline__ void put_tail_ops(uint64_t (&ops)[kOpsW], uint32_t d, WORD mis, uint32_t n) {
+    constexpr uint32_t kBits = sizeof(WORD) * 8;
+    WORD rev;
+    if constexpr (sizeof(WORD) == 8) rev = (WORD)__builtin_bitreverse64((uint64_t)mis);
+    else rev = (WORD)__builtin_bitreverse32((uint32_t)mis);
+    rev = n >= kBits ? rev : rev >> (kBits - n);               // bit k = bit n - 1 - k of mis
+    const WORD keep = n >= kBits ? ~(WORD)0 : (((WORD)1 << n) - 1);
+    // 2n-bit field of ops (s1:s0) and its mask (k1:k0), placed at bit 2d
+    const uint64_t s0 = spread2((uint32_t)rev), k0 = spread2((uint32_t)keep) * 3ull;
+    uint64_t s1 = 0, k1 = 0;
+    if constexpr (sizeof(WORD) == 8) {
+        s1 = spread2((uint32_t)((uint64_t)rev >> 32));
+        k1 = spread2((uint32_t)((uint64_t)keep >> 32)) * 3ull;
+    }
+    const uint32_t b = 2u * d;
+    if constexpr (kOpsW == 1) {
+        // n + d <= 32: the field fits one word
+        ops[0] = (ops[0] & ~(k0 << b)) | (s0 << b);
+    } else {
+        // 128-bit field shifted by b in [0, 128) over ops[0], ops[1]
+        uint64_t m0, m1, v0, v1;
+        if (b == 0) { m0 = k0; m1 = k1; v0 = s0; v1 = s1; }
+        else if (b < 64) {
+            m0 = k0 << b; m1 = (k1 << b) | (k0 >> (64 - b));
+            v0 = s0 << b; v1 = (s1 << b) | (s0 >> (64 - b));
+        } else {
+            m0 = 0; v0 = 0;
+            m1 = k0 << (b - 64); v1 = s0 << (b - 64);
+        }
+        ops[0] = (ops[0] & ~m0) | v0;
+        ops[1] = (ops[1] & ~m1) | v1;
+    }
+}
+
 template <typename WORD>
 __device__ __forceinline__ uint32_t popc_w(WORD x) {
     if constexpr (sizeof(WORD) == 8) return (uint32_t)__popcll((unsigned long long)x);
@@ -424,6 +474,14 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     uint64_t ts = 0;
     WORD z = 0, tN = 0, tA = 0, tC = 0, tG = 0, tT = 0;   // window planes
     WORD pa = 0, pc = 0, pg = 0, pt = 0, fa = 0, fc = 0, fg = 0, ft = 0, pn = 0;  // pattern planes
+    // Once both bulge budgets are spent only diagonal steps remain, and every
+    // such node lies on the diagonal jj - i = n - L - D + R.  Over pattern
+    // positions p of that diagonal: tail_ok = the step onto text position
+    // n - L - D + R + p is admissible apart from the mismatch cap (in the
+    // window, no padding, filter cell passes, no mismatch at an N);
+    // tail_mm = it is a mismatch.  A node entering that state resolves its
+    // whole chain from these two masks instead of one push per step.
+    WORD tail_ok = 0, tail_mm = 0;
     uint32_t L = 0, M = 0, D = 0, R = 0, E = 0, pid = 0, last_chunk = 0;
     uint64_t hash_id = 0;
     // DFS state
@@ -635,6 +693,27 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                             cvn[(ii + 1) * 64 + lane] = VN;
                         }
                     }
+                    {
+                        const int delta = (int)n - (int)L - (int)D + (int)R;
+                        constexpr int kBits = (int)sizeof(WORD) * 8;
+                        auto shift = [&](WORD x) -> WORD {
+                            return delta >= 0 ? (delta >= kBits ? (WORD)0 : x >> delta)
+                                              : (-delta >= kBits ? (WORD)0 : x << -delta);
+                        };
+                        // pattern positions whose text position lies in [0, n)
+                        const int hi = (int)n - delta;        // p < hi
+                        WORD valid = hi >= kBits ? ~(WORD)0 : hi <= 0 ? (WORD)0 : (((WORD)1 << hi) - 1);
+                        if (delta < 0) valid &= -delta >= kBits ? (WORD)0 : ~(((WORD)1 << -delta) - 1);
+                        const WORD nz = ~shift(z) & valid, re = nz & ~shift(tN);
+                        const WORD sA = shift(tA), sC = shift(tC), sG = shift(tG), sT = shift(tT);
+                        const WORD any_p = (pa & sA) | (pc & sC) | (pg & sG) | (pt & sT);
+                        const WORD match = (pn & nz) | (~pn & any_p & re);
+                        const WORD fF = fa & fc & fg & ft;
+                        const WORD any_f = (fa & sA) | (fc & sC) | (fg & sG) | (ft & sT);
+                        const WORD cell = (fF & nz) | (~fF & any_f & re);
+                        tail_mm = ~match & cell;
+                        tail_ok = cell & ~(tail_mm & pn);
+                    }
                     i = L; jj = n; d = 0; d_floor = 0; cost = mm = db = rb = 0;
                     // root admission: padding at the window end, DP bound
                     busy = !bit(z, n - 1) && lbc(L, VP, VN, n) <= E;
@@ -709,7 +788,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                                r_tT = xferw(tT);
                     const WORD r_pa = xferw(pa), r_pc = xferw(pc), r_pg = xferw(pg), r_pt = xferw(pt),
                                r_fa = xferw(fa), r_fc = xferw(fc), r_fg = xferw(fg), r_ft = xferw(ft),
-                               r_pn = xferw(pn);
+                               r_pn = xferw(pn), r_tok = xferw(tail_ok), r_tmm = xferw(tail_mm);
                     const uint32_t r_L = xfer32(L), r_M = xfer32(M), r_D = xfer32(D), r_R = xfer32(R),
                                    r_E = xfer32(E), r_pid = xfer32(pid), r_last = xfer32(last_chunk),
                                    r_chunk = xfer32(cand_chunk);
@@ -731,6 +810,8 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         z = r_z; tN = r_tN; tA = r_tA; tC = r_tC; tG = r_tG; tT = r_tT;
                         pa = r_pa; pc = r_pc; pg = r_pg; pt = r_pt; fa = r_fa; fc = r_fc; fg = r_fg; ft = r_ft;
                         pn = r_pn;
+                        tail_ok = r_tok;
+                        tail_mm = r_tmm;
                         L = r_L; M = r_M; D = r_D; R = r_R; E = r_E; pid = r_pid; last_chunk = r_last;
                         cand_chunk = r_chunk;
                         if (sink.count_only) load_c0();
@@ -799,8 +880,33 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             busy = false;
             continue;
         }
-        if (i == 0) emit();
-        set4(d, children());
+        if (i == 0) {
+            emit();
+            set4(d, 0u);
+        } else if (db == D && rb == R) {
+            // only the diagonal chain to i = 0 remains (see tail_ok)
+            const WORD pm = i >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << i) - 1);
+            const WORD tm = tail_mm & pm;
+            const uint32_t tmm = popc_w<WORD>(tm);
+            if ((tail_ok & pm) == pm && mm + tmm <= M) {
+                // ops of depths d .. d+i-1: the step at depth d + k is pattern
+                // position i - 1 - k, op 1 where it is a mismatch, else 0
+                put_tail_ops(ops, d, tm, i);
+                const uint32_t d0 = d, jj0 = jj, mm0 = mm;
+                d += i;
+                jj -= i;
+                mm += tmm;
+                cost += tmm;
+                emit();
+                d = d0;
+                jj = jj0;
+                mm = mm0;
+                cost -= tmm;
+            }
+            set4(d, 0u);
+        } else {
+            set4(d, children());
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         my_kept += __shfl_down(my_kept, off);
