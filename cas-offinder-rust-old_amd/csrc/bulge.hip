@@ -260,47 +260,119 @@ __device__ __forceinline__ uint32_t popc_w(WORD x) {
     else return (uint32_t)__popc((uint32_t)x);
 }
 
+// One step of Myers' bit-parallel edit distance with the pattern in the
+// bit-vector and the text streaming (semi-global: any text start); score is
+// the distance of the whole pattern to the best text suffix so far.  Bits
+// above L - 1 may hold garbage: carries and shifts only move upward, so they
+// never reach the bits that are read.
+template <typename WORD>
+__device__ __forceinline__ void myers_scan_step(WORD eq, WORD& VP, WORD& VN, uint32_t& score, uint32_t last) {
+    const WORD X = eq | VN;
+    const WORD D0 = (((X & VP) + VP) ^ VP) | X;
+    const WORD HN = VP & D0;
+    const WORD HP = VN | ~(VP | D0);
+    const WORD Xs = HP << 1;
+    VN = Xs & D0;
+    VP = (HN << 1) | ~(Xs | D0);
+    score = score + (uint32_t)((HP >> last) & 1) - (uint32_t)((HN >> last) & 1);
+}
+
+// Filter of one tile of end positions for every group: the exact quick PAM
+// check per end position (check_pam_quick, search.rs:703-755) into a bitmap,
+// then for each pattern a Myers scan along the tile.  The reference scores
+// each end position's own window of W = L + D characters
+// (search.rs:841-877); a scan started W - 1 characters before its first end
+// position scores each end over all text starts from there, which is never
+// more than the window score, so it only pre-selects: a pre-selected end
+// whose quick check passed is scored again over its exact window
+// (transposed Myers, text in the bit-vector) before it is queued.  Lanes take
+// (pattern, segment) tasks, each lane scanning its own segment of the tile's
+// LDS copy of the genome with its own nibble -> match-mask table in LDS, so
+// groups of any size keep the lanes busy.
 template <typename WORD>
 __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     const uint32_t* __restrict__ genome, const BTile* __restrict__ tiles, const BGroup* __restrict__ groups,
     uint32_t n_groups, const BPat* __restrict__ pats, BCand* __restrict__ queue, uint32_t* __restrict__ q_count,
     uint32_t q_cap, const uint32_t* __restrict__ tile_ids, uint32_t* __restrict__ tile_cand) {
     __shared__ uint32_t gw[kWinWords];
-    __shared__ uint16_t cand[kTileEnds];
-    __shared__ uint32_t n_cand;
-    // per-wave staging of Myers survivors: (end offset << 20 | pattern - group
-    // base); a wave reserves queue space once per flush, not per pattern
+    __shared__ uint32_t qbits[kTileEnds / 32];
+    __shared__ WORD eqt[kFilterThreads / 64][16][64];
+    // per-wave staging of pre-selected ends: (end offset << 20 | pattern -
+    // group base); they are re-scored and queued a wave's batch at a time
     __shared__ uint32_t stage[kFilterThreads / 64][kStage];
 
     const uint32_t tile_id = tile_ids[blockIdx.x];
     const BTile T = tiles[tile_id];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint32_t my_cand = 0;
-    uint32_t* my_stage = stage[tid >> 6];
+    uint32_t* my_stage = stage[wave];
     uint32_t n_st = 0;   // staged entries of this wave (wave-uniform)
     // LDS window: from the lowest text start any end in the tile can use
     const uint64_t lo = T.j0 > T.floor + (kMaxW + 8) ? T.j0 - (kMaxW + 8) : T.floor;
     const uint64_t wlo = lo >> 3;
     const uint64_t base = wlo * 8;
+    const uint32_t j_off = (uint32_t)(T.j0 - base);     // LDS nibble of end offset 0
     for (uint32_t i = tid; i < (uint32_t)kWinWords; i += kFilterThreads) gw[i] = genome[wlo + i];
     __syncthreads();
 
-    // copies the wave's staged survivors to the queue; once the queue count
+    // Exact window score of the staged ends; the survivors are compacted in
+    // place and then queued with one reservation.  Once the queue count
     // exceeds q_cap it stops growing (the host redoes such a round), so the
-    // 32-bit count cannot wrap
-    auto flush = [&](uint32_t pat_base) {
+    // 32-bit count cannot wrap.
+    auto flush = [&](const BGroup& G) {
         wave_lds_sync();
-        uint32_t qb = q_cap;
-        if (lane == 0) {
-            if (__hip_atomic_load(q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= q_cap)
-                qb = atomicAdd(q_count, n_st);
-            my_cand += n_st;
+        uint32_t n_pass = 0;    // survivors so far, at my_stage[0, n_pass) (<= entries read)
+        for (uint32_t i0 = 0; i0 < n_st; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            bool pass = false;
+            uint32_t v = 0;
+            if (i < n_st) {
+                v = my_stage[i];
+                const uint64_t j = T.j0 + (v >> 20);
+                const uint64_t ts = j + 1 >= T.floor + G.W ? j + 1 - G.W : T.floor;
+                const uint32_t n = (uint32_t)(j - ts + 1);
+                const uint32_t q = (uint32_t)(ts - base);
+                const uint32_t qw = q >> 3, sh = (q & 7u) * 4u;
+                uint32_t X[sizeof(WORD) * 2];
+#pragma unroll
+                for (int k = 0; k < (int)sizeof(WORD) * 2; k++)
+                    X[k] = __builtin_amdgcn_alignbit(gw[qw + k + 1], gw[qw + k], sh);
+                Masks<WORD> mk;
+                window_masks<WORD>(X, n, mk, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                const WORD nmask = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
+                const BPat& P = pats[G.pat_begin + (v & 0xFFFFFu)];
+                const uint32_t L = P.L;
+                WORD VP = 0, VN = 0;
+                for (uint32_t w = 0, ii = 0; ii < L; w++) {
+                    uint64_t cw = P.cls_bits[w];
+                    for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3)
+                        myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
+                }
+                const int32_t score = (int32_t)L + (int32_t)popc_w<WORD>(VP & nmask) - (int32_t)popc_w<WORD>(VN & nmask);
+                pass = score <= (int32_t)P.E;
+            }
+            const uint64_t pb = __ballot(pass);
+            const uint32_t off =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
+            wave_lds_sync();        // every lane has read its entry before any is overwritten
+            if (pass) my_stage[n_pass + off] = v;
+            n_pass += (uint32_t)__popcll(pb);
         }
-        qb = __builtin_amdgcn_readlane(qb, 0);
-        for (uint32_t i = lane; i < n_st; i += 64) {
-            const uint32_t v = my_stage[i];
-            if (qb + i < q_cap)
-                queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, pat_base + (v & 0xFFFFFu), T.last_chunk, tile_id, 0u};
+        wave_lds_sync();
+        if (n_pass) {
+            uint32_t qb = q_cap;
+            if (lane == 0) {
+                if (__hip_atomic_load(q_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= q_cap)
+                    qb = atomicAdd(q_count, n_pass);
+                my_cand += n_pass;
+            }
+            qb = __builtin_amdgcn_readlane(qb, 0);
+            for (uint32_t i = lane; i < n_pass; i += 64) {
+                const uint32_t v = my_stage[i];
+                if (qb + i < q_cap)
+                    queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, G.pat_begin + (v & 0xFFFFFu), T.last_chunk,
+                                          tile_id, 0u};
+            }
         }
         n_st = 0;
         wave_lds_sync();
@@ -308,25 +380,23 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
 
     for (uint32_t gi = 0; gi < n_groups; gi++) {
         const BGroup& G = groups[gi];
-        if (tid == 0) n_cand = 0;
-        __syncthreads();
         // quick PAM check per end position (check_pam_quick, search.rs:703-755)
         const uint32_t dbr = G.pam_off == 0 ? G.D + 1 : 1, rbr = G.pam_off == 0 ? G.R + 1 : 1;
-        for (uint32_t b0 = 0; b0 < T.n; b0 += kFilterThreads) {
-            uint32_t e = b0 + tid;
+        for (uint32_t b0 = 0; b0 < (uint32_t)kTileEnds; b0 += kFilterThreads) {
+            const uint32_t e = b0 + tid;
             bool ok = false;
             if (e < T.n) {
-                uint64_t j = T.j0 + e;
-                uint64_t jl = j - T.floor;              // batch-local end position
+                const uint64_t j = T.j0 + e;
+                const uint64_t jl = j - T.floor;              // batch-local end position
                 if (jl + 1 >= G.L) {
                     for (uint32_t bd = 0; bd < dbr && !ok; bd++) {
                         for (uint32_t br = 0; br < rbr && !ok; br++) {
-                            int64_t span = (int64_t)G.L + bd - br;
+                            const int64_t span = (int64_t)G.L + bd - br;
                             if (span <= 0 || span > (int64_t)jl + 1) continue;
-                            uint64_t as = j + 1 - (uint64_t)span;
+                            const uint64_t as = j + 1 - (uint64_t)span;
                             bool good = true;
                             for (uint32_t k = 0; k < G.pam_len && good; k++) {
-                                uint64_t gp = as + G.pam_off + k;
+                                const uint64_t gp = as + G.pam_off + k;
                                 if (gp >= T.limit) { good = false; break; }
                                 good = pam_cell_ok(lds_nib(gw, (uint32_t)(gp - base)), G.pam_filter[k]);
                             }
@@ -335,54 +405,111 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                     }
                 }
             }
-            uint64_t bal = __ballot(ok);
-            uint32_t cnt = (uint32_t)__popcll(bal);
-            uint32_t wb = 0;
-            if (lane == 0 && cnt) wb = atomicAdd(&n_cand, cnt);
-            wb = __builtin_amdgcn_readlane(wb, 0);
-            if (ok) cand[wb + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))] = (uint16_t)e;
+            const uint64_t bal = __ballot(ok);
+            if (lane == 0) {
+                qbits[(b0 >> 5) + wave * 2] = (uint32_t)bal;
+                qbits[(b0 >> 5) + wave * 2 + 1] = (uint32_t)(bal >> 32);
+            }
         }
         __syncthreads();
-        const uint32_t nc = n_cand;
-        for (uint32_t c0 = (tid & ~63u); c0 < nc; c0 += kFilterThreads) {
-            uint32_t c = c0 + lane;
-            bool active = c < nc;
-            uint32_t e = active ? cand[c] : 0;
-            uint64_t j = T.j0 + e;
-            uint64_t ts = j + 1 >= T.floor + G.W ? j + 1 - G.W : T.floor;
-            uint32_t n = (uint32_t)(j - ts + 1);
-            // window words aligned to its first nibble
-            uint32_t q = (uint32_t)(ts - base);
-            uint32_t qw = q >> 3, sh = (q & 7u) * 4u;
-            uint32_t X[sizeof(WORD) * 2];
-#pragma unroll
-            for (int k = 0; k < (int)sizeof(WORD) * 2; k++) X[k] = __builtin_amdgcn_alignbit(gw[qw + k + 1], gw[qw + k], sh);
-            Masks<WORD> mk;
-            window_masks<WORD>(X, n, mk, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-            WORD nmask = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
-            for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
-                const BPat& P = pats[p];
-                const uint32_t L = P.L;
-                WORD VP = 0, VN = 0;
-                // pattern classes stream from scalar registers, 21 per word
-                for (uint32_t w = 0, i = 0; i < L; w++) {
-                    uint64_t cw = P.cls_bits[w];
-                    for (uint32_t k = 0; k < 21 && i < L; k++, i++, cw >>= 3)
-                        myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
+        // tasks (pattern, segment of S end positions): segments shrink until
+        // the group's tasks fill the workgroup a few times over, down to 64
+        // ends (128 for 64-bit windows, whose warm-up is longer)
+        const uint32_t np = G.pat_count, W = G.W;
+        const uint32_t s_min = sizeof(WORD) == 8 ? 128u : 64u;
+        uint32_t nseg = 1;
+        while (nseg * np < 4u * kFilterThreads && (uint32_t)kTileEnds / (nseg * 2u) >= s_min) nseg *= 2;
+        const uint32_t S = (uint32_t)kTileEnds / nseg;
+        const uint32_t n_tasks = np * nseg;
+        // every lane of a wave steps through the same number of characters:
+        // W - 1 warm-up characters, then S scored end positions
+        for (uint32_t t0 = wave * 64; t0 < n_tasks; t0 += kFilterThreads) {
+            const uint32_t task = t0 + lane;
+            const bool act = task < n_tasks;
+            // consecutive lanes take consecutive patterns of one segment, so
+            // their text and quick-bit reads are LDS broadcasts (segments
+            // lie a multiple of 32 words apart: one bank)
+            const uint32_t pi = act ? task % np : 0u;
+            const uint32_t e0 = (act ? task / np : 0u) * S;
+            const BPat& P = pats[G.pat_begin + pi];
+            const uint32_t L = P.L, E = P.E;
+            // Myers class masks of the pattern (myers.rs:20-50) and the
+            // lane's match mask for every text nibble
+            WORD pA = 0, pC = 0, pG = 0, pT = 0, pN = 0;
+            for (uint32_t w = 0, ii = 0; ii < L; w++) {
+                uint64_t cw = P.cls_bits[w];
+                for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3) {
+                    const uint32_t c = (uint32_t)(cw & 7u);
+                    const WORD b = (WORD)1 << ii;
+                    pA |= (c == 0u || c == 4u) ? b : (WORD)0;
+                    pC |= (c == 1u || c == 4u) ? b : (WORD)0;
+                    pG |= (c == 2u || c == 4u) ? b : (WORD)0;
+                    pT |= (c == 3u || c == 4u) ? b : (WORD)0;
+                    pN |= c == 4u ? b : (WORD)0;
                 }
-                int32_t score = (int32_t)L + (int32_t)popc_w<WORD>(VP & nmask) - (int32_t)popc_w<WORD>(VN & nmask);
-                const bool pass = active && score <= (int32_t)P.E;
-                const uint64_t pb = __ballot(pass);
-                if (pb) {
-                    const uint32_t off =
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
-                    if (pass) my_stage[n_st + off] = (e << 20) | (p - G.pat_begin);
-                    n_st += (uint32_t)__popcll(pb);
-                    if (n_st > (uint32_t)kStage - 64u) flush(G.pat_begin);
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < 16u; c++) {
+                WORD m = ((c & 4u) ? pA : (WORD)0) | ((c & 2u) ? pC : (WORD)0) | ((c & 8u) ? pG : (WORD)0) |
+                         ((c & 1u) ? pT : (WORD)0);
+                eqt[wave][c][lane] = c == 0u ? (WORD)0 : c == 15u ? pN : m;
+            }
+            // The scan starts at least W - 1 characters before the segment's
+            // first end, in blocks of 8 characters (one LDS word each, at
+            // the same nibble offset for every lane: segment starts are
+            // multiples of 64), or at the window's first character when
+            // there is not that much text before it.
+            const uint32_t qa = j_off + e0;                 // character of end offset e0
+            const uint32_t sh = (qa & 7u) * 4u;
+            const uint32_t nw = (W + 6u) & ~7u;            // >= W - 1, a multiple of 8
+            const uint32_t last = L - 1;
+            WORD VP = ~(WORD)0, VN = 0;
+            uint32_t score = L;
+            if (qa >= nw) {
+                for (uint32_t wi = (qa - nw) >> 3; wi < qa >> 3; wi++) {
+                    const uint32_t x = __builtin_amdgcn_alignbit(gw[wi + 1], gw[wi], sh);
+                    WORD eqk[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) eqk[u] = eqt[wave][(x >> (4 * u)) & 15u][lane];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) myers_scan_step<WORD>(eqk[u], VP, VN, score, last);
+                }
+            } else {
+                for (uint32_t q = 0; q < qa; q++) myers_scan_step<WORD>(eqt[wave][lds_nib(gw, q)][lane], VP, VN, score, last);
+            }
+            const uint32_t e_end = act ? min(e0 + S, T.n) : 0u;
+            for (uint32_t k0 = 0; k0 < S; k0 += 8) {
+                const uint32_t e = e0 + k0;
+                const uint32_t wi = (qa + k0) >> 3;
+                const uint32_t x = __builtin_amdgcn_alignbit(gw[wi + 1], gw[wi], sh);
+                WORD eqk[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) eqk[u] = eqt[wave][(x >> (4 * u)) & 15u][lane];
+                // quick-check bits of ends e .. e + 7 (e is a multiple of 8)
+                uint32_t hit = (qbits[e >> 5] >> (e & 31u)) & 0xFFu;
+                if (e + 8u > e_end) hit &= e_end > e ? ((1u << (e_end - e)) - 1u) : 0u;
+                uint32_t low = 0;
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    myers_scan_step<WORD>(eqk[u], VP, VN, score, last);
+                    low |= (score <= E ? 1u : 0u) << u;
+                }
+                hit &= low;
+                if (__ballot(hit != 0u)) {
+                    for (uint32_t u = 0; u < 8u; u++) {
+                        const bool sel = (hit >> u) & 1u;
+                        const uint64_t pb = __ballot(sel);
+                        if (!pb) continue;
+                        const uint32_t off =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
+                        if (sel) my_stage[n_st + off] = ((e + u) << 20) | pi;
+                        n_st += (uint32_t)__popcll(pb);
+                        if (n_st > (uint32_t)kStage - 64u) flush(G);
+                    }
                 }
             }
         }
-        if (n_st) flush(G.pat_begin);
+        if (n_st) flush(G);
         __syncthreads();
     }
     // candidates of this tile (one workgroup per tile)
