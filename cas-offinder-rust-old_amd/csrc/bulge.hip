@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <map>
 #include <tuple>
+#include <type_traits>
 
 #include "engine.hpp"
 #include "hip_check.hpp"
@@ -1062,22 +1063,27 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
 // exactly up to the padding rules, whether any admissible alignment exists,
 // and only candidates that have one enter the traceback queue.
 //
-// One lane per candidate, bit-parallel over the text window: S[d][r][k] is
-// the set of window end positions j (bit j - 1) at which the pattern prefix
-// of the current row is alignable to a text suffix ending at j with <= d DNA
-// bulges, <= r RNA bulges and <= k mismatches, under enumerate()'s step rules
-// (traceback.rs:398-468): a diagonal step needs a filter-cell pass
-// (search.rs:634-671) and no mismatch at an N position; an RNA bulge none of
-// the N neighbours; a DNA bulge i < L and not two N neighbours.  Row i of the
-// pattern updates, with the previous row's sets (old) and this row's (new):
-//   S[d][r][k] = (old[d][r][k] << 1 | 1) & match | (old[d][r][k-1] << 1 | 1) & mismatch
-//              | old[d][r-1][k] & rna_ok | (new[d-1][r][k] << 1 | 1) & dna_ok
-// The shifted-in 1 treats every prefix as alignable to the empty text before
-// the window, which relaxes only alignments reaching the window start (the
-// traceback's padding tests are likewise left out): the screen keeps every
-// candidate the traceback could emit from.  A candidate passes when bit
-// n - 1 of S[D][R][M] is set after row L.  Patterns with D or R above 2 or
-// M above KM are passed unchecked.
+// One lane per candidate.  S[d][r][k] is the set of text ends j at which the
+// pattern prefix of the current row i is alignable to a text suffix ending
+// at j with <= d DNA bulges, <= r RNA bulges and <= k mismatches, under
+// enumerate()'s step rules (traceback.rs:398-468): a diagonal step needs a
+// filter-cell pass (search.rs:634-671) and no mismatch at an N position; an
+// RNA bulge none of the N neighbours; a DNA bulge i < L and not two N
+// neighbours.  Only ends from which the rest of the pattern can still reach
+// the window end n with the bulges allowed matter: j - (n - (L - i) - D) in
+// [0, D + R], a band of at most 5 positions.  In band coordinates a diagonal
+// step keeps the position, an RNA bulge (next row, same j) lowers it by one
+// and a DNA bulge (same row, next j) raises it by one, so for one (d, r) all
+// mismatch budgets k fit one word as 5-bit fields:
+//   S[d][r] = S'[d][r] & match | (S'[d][r] << 5) & mismatch           (S': row i-1)
+//           | (S'[d][r-1] >> 1) & rna_ok | (S[d-1][r] << 1) & dna_ok
+// with match / mismatch the band's admissible diagonal steps of row i in
+// every field.  Row 0 (the empty prefix) holds every band position, which
+// also admits starts before the window (a relaxation, as is leaving out the
+// traceback's padding tests): the screen keeps every candidate the traceback
+// could emit from.  A candidate passes when bit D of field M of S[D][R] is
+// set after row L.  Patterns with D or R above 2 or M above KM are passed
+// unchecked.
 template <typename WORD, int KM>
 __global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __restrict__ genome,
                                                           const BCand* __restrict__ queue,
@@ -1086,6 +1092,16 @@ __global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __res
                                                           uint32_t* __restrict__ out_count,
                                                           unsigned long long* __restrict__ n_out) {
     constexpr int DM = 2, RM = 2;
+    // (KM + 1) fields of 5 band bits
+    using PK = typename std::conditional<5 * (KM + 1) <= 32, uint32_t, uint64_t>::type;
+    constexpr PK kRep = [] {
+        PK r = 0;
+        for (int k = 0; k <= KM; k++) r |= (PK)1 << (5 * k);
+        return r;
+    }();
+    constexpr PK kAll = kRep * (PK)31;         // every field, every band bit
+    constexpr PK kNotB0 = kAll & ~kRep;        // a field's bit 0 cleared (target of << 1)
+    constexpr PK kNotB4 = kAll & ~(kRep << 4); // a field's bit 4 cleared (target of >> 1)
     const uint32_t n_queue = *q_count;
     if (n_queue > q_cap) {
         // overflowed round: hand the count on, the traceback flags it
@@ -1124,56 +1140,67 @@ __global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __res
                 WORD z, tN, tA, tC, tG, tT;
                 window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
                 const WORD keep = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
-                const WORD nz = ~z & keep;            // any nibble but padding
-                const WORD real = nz & ~tN;            // a base (not N, not padding)
-                WORD S[DM + 1][RM + 1][KM + 1];
+                const WORD pa = (WORD)P.pa, pc = (WORD)P.pc, pg = (WORD)P.pg, pt = (WORD)P.pt;
+                const WORD fa = (WORD)P.fa, fc = (WORD)P.fc, fg = (WORD)P.fg, ft = (WORD)P.ft, pn = (WORD)P.is_n;
+                const WORD fF = fa & fc & fg & ft;
+                // admissible diagonal steps over pattern positions p on the
+                // band's five diagonals: band position b at row i = p + 1 is
+                // text position n - L - D + b + p
+                WORD dmt[5], dms[5];
+                constexpr int kBits = (int)sizeof(WORD) * 8;
+#pragma unroll
+                for (int bb = 0; bb < 5; bb++) {
+                    const int delta = (int)n - (int)L - (int)D + bb;
+                    auto shift = [&](WORD x) -> WORD {
+                        return delta >= 0 ? (delta >= kBits ? (WORD)0 : x >> delta)
+                                          : (-delta >= kBits ? (WORD)0 : x << -delta);
+                    };
+                    const int hi = (int)n - delta;          // text position < n
+                    WORD valid = hi >= kBits ? ~(WORD)0 : hi <= 0 ? (WORD)0 : (((WORD)1 << hi) - 1);
+                    if (delta < 0) valid &= -delta >= kBits ? (WORD)0 : ~(((WORD)1 << -delta) - 1);
+                    const WORD nz = ~shift(z & keep) & valid, re = nz & ~shift(tN);
+                    const WORD sA = shift(tA), sC = shift(tC), sG = shift(tG), sT = shift(tT);
+                    const WORD match = (pn & nz) | (~pn & ((pa & sA) | (pc & sC) | (pg & sG) | (pt & sT)) & re);
+                    const WORD cell = (fF & nz) | (~fF & ((fa & sA) | (fc & sC) | (fg & sG) | (ft & sT)) & re);
+                    dmt[bb] = match & cell;
+                    dms[bb] = ~match & cell & ~pn;
+                }
+                PK S[DM + 1][RM + 1];
 #pragma unroll
                 for (int d = 0; d <= DM; d++)
 #pragma unroll
-                    for (int r = 0; r <= RM; r++)
-#pragma unroll
-                        for (int k = 0; k <= KM; k++) S[d][r][k] = keep;   // row 0: the empty prefix
-                const uint64_t pn = P.is_n;
+                    for (int r = 0; r <= RM; r++) S[d][r] = kAll;   // row 0: the empty prefix
                 for (uint32_t i = 1; i <= L; i++) {
                     const uint32_t p = i - 1;
-                    const uint32_t pa = (uint32_t)(P.pa >> p) & 1u, pc = (uint32_t)(P.pc >> p) & 1u,
-                                   pg = (uint32_t)(P.pg >> p) & 1u, pt = (uint32_t)(P.pt >> p) & 1u;
-                    const uint32_t fa = (uint32_t)(P.fa >> p) & 1u, fc = (uint32_t)(P.fc >> p) & 1u,
-                                   fg = (uint32_t)(P.fg >> p) & 1u, ft = (uint32_t)(P.ft >> p) & 1u;
+                    uint32_t bmt = 0, bms = 0;
+#pragma unroll
+                    for (int bb = 0; bb < 5; bb++) {
+                        bmt |= (uint32_t)((dmt[bb] >> p) & 1) << bb;
+                        bms |= (uint32_t)((dms[bb] >> p) & 1) << bb;
+                    }
+                    const PK mt = (PK)bmt * kRep, ms = (PK)bms * kRep;
                     const uint32_t n_p = (uint32_t)(pn >> p) & 1u;
                     const uint32_t n_next = i < L ? (uint32_t)(pn >> i) & 1u : 0u;
                     const uint32_t n_prev = i >= 2 ? (uint32_t)(pn >> (i - 2)) & 1u : 0u;
-                    // cmp_loose_bit4 and pam_cell_ok as window masks
-                    WORD m = (pa ? tA : 0) | (pc ? tC : 0) | (pg ? tG : 0) | (pt ? tT : 0);
-                    m = (pa & pc & pg & pt) ? nz : (m & real);
-                    WORD cell = (fa ? tA : 0) | (fc ? tC : 0) | (fg ? tG : 0) | (ft ? tT : 0);
-                    cell = (fa & fc & fg & ft) ? nz : (cell & real);
-                    const WORD mt = m & cell;
-                    const WORD ms = n_p ? (WORD)0 : (~m & cell);
-                    const WORD rna = (n_p | n_next | n_prev) ? (WORD)0 : ~(WORD)0;
-                    const WORD dna = (i < L && !(n_p & n_next)) ? ~(WORD)0 : (WORD)0;
+                    const PK rna = (n_p | n_next | n_prev) ? (PK)0 : kNotB4;
+                    const PK dna = (i < L && !(n_p & n_next)) ? kNotB0 : (PK)0;
 #pragma unroll
                     for (int d = 0; d <= DM; d++)
 #pragma unroll
-                        for (int r = RM; r >= 0; r--)
-#pragma unroll
-                            for (int k = KM; k >= 0; k--) {
-                                WORD v = ((S[d][r][k] << 1) | (WORD)1) & mt;
-                                if (k > 0) v |= ((S[d][r][k - 1] << 1) | (WORD)1) & ms;
-                                if (r > 0) v |= S[d][r - 1][k] & rna;
-                                if (d > 0) v |= ((S[d - 1][r][k] << 1) | (WORD)1) & dna;
-                                S[d][r][k] = v & keep;
-                            }
+                        for (int r = RM; r >= 0; r--) {
+                            PK v = (S[d][r] & mt) | ((S[d][r] << 5) & ms);
+                            if (r > 0) v |= (S[d][r - 1] >> 1) & rna;
+                            if (d > 0) v |= (S[d - 1][r] << 1) & dna;
+                            S[d][r] = v & kAll;
+                        }
                 }
-                WORD fin = 0;
+                PK fin = 0;
 #pragma unroll
                 for (int d = 0; d <= DM; d++)
 #pragma unroll
                     for (int r = 0; r <= RM; r++)
-#pragma unroll
-                        for (int k = 0; k <= KM; k++)
-                            if ((uint32_t)d == D && (uint32_t)r == R && (uint32_t)k == M) fin = S[d][r][k];
-                pass = ((fin >> (n - 1)) & 1) != 0;
+                        if ((uint32_t)d == D && (uint32_t)r == R) fin = S[d][r];
+                pass = ((fin >> (5u * M + D)) & 1) != 0;
             }
         }
         // wave compaction: one reservation per wave
