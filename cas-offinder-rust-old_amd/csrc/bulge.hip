@@ -1109,15 +1109,29 @@ __global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __res
         return;
     }
     const uint32_t lane = threadIdx.x & 63u;
+    // passing queue indices are staged per wave and copied out with one
+    // reservation of the output count per batch (a reservation per wave and
+    // step contends on the one counter)
+    __shared__ uint32_t stage[4][kStage];
+    uint32_t* my_stage = stage[threadIdx.x >> 6];
+    uint32_t n_st = 0;   // wave-uniform
+    auto flush = [&]() {
+        wave_lds_sync();
+        uint32_t b0 = 0;
+        if (lane == 0) b0 = atomicAdd(out_count, n_st);
+        b0 = __builtin_amdgcn_readlane(b0, 0);
+        for (uint32_t i = lane; i < n_st; i += 64) out[b0 + i] = queue[my_stage[i]];
+        n_st = 0;
+        wave_lds_sync();
+    };
     uint32_t kept = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n_queue; base += stride) {
         const uint32_t qi = base + threadIdx.x;
         const bool active = qi < n_queue;
         bool pass = false;
-        BCand C{};
         if (active) {
-            C = queue[qi];
+            const BCand C = queue[qi];
             const BPat& P = pats[C.pat];
             const uint32_t L = P.L, M = P.M, D = P.D, R = P.R;
             if (D > (uint32_t)DM || R > (uint32_t)RM || M > (uint32_t)KM) {
@@ -1203,17 +1217,16 @@ __global__ __launch_bounds__(256) void bulge_screen_kernel(const uint32_t* __res
                 pass = ((fin >> (5u * M + D)) & 1) != 0;
             }
         }
-        // wave compaction: one reservation per wave
         const uint64_t pb = __ballot(pass);
         if (pb) {
-            uint32_t b0 = 0;
-            if (lane == (uint32_t)__builtin_ctzll(pb)) b0 = atomicAdd(out_count, (uint32_t)__popcll(pb));
-            b0 = __shfl(b0, (int)__builtin_ctzll(pb));
             const uint32_t off = __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
-            if (pass) out[b0 + off] = C;
+            if (pass) my_stage[n_st + off] = qi;
+            n_st += (uint32_t)__popcll(pb);
             kept += pass ? 1u : 0u;
+            if (n_st > (uint32_t)kStage - 64u) flush();
         }
     }
+    if (n_st) flush();
     for (int off = 32; off > 0; off >>= 1) kept += __shfl_down(kept, off);
     if (lane == 0 && kept) atomicAdd(n_out, (unsigned long long)kept);
 }
