@@ -589,10 +589,12 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     // depth d <= L + D = window length <= bits of WORD.  Ops are stored for
     // depths 0..W-1 (2 bits each).  The unexplored children of the node at
     // each depth are three bit planes (bit d of cm0/cm1/cm2: diagonal, RNA,
-    // DNA), so no access indexes a register array.  Depth 64 (only with
-    // W = 64) is always a leaf (L + D ops consume the whole pattern), whose
-    // child mask is 0 and needs no storage.
+    // DNA), so no access indexes a register array.  Depth W is always a
+    // leaf (L + D ops consume the whole pattern), whose child mask is 0 and
+    // needs no storage: the planes are words of WORD's width.
     constexpr int kOpsW = sizeof(WORD) / 4;        // u64 words of ops
+    using CM = WORD;
+    constexpr uint32_t kCmBits = sizeof(CM) * 8;
     extern __shared__ unsigned char verify_lds[];
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
     WORD* cvn = cvp + (size_t)n_cols * 64;
@@ -618,7 +620,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     uint32_t d_floor = 0;   // depth the lane's subtree starts at (> 0 for donated work)
     __shared__ uint32_t pairing[64];
     uint64_t ops[kOpsW];
-    uint64_t cm0 = 0, cm1 = 0, cm2 = 0, cm3 = 0;   // cm3: the diagonal child is a mismatch
+    CM cm0 = 0, cm1 = 0, cm2 = 0, cm3 = 0;   // cm3: the diagonal child is a mismatch
 #pragma unroll
     for (int k = 0; k < kOpsW; k++) ops[k] = 0;
     uint64_t my_kept = 0, my_sum = 0, my_steps = 0, my_slots = 0, my_busy = 0;
@@ -639,13 +641,13 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     };
     // bits 0-2: unexplored children (diagonal, RNA, DNA); bit 3: diagonal is a mismatch
     auto get4 = [&](uint32_t dd) -> uint32_t {
-        if (dd >= 64) return 0u;
+        if (dd >= kCmBits) return 0u;
         return ((uint32_t)(cm0 >> dd) & 1u) | (((uint32_t)(cm1 >> dd) & 1u) << 1) | (((uint32_t)(cm2 >> dd) & 1u) << 2) |
                (((uint32_t)(cm3 >> dd) & 1u) << 3);
     };
     auto set4 = [&](uint32_t dd, uint32_t val) {
-        if (dd >= 64) return;
-        const uint64_t m = 1ull << dd;
+        if (dd >= kCmBits) return;
+        const CM m = (CM)1 << dd;
         cm0 = (val & 1u) ? (cm0 | m) : (cm0 & ~m);
         cm1 = (val & 2u) ? (cm1 | m) : (cm1 & ~m);
         cm2 = (val & 4u) ? (cm2 | m) : (cm2 & ~m);
@@ -864,10 +866,10 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                 uint32_t kd = 0xFFFFFFFFu;
                 if (busy) {
                     // pending siblings at depths [d_floor, d)
-                    uint64_t v = cm0 | cm1 | cm2;
-                    if (d < 64) v &= (1ull << d) - 1ull;
-                    v &= ~((1ull << d_floor) - 1ull);
-                    if (v) kd = (uint32_t)__builtin_ctzll(v);
+                    CM v = cm0 | cm1 | cm2;
+                    if (d < kCmBits) v &= ((CM)1 << d) - 1;
+                    v &= ~(((CM)1 << d_floor) - 1);
+                    if (v) kd = (uint32_t)__builtin_ctzll((uint64_t)v);
                 }
                 const bool can = kd != 0xFFFFFFFFu;
                 const uint64_t donors = __ballot(can);
@@ -961,9 +963,9 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         // Backtrack in one step: the deepest depth in [d_floor, d] that still
         // has unexplored children is the highest set bit of the child planes;
         // the state there follows from the counts of the ops above it.
-        uint64_t pend = cm0 | cm1 | cm2;
-        if (d < 63) pend &= (2ull << d) - 1ull;
-        pend &= ~((1ull << d_floor) - 1ull);
+        CM pend = cm0 | cm1 | cm2;
+        if (d + 1 < kCmBits) pend &= ((CM)2 << d) - 1;
+        pend &= ~(((CM)1 << d_floor) - 1);
         if (!pend) {
             // subtree exhausted: candidate (or donated part) done
             my_steps += d - d_floor + 1u;
@@ -972,7 +974,8 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             if (sink.cand_steps) sink.cand_steps[cand_idx] = cand_steps;
             continue;
         }
-        const uint32_t k = 63u - (uint32_t)__builtin_clzll(pend);
+        const uint32_t k = sizeof(CM) == 8 ? 63u - (uint32_t)__builtin_clzll((uint64_t)pend)
+                                           : 31u - (uint32_t)__builtin_clz((uint32_t)pend);
         my_steps += d - k + 1u;
         cand_steps += d - k + 1u;
         if (k < d) {
