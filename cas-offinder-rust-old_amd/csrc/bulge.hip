@@ -598,22 +598,45 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     extern __shared__ unsigned char verify_lds[];
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
     WORD* cvn = cvp + (size_t)n_cols * 64;
+    // Diagonal masks: every diagonal step of the DFS lies on one of the
+    // D + R + 1 diagonals jj - i = n - L - D + b (b = D - db + rb).  For
+    // diagonal b: over pattern positions p, the steps on it that are
+    // admissible apart from the mismatch cap (text position in the window,
+    // no padding, filter cell passes, no mismatch at an N), and the steps
+    // that are mismatches.  With D + R <= 4 they are registers dk0..dk4 /
+    // dm0..dm4, otherwise LDS rows 2b / 2b + 1 after the Myers columns.
+    // Once both bulge budgets are spent every node lies on diagonal R and
+    // only diagonal steps remain: such a node resolves its whole chain from
+    // that diagonal's masks at once.
+    WORD* dgm = cvn + (size_t)n_cols * 64;
     const uint32_t lane = threadIdx.x;
 
     // candidate state
     uint64_t ts = 0;
-    WORD z = 0, tN = 0, tA = 0, tC = 0, tG = 0, tT = 0;   // window planes
-    WORD pa = 0, pc = 0, pg = 0, pt = 0, fa = 0, fc = 0, fg = 0, ft = 0, pn = 0;  // pattern planes
-    // Once both bulge budgets are spent only diagonal steps remain, and every
-    // such node lies on the diagonal jj - i = n - L - D + R.  Over pattern
-    // positions p of that diagonal: tail_ok = the step onto text position
-    // n - L - D + R + p is admissible apart from the mismatch cap (in the
-    // window, no padding, filter cell passes, no mismatch at an N);
-    // tail_mm = it is a mismatch.  A node entering that state resolves its
-    // whole chain from these two masks instead of one push per step.
-    WORD tail_ok = 0, tail_mm = 0;
+    WORD z = 0;                       // window padding plane
+    WORD rna_pm = 0, dna_pm = 0;      // bulges' pattern-side N rules: bit i - 1 / bit i of a node at row i
+    int dg_base = 0;                  // n - L - D
     uint32_t L = 0, M = 0, D = 0, R = 0, E = 0, pid = 0, last_chunk = 0;
     uint64_t hash_id = 0;
+    WORD dk0 = 0, dk1 = 0, dk2 = 0, dk3 = 0, dk4 = 0, dm0 = 0, dm1 = 0, dm2 = 0, dm3 = 0, dm4 = 0;
+    auto pick5 = [](uint32_t b, WORD a0, WORD a1, WORD a2, WORD a3, WORD a4) -> WORD {
+        WORD r = a0;
+        r = b == 1u ? a1 : r;
+        r = b == 2u ? a2 : r;
+        r = b == 3u ? a3 : r;
+        r = b == 4u ? a4 : r;
+        return r;
+    };
+    // admissible steps (ok) and mismatches (mis) of diagonal b
+    auto diag_of = [&](uint32_t b, WORD& ok, WORD& mis) {
+        if (D + R <= 4u) {
+            ok = pick5(b, dk0, dk1, dk2, dk3, dk4);
+            mis = pick5(b, dm0, dm1, dm2, dm3, dm4);
+        } else {
+            ok = dgm[(2u * b) * 64u + lane];
+            mis = dgm[(2u * b + 1u) * 64u + lane];
+        }
+    };
     // DFS state
     bool busy = false;
     uint32_t i = 0, jj = 0, d = 0, cost = 0, mm = 0, db = 0, rb = 0;
@@ -671,12 +694,6 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             nR += (uint32_t)__popcll(b0 & b1);
         }
     };
-    auto text_at = [&](uint32_t a) -> uint32_t {
-        uint32_t t = bit(tT, a) | bit(tC, a) << 1 | bit(tA, a) << 2 | bit(tG, a) << 3;
-        return bit(tN, a) ? 0xFu : bit(z, a) ? 0u : t;
-    };
-    auto pat_at = [&](uint32_t k) -> uint32_t { return bit(pt, k) | bit(pc, k) << 1 | bit(pa, k) << 2 | bit(pg, k) << 3; };
-    auto filt_at = [&](uint32_t k) -> uint32_t { return bit(ft, k) | bit(fc, k) << 1 | bit(fa, k) << 2 | bit(fg, k) << 3; };
     auto lbc = [&](uint32_t ii, WORD vp, WORD vn, uint32_t jx) -> uint32_t {
         WORD m = jx >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << jx) - 1);
         return ii + popc_w<WORD>(vp & m) - popc_w<WORD>(vn & m);
@@ -696,23 +713,20 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         const uint32_t has_j = jj > 0u;
         const uint32_t j1 = jj - has_j;                     // jj - 1 (or 0)
         const uint32_t has_j2 = jj >= 2u;
-        const uint32_t j2 = has_j2 ? jj - 2u : 0u;
         const uint32_t z1 = has_j & bit(z, j1);             // padding at jj - 1
-        const uint32_t z2 = has_j2 & bit(z, j2);            // padding at jj - 2
-        const uint32_t n_im1 = bit(pn, i - 1);
-        const uint32_t n_i = i < L ? bit(pn, i < L ? i : 0u) : 0u;
-        const uint32_t n_im2 = i >= 2u ? bit(pn, i >= 2u ? i - 2u : 0u) : 0u;
+        const uint32_t z2 = has_j2 & bit(z, has_j2 ? jj - 2u : 0u);   // padding at jj - 2
         const uint32_t i2 = i >= 2u;
-        // diagonal (match or substitution)
-        const uint32_t tv = text_at(j1);
-        const uint32_t mc = 1u - cmp_loose01(pat_at(i - 1), tv);
-        const uint32_t diag = has_j & (1u - z1) & (1u - (mc & n_im1)) & (mm + mc <= M) & (1u - (i2 & z2)) &
-                              pam_cell_ok01(tv, filt_at(i - 1)) & (cost + mc + lbc(i - 1, vp1, vn1, j1) <= E);
+        // diagonal (match or substitution): pattern i - 1 on text jj - 1
+        WORD okw, misw;
+        diag_of((uint32_t)((int)jj - (int)i - dg_base), okw, misw);
+        const uint32_t mc = bit(misw, i - 1);
+        const uint32_t diag = bit(okw, i - 1) & (mm + mc <= M) & (1u - (i2 & z2)) &
+                              (cost + mc + lbc(i - 1, vp1, vn1, j1) <= E);
         // RNA bulge (skip a pattern base)
-        const uint32_t rna = (rb < R) & (1u - n_im1) & (1u - n_i) & (1u - n_im2) & (1u - (i2 & z1)) &
+        const uint32_t rna = (rb < R) & bit(rna_pm, i - 1) & (1u - (i2 & z1)) &
                              (cost + 1u + lbc(i - 1, vp1, vn1, jj) <= E);
         // DNA bulge (skip a text base)
-        const uint32_t dna = (db < D) & has_j & (i < L) & (1u - (n_im1 & n_i)) & (1u - z2) & (1u - z1) &
+        const uint32_t dna = (db < D) & has_j & bit(dna_pm, i) & (1u - z2) & (1u - z1) &
                              (cost + 1u + lbc(i, vp0, vn0, j1) <= E);
         return diag | (rna << 1) | (dna << 2) | (mc << 3);
     };
@@ -785,9 +799,6 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     cand_idx = qi;
                     cand_steps = 0;
                     L = P.L; M = P.M; D = P.D; R = P.R; E = P.E; pid = P.pid; hash_id = P.hash_id;
-                    pa = (WORD)P.pa; pc = (WORD)P.pc; pg = (WORD)P.pg; pt = (WORD)P.pt;
-                    fa = (WORD)P.fa; fc = (WORD)P.fc; fg = (WORD)P.fg; ft = (WORD)P.ft;
-                    pn = (WORD)P.is_n;
                     last_chunk = C.last_chunk;
                     if (sink.count_only) {
                         uint32_t c = sink.chunk_lut[C.j >> 16];
@@ -811,6 +822,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         prev = nx;
                     }
                     Masks<WORD> mk;
+                    WORD tN, tA, tC, tG, tT;
                     window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
                     WORD VP = 0, VN = 0;
                     cvp[lane] = 0;
@@ -824,25 +836,39 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         }
                     }
                     {
-                        const int delta = (int)n - (int)L - (int)D + (int)R;
+                        const WORD pa = (WORD)P.pa, pc = (WORD)P.pc, pg = (WORD)P.pg, pt = (WORD)P.pt;
+                        const WORD fa = (WORD)P.fa, fc = (WORD)P.fc, fg = (WORD)P.fg, ft = (WORD)P.ft;
+                        const WORD pn = (WORD)P.is_n, fF = fa & fc & fg & ft;
+                        const WORD lm = L >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << L) - 1);
+                        rna_pm = ~pn & ~(pn >> 1) & ~(pn << 1) & lm;
+                        dna_pm = ~(pn & (pn << 1)) & lm;
+                        dg_base = (int)n - (int)L - (int)D;
                         constexpr int kBits = (int)sizeof(WORD) * 8;
-                        auto shift = [&](WORD x) -> WORD {
-                            return delta >= 0 ? (delta >= kBits ? (WORD)0 : x >> delta)
-                                              : (-delta >= kBits ? (WORD)0 : x << -delta);
-                        };
-                        // pattern positions whose text position lies in [0, n)
-                        const int hi = (int)n - delta;        // p < hi
-                        WORD valid = hi >= kBits ? ~(WORD)0 : hi <= 0 ? (WORD)0 : (((WORD)1 << hi) - 1);
-                        if (delta < 0) valid &= -delta >= kBits ? (WORD)0 : ~(((WORD)1 << -delta) - 1);
-                        const WORD nz = ~shift(z) & valid, re = nz & ~shift(tN);
-                        const WORD sA = shift(tA), sC = shift(tC), sG = shift(tG), sT = shift(tT);
-                        const WORD any_p = (pa & sA) | (pc & sC) | (pg & sG) | (pt & sT);
-                        const WORD match = (pn & nz) | (~pn & any_p & re);
-                        const WORD fF = fa & fc & fg & ft;
-                        const WORD any_f = (fa & sA) | (fc & sC) | (fg & sG) | (ft & sT);
-                        const WORD cell = (fF & nz) | (~fF & any_f & re);
-                        tail_mm = ~match & cell;
-                        tail_ok = cell & ~(tail_mm & pn);
+                        for (uint32_t bb = 0; bb <= D + R; bb++) {
+                            const int delta = dg_base + (int)bb;
+                            auto shift = [&](WORD x) -> WORD {
+                                return delta >= 0 ? (delta >= kBits ? (WORD)0 : x >> delta)
+                                                  : (-delta >= kBits ? (WORD)0 : x << -delta);
+                            };
+                            // pattern positions whose text position lies in [0, n)
+                            const int hi = (int)n - delta;
+                            WORD valid = hi >= kBits ? ~(WORD)0 : hi <= 0 ? (WORD)0 : (((WORD)1 << hi) - 1);
+                            if (delta < 0) valid &= -delta >= kBits ? (WORD)0 : ~(((WORD)1 << -delta) - 1);
+                            const WORD nz = ~shift(z) & valid, re = nz & ~shift(tN);
+                            const WORD sA = shift(tA), sC = shift(tC), sG = shift(tG), sT = shift(tT);
+                            const WORD match = (pn & nz) | (~pn & ((pa & sA) | (pc & sC) | (pg & sG) | (pt & sT)) & re);
+                            const WORD cell = (fF & nz) | (~fF & ((fa & sA) | (fc & sC) | (fg & sG) | (ft & sT)) & re);
+                            const WORD mis = ~match & cell, ok = cell & ~(mis & pn);
+                            if (D + R <= 4u) {
+                                dk0 = bb == 0u ? ok : dk0; dk1 = bb == 1u ? ok : dk1; dk2 = bb == 2u ? ok : dk2;
+                                dk3 = bb == 3u ? ok : dk3; dk4 = bb == 4u ? ok : dk4;
+                                dm0 = bb == 0u ? mis : dm0; dm1 = bb == 1u ? mis : dm1; dm2 = bb == 2u ? mis : dm2;
+                                dm3 = bb == 3u ? mis : dm3; dm4 = bb == 4u ? mis : dm4;
+                            } else {
+                                dgm[(2u * bb) * 64u + lane] = ok;
+                                dgm[(2u * bb + 1u) * 64u + lane] = mis;
+                            }
+                        }
                     }
                     i = L; jj = n; d = 0; d_floor = 0; cost = mm = db = rb = 0;
                     // root admission: padding at the window end, DP bound
@@ -914,18 +940,19 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                                    r_db = xfer32(s_db), r_rb = xfer32(s_rb), r_mask = xfer32(s_mask),
                                    r_kd = xfer32(kd);
                     const uint64_t r_ts = xfer64(ts), r_hash = xfer64(hash_id);
-                    const WORD r_z = xferw(z), r_tN = xferw(tN), r_tA = xferw(tA), r_tC = xferw(tC), r_tG = xferw(tG),
-                               r_tT = xferw(tT);
-                    const WORD r_pa = xferw(pa), r_pc = xferw(pc), r_pg = xferw(pg), r_pt = xferw(pt),
-                               r_fa = xferw(fa), r_fc = xferw(fc), r_fg = xferw(fg), r_ft = xferw(ft),
-                               r_pn = xferw(pn), r_tok = xferw(tail_ok), r_tmm = xferw(tail_mm);
+                    const WORD r_z = xferw(z), r_rpm = xferw(rna_pm), r_dpm = xferw(dna_pm);
+                    const WORD r_dk0 = xferw(dk0), r_dk1 = xferw(dk1), r_dk2 = xferw(dk2), r_dk3 = xferw(dk3),
+                               r_dk4 = xferw(dk4), r_dm0 = xferw(dm0), r_dm1 = xferw(dm1), r_dm2 = xferw(dm2),
+                               r_dm3 = xferw(dm3), r_dm4 = xferw(dm4);
+                    const int r_dgb = (int)xfer32((uint32_t)dg_base);
                     const uint32_t r_L = xfer32(L), r_M = xfer32(M), r_D = xfer32(D), r_R = xfer32(R),
                                    r_E = xfer32(E), r_pid = xfer32(pid), r_last = xfer32(last_chunk),
                                    r_chunk = xfer32(cand_chunk);
                     uint64_t r_ops[kOpsW];
 #pragma unroll
                     for (int w = 0; w < kOpsW; w++) r_ops[w] = xfer64(ops[w]);
-                    // the candidate's Myers columns, rows 0..L, into this lane's slots
+                    // the candidate's Myers columns (rows 0..L) and diagonal masks
+                    // (rows 0..2(D + R) + 1) into this lane's slots
                     for (uint32_t r = 0; r <= r_L; r++) {
                         const WORD vp = cvp[r * 64 + (uint32_t)src], vn = cvn[r * 64 + (uint32_t)src];
                         if (recv) {
@@ -933,15 +960,20 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                             cvn[r * 64 + lane] = vn;
                         }
                     }
+                    for (uint32_t r = 0; r_D + r_R > 4u && r < 2u * (r_D + r_R + 1u); r++) {
+                        const WORD v = dgm[r * 64 + (uint32_t)src];
+                        if (recv) dgm[r * 64 + lane] = v;
+                    }
                     if (recv) {
                         i = r_i; jj = r_jj; cost = r_cost; mm = r_mm; db = r_db; rb = r_rb;
                         d = r_kd; d_floor = r_kd;
                         ts = r_ts; hash_id = r_hash;
-                        z = r_z; tN = r_tN; tA = r_tA; tC = r_tC; tG = r_tG; tT = r_tT;
-                        pa = r_pa; pc = r_pc; pg = r_pg; pt = r_pt; fa = r_fa; fc = r_fc; fg = r_fg; ft = r_ft;
-                        pn = r_pn;
-                        tail_ok = r_tok;
-                        tail_mm = r_tmm;
+                        z = r_z;
+                        dk0 = r_dk0; dk1 = r_dk1; dk2 = r_dk2; dk3 = r_dk3; dk4 = r_dk4;
+                        dm0 = r_dm0; dm1 = r_dm1; dm2 = r_dm2; dm3 = r_dm3; dm4 = r_dm4;
+                        rna_pm = r_rpm;
+                        dna_pm = r_dpm;
+                        dg_base = r_dgb;
                         L = r_L; M = r_M; D = r_D; R = r_R; E = r_E; pid = r_pid; last_chunk = r_last;
                         cand_chunk = r_chunk;
                         if (sink.count_only) load_c0();
@@ -1015,7 +1047,9 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
             emit();
             set4(d, 0u);
         } else if (db == D && rb == R) {
-            // only the diagonal chain to i = 0 remains (see tail_ok)
+            // only the diagonal chain to i = 0 remains, on diagonal b = R
+            WORD tail_ok, tail_mm;
+            diag_of(R, tail_ok, tail_mm);
             const WORD pm = i >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << i) - 1);
             const WORD tm = tail_mm & pm;
             const uint32_t tmm = popc_w<WORD>(tm);
@@ -1244,7 +1278,17 @@ struct BulgePlan {
     bool count_only = false;
     uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
     uint32_t screen_km = 0;       // mismatch states of the budget screen (2, 4, 6; 0: no pattern screened)
+    uint32_t n_diag = 1;          // traceback diagonal-mask pairs per lane (max D + R + 1)
 };
+
+uint32_t max_diagonals(const std::vector<BPat>& pats, const std::vector<BGroup>& g32, const std::vector<BGroup>& g64) {
+    uint32_t m = 1;
+    for (const auto* gs : {&g32, &g64})
+        for (const BGroup& G : *gs)
+            for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++)
+                m = std::max(m, pats[p].D + pats[p].R + 1);
+    return m;
+}
 
 // Mismatch states the budget screen needs for the patterns of the given
 // groups: the smallest of 2, 4, 6 covering their largest M (patterns with
@@ -1326,6 +1370,7 @@ BulgePlan* bulge_plan_create(const Query& q) {
     up(bp->d_pats, bp->pats);
     for (const auto& P : bp->pats) bp->n_cols = std::max(bp->n_cols, P.L + 1);
     bp->screen_km = screen_states(bp->pats, bp->groups32, bp->groups64);
+    bp->n_diag = max_diagonals(bp->pats, bp->groups32, bp->groups64);
     return bp;
 }
 
@@ -1416,8 +1461,11 @@ void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uin
     } else {
         hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(64), 0, st, q_count, q_cap, c.ws.sums + 5);
     }
-    size_t lds32 = (size_t)c.bp->n_cols * 64 * sizeof(uint32_t) * 2;
-    size_t lds64 = (size_t)c.bp->n_cols * 64 * sizeof(uint64_t) * 2;
+    // Myers columns (2 x n_cols rows) and, when some pattern has D + R > 4,
+    // diagonal masks (2 x n_diag rows), 64 lanes
+    const size_t rows = 2 * ((size_t)c.bp->n_cols + (c.bp->n_diag > 5 ? c.bp->n_diag : 0));
+    size_t lds32 = rows * 64 * sizeof(uint32_t);
+    size_t lds64 = rows * 64 * sizeof(uint64_t);
     Sink sink = c.base_sink;
     sink.cand_steps = steps;
     if (c.bp->groups64.empty())
@@ -1846,6 +1894,7 @@ BulgePlan groups_view(const BulgePlan& bp, bool wide, uint32_t min_cols, uint32_
     }
     if (!kept.empty()) (wide ? v.d_g64 : v.d_g32) = (wide ? bp.d_g64 : bp.d_g32) + first;
     v.screen_km = screen_states(bp.pats, v.groups32, v.groups64);
+    v.n_diag = max_diagonals(bp.pats, v.groups32, v.groups64);
     return v;
 }
 

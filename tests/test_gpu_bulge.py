@@ -56,6 +56,8 @@ CASES = [
     ("TTTN", False, 20, 3, 1, 1),
     ("NNGRRT", True, 21, 3, 1, 2),
     ("NRG", True, 28, 5, 2, 1),     # W = 33: 64-bit window kernels
+    ("NGG", True, 20, 3, 3, 2),     # D + R > 4: traceback diagonal masks in LDS, screen skipped (D > 2)
+    ("TTTN", False, 20, 2, 1, 4),   # 5' PAM, R > 2
 ]
 
 
