@@ -571,13 +571,83 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v
     if (i < n) p[i] = v;
 }
 
+// Feasibility table of one candidate for the traceback (patterns with D, R
+// <= 2 and M <= 5): the budget screen's banded recurrence (see
+// bulge_screen_kernel), all mismatch budgets 0..5 as 5-bit fields of one
+// 32-bit word per (DNA, RNA) budget pair, run over the pattern rows.  Row i of the
+// lane's table holds, at bits 3 (3d' + r'), the fewest mismatches (7: none)
+// with which the prefix of length i aligns within d' DNA and r' RNA bulges
+// ending at band position d' + R - r' — the one end a DFS node of row i with
+// remaining budgets (d', r') can have.  ok_b / mis_b: the traceback's
+// diagonal masks of band diagonal b (admissible steps, mismatches).
+template <typename WORD>
+__device__ __forceinline__ void exact_table(uint32_t* xt, uint32_t lane, uint32_t L, uint32_t D, uint32_t R,
+                                            WORD ok0, WORD ok1, WORD ok2, WORD ok3, WORD ok4, WORD mis0, WORD mis1,
+                                            WORD mis2, WORD mis3, WORD mis4, WORD pn) {
+    constexpr int KM = 5;
+    using PK = uint32_t;
+    constexpr PK kRep = [] {
+        PK r = 0;
+        for (int k = 0; k <= KM; k++) r |= (PK)1 << (5 * k);
+        return r;
+    }();
+    constexpr PK kAll = kRep * (PK)31, kNotB0 = kAll & ~kRep, kNotB4 = kAll & ~(kRep << 4);
+    // match steps (no mismatch) and mismatch steps of each band diagonal
+    const WORD mt0 = ok0 & ~mis0, mt1 = ok1 & ~mis1, mt2 = ok2 & ~mis2, mt3 = ok3 & ~mis3, mt4 = ok4 & ~mis4;
+    const WORD ms0 = ok0 & mis0, ms1 = ok1 & mis1, ms2 = ok2 & mis2, ms3 = ok3 & mis3, ms4 = ok4 & mis4;
+    PK S[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) S[d][r] = kAll;
+    xt[lane] = 0u;                          // the empty prefix: no mismatch needed
+    for (uint32_t i = 1; i <= L; i++) {
+        const uint32_t p = i - 1;
+        const uint32_t bmt = (uint32_t)((mt0 >> p) & 1) | (uint32_t)((mt1 >> p) & 1) << 1 |
+                             (uint32_t)((mt2 >> p) & 1) << 2 | (uint32_t)((mt3 >> p) & 1) << 3 |
+                             (uint32_t)((mt4 >> p) & 1) << 4;
+        const uint32_t bms = (uint32_t)((ms0 >> p) & 1) | (uint32_t)((ms1 >> p) & 1) << 1 |
+                             (uint32_t)((ms2 >> p) & 1) << 2 | (uint32_t)((ms3 >> p) & 1) << 3 |
+                             (uint32_t)((ms4 >> p) & 1) << 4;
+        const PK mt = (PK)bmt * kRep, ms = (PK)bms * kRep;
+        const uint32_t n_p = (uint32_t)(pn >> p) & 1u;
+        const uint32_t n_next = i < L ? (uint32_t)(pn >> i) & 1u : 0u;
+        const uint32_t n_prev = i >= 2 ? (uint32_t)(pn >> (i - 2)) & 1u : 0u;
+        const PK rna = (n_p | n_next | n_prev) ? (PK)0 : kNotB4;
+        const PK dna = (i < L && !(n_p & n_next)) ? kNotB0 : (PK)0;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int r = 2; r >= 0; r--) {
+                PK v = (S[d][r] & mt) | ((S[d][r] << 5) & ms);
+                if (r > 0) v |= (S[d][r - 1] >> 1) & rna;
+                if (d > 0) v |= (S[d - 1][r] << 1) & dna;
+                S[d][r] = v & kAll;
+            }
+        uint32_t row = 0xFFFFFFFFu;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                if ((uint32_t)d > D || (uint32_t)r > R) continue;
+                const uint32_t b = (uint32_t)d + R - (uint32_t)r;
+                const PK y = (S[d][r] >> b) & kRep;
+                const uint32_t mk = y ? ((uint32_t)__builtin_ctz(y) * 13u) >> 6 : 7u;   // field index = bit / 5
+                const uint32_t sh = 3u * (3u * (uint32_t)d + (uint32_t)r);
+                row = (row & ~(7u << sh)) | (mk << sh);
+            }
+        xt[i * 64 + lane] = row;
+    }
+}
+
 template <typename WORD>
 __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __restrict__ genome,
                                                           const BCand* __restrict__ queue,
                                                           const uint32_t* __restrict__ q_count, uint32_t q_cap,
                                                           uint32_t* __restrict__ overflow,
                                                           uint32_t* __restrict__ q_next, const BPat* __restrict__ pats,
-                                                          uint32_t n_cols, Sink sink) {
+                                                          uint32_t n_cols, uint32_t has_myers, uint32_t has_exact,
+                                                          Sink sink) {
     // the candidate count comes from device memory so rounds need no host
     // round trip; an overflowed queue is skipped and redone by the host
     const uint32_t n_queue = *q_count;
@@ -596,8 +666,19 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     using CM = WORD;
     constexpr uint32_t kCmBits = sizeof(CM) * 8;
     extern __shared__ unsigned char verify_lds[];
+    // Per lane, rows 0..L of one of two bounds: the Myers columns (cvp/cvn,
+    // D[i][*] by masked popcounts: a unit-cost lower bound) or, for patterns
+    // with D, R <= 2 and M <= 5, the exact feasibility table xt: row i packs,
+    // for each remaining budget pair (d', r') at 3 bits from 3 (3d' + r'),
+    // the fewest mismatches with which pattern prefix i aligns within d'
+    // DNA and r' RNA bulges ending at the one text end a node of row i with
+    // those budgets can have (band position d' + R - r' of the budget
+    // screen's recurrence; 7: none).  A child is pushed only if its prefix
+    // can still be completed within every cap, which the unit bound alone
+    // (cost + D <= M + D + R) does not ensure.
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
-    WORD* cvn = cvp + (size_t)n_cols * 64;
+    WORD* cvn = cvp + (has_myers ? (size_t)n_cols * 64 : 0);
+    uint32_t* xt = reinterpret_cast<uint32_t*>(cvn + (has_myers ? (size_t)n_cols * 64 : 0));
     // Diagonal masks: every diagonal step of the DFS lies on one of the
     // D + R + 1 diagonals jj - i = n - L - D + b (b = D - db + rb).  For
     // diagonal b: over pattern positions p, the steps on it that are
@@ -608,7 +689,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     // Once both bulge budgets are spent every node lies on diagonal R and
     // only diagonal steps remain: such a node resolves its whole chain from
     // that diagonal's masks at once.
-    WORD* dgm = cvn + (size_t)n_cols * 64;
+    WORD* dgm = reinterpret_cast<WORD*>(xt + (has_exact ? (size_t)n_cols * 64 : 0));
     const uint32_t lane = threadIdx.x;
 
     // candidate state
@@ -616,6 +697,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     WORD z = 0;                       // window padding plane
     WORD rna_pm = 0, dna_pm = 0;      // bulges' pattern-side N rules: bit i - 1 / bit i of a node at row i
     int dg_base = 0;                  // n - L - D
+    bool exact = false;               // the candidate uses the feasibility table xt
     uint32_t L = 0, M = 0, D = 0, R = 0, E = 0, pid = 0, last_chunk = 0;
     uint64_t hash_id = 0;
     WORD dk0 = 0, dk1 = 0, dk2 = 0, dk3 = 0, dk4 = 0, dm0 = 0, dm1 = 0, dm2 = 0, dm3 = 0, dm4 = 0;
@@ -706,8 +788,6 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     // of classify_myers_candidate (search.rs:638-671) would reject the leaf.
     auto children = [&]() -> uint32_t {
         if (i == 0) return 0u;
-        const WORD vp1 = cvp[(i - 1) * 64 + lane], vn1 = cvn[(i - 1) * 64 + lane];
-        const WORD vp0 = cvp[i * 64 + lane], vn0 = cvn[i * 64 + lane];
         // every test below is a 0/1 value (straight-line code, no lane branches);
         // indices are clamped where a test does not apply and masked by it
         const uint32_t has_j = jj > 0u;
@@ -720,14 +800,26 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
         WORD okw, misw;
         diag_of((uint32_t)((int)jj - (int)i - dg_base), okw, misw);
         const uint32_t mc = bit(misw, i - 1);
-        const uint32_t diag = bit(okw, i - 1) & (mm + mc <= M) & (1u - (i2 & z2)) &
-                              (cost + mc + lbc(i - 1, vp1, vn1, j1) <= E);
+        // can each child's prefix still be completed?
+        uint32_t f_diag, f_rna, f_dna;
+        if (exact) {
+            const uint32_t xr1 = xt[(i - 1) * 64 + lane], xr0 = xt[i * 64 + lane];
+            const uint32_t q = 3u * (D - db) + (R - rb), rem = M - mm;
+            f_diag = ((xr1 >> (3u * q)) & 7u) + mc <= rem;
+            f_rna = ((xr1 >> (rb < R ? 3u * (q - 1u) : 0u)) & 7u) <= rem;
+            f_dna = ((xr0 >> (db < D ? 3u * (q - 3u) : 0u)) & 7u) <= rem;
+        } else {
+            const WORD vp1 = cvp[(i - 1) * 64 + lane], vn1 = cvn[(i - 1) * 64 + lane];
+            const WORD vp0 = cvp[i * 64 + lane], vn0 = cvn[i * 64 + lane];
+            f_diag = cost + mc + lbc(i - 1, vp1, vn1, j1) <= E;
+            f_rna = cost + 1u + lbc(i - 1, vp1, vn1, jj) <= E;
+            f_dna = cost + 1u + lbc(i, vp0, vn0, j1) <= E;
+        }
+        const uint32_t diag = bit(okw, i - 1) & (mm + mc <= M) & (1u - (i2 & z2)) & f_diag;
         // RNA bulge (skip a pattern base)
-        const uint32_t rna = (rb < R) & bit(rna_pm, i - 1) & (1u - (i2 & z1)) &
-                             (cost + 1u + lbc(i - 1, vp1, vn1, jj) <= E);
+        const uint32_t rna = (rb < R) & bit(rna_pm, i - 1) & (1u - (i2 & z1)) & f_rna;
         // DNA bulge (skip a text base)
-        const uint32_t dna = (db < D) & has_j & bit(dna_pm, i) & (1u - z2) & (1u - z1) &
-                             (cost + 1u + lbc(i, vp0, vn0, j1) <= E);
+        const uint32_t dna = (db < D) & has_j & bit(dna_pm, i) & (1u - z2) & (1u - z1) & f_dna;
         return diag | (rna << 1) | (dna << 2) | (mc << 3);
     };
     // leaf (i == 0): emit the alignment (ops stay in stack order; the host
@@ -824,15 +916,18 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     Masks<WORD> mk;
                     WORD tN, tA, tC, tG, tT;
                     window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
+                    exact = has_exact && D <= 2u && R <= 2u && M <= 5u;
                     WORD VP = 0, VN = 0;
-                    cvp[lane] = 0;
-                    cvn[lane] = 0;
-                    for (uint32_t w = 0, ii = 0; ii < L; w++) {
-                        uint64_t cw = P.cls_bits[w];
-                        for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3) {
-                            myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
-                            cvp[(ii + 1) * 64 + lane] = VP;
-                            cvn[(ii + 1) * 64 + lane] = VN;
+                    if (!exact) {
+                        cvp[lane] = 0;
+                        cvn[lane] = 0;
+                        for (uint32_t w = 0, ii = 0; ii < L; w++) {
+                            uint64_t cw = P.cls_bits[w];
+                            for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3) {
+                                myers_step<WORD>(class_mask<WORD>(mk, (uint32_t)(cw & 7u)), VP, VN);
+                                cvp[(ii + 1) * 64 + lane] = VP;
+                                cvn[(ii + 1) * 64 + lane] = VN;
+                            }
                         }
                     }
                     {
@@ -870,9 +965,12 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                             }
                         }
                     }
+                    if (exact) exact_table<WORD>(xt, lane, L, D, R, dk0, dk1, dk2, dk3, dk4, dm0, dm1, dm2, dm3, dm4,
+                                                 (WORD)P.is_n);
                     i = L; jj = n; d = 0; d_floor = 0; cost = mm = db = rb = 0;
-                    // root admission: padding at the window end, DP bound
-                    busy = !bit(z, n - 1) && lbc(L, VP, VN, n) <= E;
+                    // root admission: padding at the window end, then the bound
+                    busy = !bit(z, n - 1) &&
+                           (exact ? ((xt[L * 64 + lane] >> (3u * (3u * D + R))) & 7u) <= M : lbc(L, VP, VN, n) <= E);
                     if (busy) set4(0, children());
                     else if (sink.cand_steps) sink.cand_steps[qi] = 0;
                 }
@@ -941,6 +1039,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                                    r_kd = xfer32(kd);
                     const uint64_t r_ts = xfer64(ts), r_hash = xfer64(hash_id);
                     const WORD r_z = xferw(z), r_rpm = xferw(rna_pm), r_dpm = xferw(dna_pm);
+                    const uint32_t r_exact = xfer32(exact ? 1u : 0u);
                     const WORD r_dk0 = xferw(dk0), r_dk1 = xferw(dk1), r_dk2 = xferw(dk2), r_dk3 = xferw(dk3),
                                r_dk4 = xferw(dk4), r_dm0 = xferw(dm0), r_dm1 = xferw(dm1), r_dm2 = xferw(dm2),
                                r_dm3 = xferw(dm3), r_dm4 = xferw(dm4);
@@ -953,11 +1052,18 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     for (int w = 0; w < kOpsW; w++) r_ops[w] = xfer64(ops[w]);
                     // the candidate's Myers columns (rows 0..L) and diagonal masks
                     // (rows 0..2(D + R) + 1) into this lane's slots
-                    for (uint32_t r = 0; r <= r_L; r++) {
-                        const WORD vp = cvp[r * 64 + (uint32_t)src], vn = cvn[r * 64 + (uint32_t)src];
-                        if (recv) {
-                            cvp[r * 64 + lane] = vp;
-                            cvn[r * 64 + lane] = vn;
+                    if (r_exact) {
+                        for (uint32_t r = 0; r <= r_L; r++) {
+                            const uint32_t v = xt[r * 64 + (uint32_t)src];
+                            if (recv) xt[r * 64 + lane] = v;
+                        }
+                    } else {
+                        for (uint32_t r = 0; r <= r_L; r++) {
+                            const WORD vp = cvp[r * 64 + (uint32_t)src], vn = cvn[r * 64 + (uint32_t)src];
+                            if (recv) {
+                                cvp[r * 64 + lane] = vp;
+                                cvn[r * 64 + lane] = vn;
+                            }
                         }
                     }
                     for (uint32_t r = 0; r_D + r_R > 4u && r < 2u * (r_D + r_R + 1u); r++) {
@@ -969,6 +1075,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                         d = r_kd; d_floor = r_kd;
                         ts = r_ts; hash_id = r_hash;
                         z = r_z;
+                        exact = r_exact != 0u;
                         dk0 = r_dk0; dk1 = r_dk1; dk2 = r_dk2; dk3 = r_dk3; dk4 = r_dk4;
                         dm0 = r_dm0; dm1 = r_dm1; dm2 = r_dm2; dm3 = r_dm3; dm4 = r_dm4;
                         rna_pm = r_rpm;
@@ -1279,7 +1386,20 @@ struct BulgePlan {
     uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
     uint32_t screen_km = 0;       // mismatch states of the budget screen (2, 4, 6; 0: no pattern screened)
     uint32_t n_diag = 1;          // traceback diagonal-mask pairs per lane (max D + R + 1)
+    bool any_exact = false;       // a pattern with D, R <= 2 and M <= 5 (traceback feasibility table)
+    bool any_myers = false;       // a pattern outside that range (traceback Myers bound)
 };
+
+void exact_kinds(const std::vector<BPat>& pats, const std::vector<BGroup>& g32, const std::vector<BGroup>& g64,
+                 bool& any_exact, bool& any_myers) {
+    any_exact = any_myers = false;
+    for (const auto* gs : {&g32, &g64})
+        for (const BGroup& G : *gs)
+            for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
+                const BPat& P = pats[p];
+                (P.D <= 2 && P.R <= 2 && P.M <= 5 ? any_exact : any_myers) = true;
+            }
+}
 
 uint32_t max_diagonals(const std::vector<BPat>& pats, const std::vector<BGroup>& g32, const std::vector<BGroup>& g64) {
     uint32_t m = 1;
@@ -1371,6 +1491,7 @@ BulgePlan* bulge_plan_create(const Query& q) {
     for (const auto& P : bp->pats) bp->n_cols = std::max(bp->n_cols, P.L + 1);
     bp->screen_km = screen_states(bp->pats, bp->groups32, bp->groups64);
     bp->n_diag = max_diagonals(bp->pats, bp->groups32, bp->groups64);
+    exact_kinds(bp->pats, bp->groups32, bp->groups64, bp->any_exact, bp->any_myers);
     return bp;
 }
 
@@ -1461,19 +1582,23 @@ void launch_verify(RoundCtx& c, const BCand* queue, const uint32_t* q_count, uin
     } else {
         hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(64), 0, st, q_count, q_cap, c.ws.sums + 5);
     }
-    // Myers columns (2 x n_cols rows) and, when some pattern has D + R > 4,
-    // diagonal masks (2 x n_diag rows), 64 lanes
-    const size_t rows = 2 * ((size_t)c.bp->n_cols + (c.bp->n_diag > 5 ? c.bp->n_diag : 0));
-    size_t lds32 = rows * 64 * sizeof(uint32_t);
-    size_t lds64 = rows * 64 * sizeof(uint64_t);
+    // per lane: Myers columns (2 x n_cols rows of the window word) when some
+    // pattern uses the Myers bound, feasibility rows (n_cols u32) when some
+    // uses the exact table, diagonal masks (2 x n_diag rows) when some has
+    // D + R > 4; 64 lanes
+    const size_t wrows = (c.bp->any_myers ? 2 * (size_t)c.bp->n_cols : 0) + (c.bp->n_diag > 5 ? 2 * (size_t)c.bp->n_diag : 0);
+    const size_t xbytes = c.bp->any_exact ? (size_t)c.bp->n_cols * 64 * sizeof(uint32_t) : 0;
+    size_t lds32 = wrows * 64 * sizeof(uint32_t) + xbytes;
+    size_t lds64 = wrows * 64 * sizeof(uint64_t) + xbytes;
+    const uint32_t hm = c.bp->any_myers ? 1u : 0u, hx = c.bp->any_exact ? 1u : 0u;
     Sink sink = c.base_sink;
     sink.cand_steps = steps;
     if (c.bp->groups64.empty())
         hipLaunchKernelGGL(bulge_verify_kernel<uint32_t>, dim3(blocks), dim3(64), lds32, st, c.d_genome, queue, q_count,
-                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, sink);
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, hm, hx, sink);
     else
         hipLaunchKernelGGL(bulge_verify_kernel<uint64_t>, dim3(blocks), dim3(64), lds64, st, c.d_genome, queue, q_count,
-                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, sink);
+                           q_cap, overflow, q_next, c.bp->d_pats, c.bp->n_cols, hm, hx, sink);
     if (steps)
         hipLaunchKernelGGL(tile_steps_reduce_kernel, dim3((q_cap + 255) / 256), dim3(256), 0, st, queue, q_count, q_cap,
                            (const uint32_t*)steps, c.ws.tile_steps);
@@ -1895,6 +2020,7 @@ BulgePlan groups_view(const BulgePlan& bp, bool wide, uint32_t min_cols, uint32_
     if (!kept.empty()) (wide ? v.d_g64 : v.d_g32) = (wide ? bp.d_g64 : bp.d_g32) + first;
     v.screen_km = screen_states(bp.pats, v.groups32, v.groups64);
     v.n_diag = max_diagonals(bp.pats, v.groups32, v.groups64);
+    exact_kinds(bp.pats, v.groups32, v.groups64, v.any_exact, v.any_myers);
     return v;
 }
 
