@@ -297,7 +297,7 @@ void autotune(cof_ctx* ctx, uint64_t sample) {
             const double ms = time_of(t);
             if (ms < best) { best = ms; best_t = t; }
         }
-        for (uint32_t rf : {4u, 16u}) {
+        for (uint32_t rf : {8u, 32u}) {
             cof::BulgeTune t = best_t;
             t.refill = rf;
             const double ms = time_of(t);

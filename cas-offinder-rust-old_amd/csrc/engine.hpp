@@ -103,7 +103,7 @@ struct BulgeTune {
     uint32_t round_tiles = 0;       // tiles per pipeline round; 0 = sized from the first round
     uint32_t verify_per_cu = 24;    // persistent traceback workgroups per CU
     uint32_t queue_cap = 1u << 27;  // candidates per round before it is redone in halves
-    uint32_t refill = 8;            // traceback wave refills once this many lanes are idle
+    uint32_t refill = 16;           // traceback wave refills once this many lanes are idle
     uint32_t screen = 1;            // 1: budget screen between the filter and the traceback
     uint32_t flags = 0;             // COF_FLAG_NO_GRAPH / COF_FLAG_SYNC_ROUNDS
     // predicted traceback work per genome chunk (cost model); null: static

@@ -157,7 +157,7 @@ typedef struct {
                                       32-byte records, min 64); a round that overflows it is redone
                                       in halves */
     uint32_t verify_refill;        /* a traceback wave takes new candidates once this many of its
-                                      64 lanes are idle (1-64, default 8) */
+                                      64 lanes are idle (1-64, default 16) */
     uint32_t plain_hit_cap;        /* plain-scan hit records the device buffer starts with (a
                                       multiple of 64; 0: 2^20).  A scan whose hits do not fit grows
                                       the buffer and runs again */

@@ -223,7 +223,7 @@ def test_autotune_keeps_results(tmp_path):
     t = s.autotune(1 << 20)
     assert t["verify_blocks_per_cu"] in (4, 8, 12, 16, 24)
     assert t["bulge_round_tiles"] in (0, 256, 1024, 4096)
-    assert t["verify_refill"] in (4, 8, 16)
+    assert t["verify_refill"] in (8, 16, 32)
     assert s.tuning() == t
     assert _rows(s, tmp_path) == bwant
     s.close()
