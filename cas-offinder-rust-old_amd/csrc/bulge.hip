@@ -572,10 +572,11 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v
 }
 
 // Feasibility table of one candidate for the traceback (patterns with D, R
-// <= 2 and M <= 5): the budget screen's banded recurrence (see
-// bulge_screen_kernel), all mismatch budgets 0..5 as 5-bit fields of one
-// 32-bit word per (DNA, RNA) budget pair, run over the pattern rows.  Row i of the
-// lane's table holds, at bits 3 (3d' + r'), the fewest mismatches (7: none)
+// <= 2 and M <= 6): the budget screen's banded recurrence (see
+// bulge_screen_kernel), mismatch budgets 0..5 as 5-bit fields of one 32-bit
+// word per (DNA, RNA) budget pair, run over the pattern rows.  Row i of the
+// lane's table holds, at bits 3 (3d' + r'), the fewest mismatches (6: at
+// least 6, or none — exact for every budget below 6, a relaxation at 6)
 // with which the prefix of length i aligns within d' DNA and r' RNA bulges
 // ending at band position d' + R - r' — the one end a DFS node of row i with
 // remaining budgets (d', r') can have.  ok_b / mis_b: the traceback's
@@ -632,7 +633,7 @@ __device__ __forceinline__ void exact_table(uint32_t* xt, uint32_t lane, uint32_
                 if ((uint32_t)d > D || (uint32_t)r > R) continue;
                 const uint32_t b = (uint32_t)d + R - (uint32_t)r;
                 const PK y = (S[d][r] >> b) & kRep;
-                const uint32_t mk = y ? ((uint32_t)__builtin_ctz(y) * 13u) >> 6 : 7u;   // field index = bit / 5
+                const uint32_t mk = y ? ((uint32_t)__builtin_ctz(y) * 13u) >> 6 : 6u;   // field index = bit / 5
                 const uint32_t sh = 3u * (3u * (uint32_t)d + (uint32_t)r);
                 row = (row & ~(7u << sh)) | (mk << sh);
             }
@@ -668,12 +669,12 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
     extern __shared__ unsigned char verify_lds[];
     // Per lane, rows 0..L of one of two bounds: the Myers columns (cvp/cvn,
     // D[i][*] by masked popcounts: a unit-cost lower bound) or, for patterns
-    // with D, R <= 2 and M <= 5, the exact feasibility table xt: row i packs,
+    // with D, R <= 2 and M <= 6, the feasibility table xt: row i packs,
     // for each remaining budget pair (d', r') at 3 bits from 3 (3d' + r'),
     // the fewest mismatches with which pattern prefix i aligns within d'
     // DNA and r' RNA bulges ending at the one text end a node of row i with
     // those budgets can have (band position d' + R - r' of the budget
-    // screen's recurrence; 7: none).  A child is pushed only if its prefix
+    // screen's recurrence; 6: at least 6 or none).  A child is pushed only if its prefix
     // can still be completed within every cap, which the unit bound alone
     // (cost + D <= M + D + R) does not ensure.
     WORD* cvp = reinterpret_cast<WORD*>(verify_lds);
@@ -916,7 +917,7 @@ __global__ __launch_bounds__(64) void bulge_verify_kernel(const uint32_t* __rest
                     Masks<WORD> mk;
                     WORD tN, tA, tC, tG, tT;
                     window_masks<WORD>(X, n, mk, &z, &tN, &tA, &tC, &tG, &tT);
-                    exact = has_exact && D <= 2u && R <= 2u && M <= 5u;
+                    exact = has_exact && D <= 2u && R <= 2u && M <= 6u;
                     WORD VP = 0, VN = 0;
                     if (!exact) {
                         cvp[lane] = 0;
@@ -1386,7 +1387,7 @@ struct BulgePlan {
     uint32_t n_cols = 0;          // Myers columns per lane in the verify kernel (max L + 1)
     uint32_t screen_km = 0;       // mismatch states of the budget screen (2, 4, 6; 0: no pattern screened)
     uint32_t n_diag = 1;          // traceback diagonal-mask pairs per lane (max D + R + 1)
-    bool any_exact = false;       // a pattern with D, R <= 2 and M <= 5 (traceback feasibility table)
+    bool any_exact = false;       // a pattern with D, R <= 2 and M <= 6 (traceback feasibility table)
     bool any_myers = false;       // a pattern outside that range (traceback Myers bound)
 };
 
@@ -1397,7 +1398,7 @@ void exact_kinds(const std::vector<BPat>& pats, const std::vector<BGroup>& g32, 
         for (const BGroup& G : *gs)
             for (uint32_t p = G.pat_begin; p < G.pat_begin + G.pat_count; p++) {
                 const BPat& P = pats[p];
-                (P.D <= 2 && P.R <= 2 && P.M <= 5 ? any_exact : any_myers) = true;
+                (P.D <= 2 && P.R <= 2 && P.M <= 6 ? any_exact : any_myers) = true;
             }
 }
 
