@@ -42,6 +42,8 @@ namespace {
 
 constexpr int kFilterThreads = 256;
 constexpr int kStage = 512;                  // staged filter survivors per wave
+constexpr uint32_t kRunGroups = 8;           // groups a filter workgroup scans together
+constexpr uint32_t kRunPats = 256;           // a run stops growing once it holds this many patterns
 constexpr int kTileEnds = 4096;          // end positions per filter workgroup
 constexpr int kMaxW = 64;                // text window (L + D) supported
 constexpr int kMaxR = 16;
@@ -61,6 +63,7 @@ struct BPat {
     uint64_t fa, fc, fg, ft;  // effective-filter bit4 planes over positions
     uint64_t is_n;            // ASCII N / n positions
     uint64_t hash_id;         // pattern_id_of (checksum key)
+    uint64_t peq[5];          // Myers match masks per text base A, C, G, T and for N (myers.rs:20-50)
     uint32_t L, M, D, R, E, pid;
 };
 
@@ -296,7 +299,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     uint32_t n_groups, const BPat* __restrict__ pats, BCand* __restrict__ queue, uint32_t* __restrict__ q_count,
     uint32_t q_cap, const uint32_t* __restrict__ tile_ids, uint32_t* __restrict__ tile_cand) {
     __shared__ uint32_t gw[kWinWords];
-    __shared__ uint32_t qbits[kTileEnds / 32];
+    __shared__ uint32_t qbits[kRunGroups][kTileEnds / 32];   // quick-check bitmap per group of a run
     __shared__ WORD eqt[kFilterThreads / 64][16][64];
     // per-wave staging of pre-selected ends: (end offset << 20 | pattern -
     // group base); they are re-scored and queued a wave's batch at a time
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     // place and then queued with one reservation.  Once the queue count
     // exceeds q_cap it stops growing (the host redoes such a round), so the
     // 32-bit count cannot wrap.
-    auto flush = [&](const BGroup& G) {
+    auto flush = [&](uint32_t P0) {
         wave_lds_sync();
         uint32_t n_pass = 0;    // survivors so far, at my_stage[0, n_pass) (<= entries read)
         for (uint32_t i0 = 0; i0 < n_st; i0 += 64) {
@@ -329,8 +332,10 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
             uint32_t v = 0;
             if (i < n_st) {
                 v = my_stage[i];
+                const BPat& P = pats[P0 + (v & 0xFFFFFu)];
+                const uint32_t W = P.L + P.D;
                 const uint64_t j = T.j0 + (v >> 20);
-                const uint64_t ts = j + 1 >= T.floor + G.W ? j + 1 - G.W : T.floor;
+                const uint64_t ts = j + 1 >= T.floor + W ? j + 1 - W : T.floor;
                 const uint32_t n = (uint32_t)(j - ts + 1);
                 const uint32_t q = (uint32_t)(ts - base);
                 const uint32_t qw = q >> 3, sh = (q & 7u) * 4u;
@@ -341,7 +346,6 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                 Masks<WORD> mk;
                 window_masks<WORD>(X, n, mk, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
                 const WORD nmask = n >= sizeof(WORD) * 8 ? ~(WORD)0 : (((WORD)1 << n) - 1);
-                const BPat& P = pats[G.pat_begin + (v & 0xFFFFFu)];
                 const uint32_t L = P.L;
                 WORD VP = 0, VN = 0;
                 for (uint32_t w = 0, ii = 0; ii < L; w++) {
@@ -371,84 +375,89 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
             for (uint32_t i = lane; i < n_pass; i += 64) {
                 const uint32_t v = my_stage[i];
                 if (qb + i < q_cap)
-                    queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, G.pat_begin + (v & 0xFFFFFu), T.last_chunk,
-                                          tile_id, 0u};
+                    queue[qb + i] = BCand{T.j0 + (v >> 20), T.floor, P0 + (v & 0xFFFFFu), T.last_chunk, tile_id, 0u};
             }
         }
         n_st = 0;
         wave_lds_sync();
     };
 
-    for (uint32_t gi = 0; gi < n_groups; gi++) {
-        const BGroup& G = groups[gi];
-        // quick PAM check per end position (check_pam_quick, search.rs:703-755)
-        const uint32_t dbr = G.pam_off == 0 ? G.D + 1 : 1, rbr = G.pam_off == 0 ? G.R + 1 : 1;
-        for (uint32_t b0 = 0; b0 < (uint32_t)kTileEnds; b0 += kFilterThreads) {
-            const uint32_t e = b0 + tid;
-            bool ok = false;
-            if (e < T.n) {
-                const uint64_t j = T.j0 + e;
-                const uint64_t jl = j - T.floor;              // batch-local end position
-                if (jl + 1 >= G.L) {
-                    for (uint32_t bd = 0; bd < dbr && !ok; bd++) {
-                        for (uint32_t br = 0; br < rbr && !ok; br++) {
-                            const int64_t span = (int64_t)G.L + bd - br;
-                            if (span <= 0 || span > (int64_t)jl + 1) continue;
-                            const uint64_t as = j + 1 - (uint64_t)span;
-                            bool good = true;
-                            for (uint32_t k = 0; k < G.pam_len && good; k++) {
-                                const uint64_t gp = as + G.pam_off + k;
-                                if (gp >= T.limit) { good = false; break; }
-                                good = pam_cell_ok(lds_nib(gw, (uint32_t)(gp - base)), G.pam_filter[k]);
+    // Groups are taken in runs of consecutive groups (their patterns are
+    // consecutive in the table), up to kRunGroups groups or until the run
+    // holds kRunPats patterns, and a run's patterns share one pool of scan
+    // tasks: small groups (a few guides of one length, PAM and bulge limit)
+    // would otherwise need short segments, whose warm-up dominates.
+    for (uint32_t g0 = 0; g0 < n_groups;) {
+        uint32_t g1 = g0 + 1, np = groups[g0].pat_count, W = groups[g0].W;
+        while (g1 < n_groups && g1 - g0 < kRunGroups && np < kRunPats && groups[g1].pat_begin == groups[g0].pat_begin + np) {
+            np += groups[g1].pat_count;
+            W = max(W, groups[g1].W);
+            g1++;
+        }
+        const uint32_t P0 = groups[g0].pat_begin;
+        // quick PAM check per end position and group (check_pam_quick, search.rs:703-755)
+        for (uint32_t gr = 0; gr < g1 - g0; gr++) {
+            const BGroup& G = groups[g0 + gr];
+            const uint32_t dbr = G.pam_off == 0 ? G.D + 1 : 1, rbr = G.pam_off == 0 ? G.R + 1 : 1;
+            for (uint32_t b0 = 0; b0 < (uint32_t)kTileEnds; b0 += kFilterThreads) {
+                const uint32_t e = b0 + tid;
+                bool ok = false;
+                if (e < T.n) {
+                    const uint64_t j = T.j0 + e;
+                    const uint64_t jl = j - T.floor;              // batch-local end position
+                    if (jl + 1 >= G.L) {
+                        for (uint32_t bd = 0; bd < dbr && !ok; bd++) {
+                            for (uint32_t br = 0; br < rbr && !ok; br++) {
+                                const int64_t span = (int64_t)G.L + bd - br;
+                                if (span <= 0 || span > (int64_t)jl + 1) continue;
+                                const uint64_t as = j + 1 - (uint64_t)span;
+                                bool good = true;
+                                for (uint32_t k = 0; k < G.pam_len && good; k++) {
+                                    const uint64_t gp = as + G.pam_off + k;
+                                    if (gp >= T.limit) { good = false; break; }
+                                    good = pam_cell_ok(lds_nib(gw, (uint32_t)(gp - base)), G.pam_filter[k]);
+                                }
+                                ok = good;
                             }
-                            ok = good;
                         }
                     }
                 }
-            }
-            const uint64_t bal = __ballot(ok);
-            if (lane == 0) {
-                qbits[(b0 >> 5) + wave * 2] = (uint32_t)bal;
-                qbits[(b0 >> 5) + wave * 2 + 1] = (uint32_t)(bal >> 32);
+                const uint64_t bal = __ballot(ok);
+                if (lane == 0) {
+                    qbits[gr][(b0 >> 5) + wave * 2] = (uint32_t)bal;
+                    qbits[gr][(b0 >> 5) + wave * 2 + 1] = (uint32_t)(bal >> 32);
+                }
             }
         }
         __syncthreads();
-        // tasks (pattern, segment of S end positions): segments shrink until
-        // the group's tasks fill the workgroup a few times over, down to 64
-        // ends (128 for 64-bit windows, whose warm-up is longer)
-        const uint32_t np = G.pat_count, W = G.W;
+        // tasks (pattern of the run, segment of S end positions): segments
+        // shrink until the run's tasks fill the workgroup a few times over,
+        // down to 64 ends (128 for 64-bit windows, whose warm-up is longer)
         const uint32_t s_min = sizeof(WORD) == 8 ? 128u : 64u;
         uint32_t nseg = 1;
         while (nseg * np < 4u * kFilterThreads && (uint32_t)kTileEnds / (nseg * 2u) >= s_min) nseg *= 2;
         const uint32_t S = (uint32_t)kTileEnds / nseg;
         const uint32_t n_tasks = np * nseg;
         // every lane of a wave steps through the same number of characters:
-        // W - 1 warm-up characters, then S scored end positions
+        // the run's widest window - 1 warm-up characters (more than a
+        // narrower pattern needs: harmless), then S scored end positions
         for (uint32_t t0 = wave * 64; t0 < n_tasks; t0 += kFilterThreads) {
             const uint32_t task = t0 + lane;
             const bool act = task < n_tasks;
             // consecutive lanes take consecutive patterns of one segment, so
-            // their text and quick-bit reads are LDS broadcasts (segments
-            // lie a multiple of 32 words apart: one bank)
+            // their text reads are LDS broadcasts (segments lie a multiple of
+            // 32 words apart: one bank)
             const uint32_t pi = act ? task % np : 0u;
             const uint32_t e0 = (act ? task / np : 0u) * S;
-            const BPat& P = pats[G.pat_begin + pi];
+            const BPat& P = pats[P0 + pi];
             const uint32_t L = P.L, E = P.E;
-            // Myers class masks of the pattern (myers.rs:20-50) and the
-            // lane's match mask for every text nibble
-            WORD pA = 0, pC = 0, pG = 0, pT = 0, pN = 0;
-            for (uint32_t w = 0, ii = 0; ii < L; w++) {
-                uint64_t cw = P.cls_bits[w];
-                for (uint32_t k = 0; k < 21 && ii < L; k++, ii++, cw >>= 3) {
-                    const uint32_t c = (uint32_t)(cw & 7u);
-                    const WORD b = (WORD)1 << ii;
-                    pA |= (c == 0u || c == 4u) ? b : (WORD)0;
-                    pC |= (c == 1u || c == 4u) ? b : (WORD)0;
-                    pG |= (c == 2u || c == 4u) ? b : (WORD)0;
-                    pT |= (c == 3u || c == 4u) ? b : (WORD)0;
-                    pN |= c == 4u ? b : (WORD)0;
-                }
-            }
+            // the lane's group within the run (its quick-check bitmap)
+            uint32_t gr = 0;
+            for (uint32_t k = 1; k < g1 - g0; k++) gr += P0 + pi >= groups[g0 + k].pat_begin ? 1u : 0u;
+            const uint32_t* qb_row = qbits[gr];
+            // the lane's match mask for every text nibble (myers.rs:20-50)
+            const WORD pA = (WORD)P.peq[0], pC = (WORD)P.peq[1], pG = (WORD)P.peq[2], pT = (WORD)P.peq[3],
+                       pN = (WORD)P.peq[4];
 #pragma unroll
             for (uint32_t c = 0; c < 16u; c++) {
                 WORD m = ((c & 4u) ? pA : (WORD)0) | ((c & 2u) ? pC : (WORD)0) | ((c & 8u) ? pG : (WORD)0) |
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
 #pragma unroll
                 for (int u = 0; u < 8; u++) eqk[u] = eqt[wave][(x >> (4 * u)) & 15u][lane];
                 // quick-check bits of ends e .. e + 7 (e is a multiple of 8)
-                uint32_t hit = (qbits[e >> 5] >> (e & 31u)) & 0xFFu;
+                uint32_t hit = (qb_row[e >> 5] >> (e & 31u)) & 0xFFu;
                 if (e + 8u > e_end) hit &= e_end > e ? ((1u << (e_end - e)) - 1u) : 0u;
                 uint32_t low = 0;
 #pragma unroll
@@ -505,13 +514,14 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
                             __builtin_amdgcn_mbcnt_hi((uint32_t)(pb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pb, 0u));
                         if (sel) my_stage[n_st + off] = ((e + u) << 20) | pi;
                         n_st += (uint32_t)__popcll(pb);
-                        if (n_st > (uint32_t)kStage - 64u) flush(G);
+                        if (n_st > (uint32_t)kStage - 64u) flush(P0);
                     }
                 }
             }
         }
-        if (n_st) flush(G);
+        if (n_st) flush(P0);
         __syncthreads();
+        g0 = g1;
     }
     // candidates of this tile (one workgroup per tile)
     for (int off = 32; off > 0; off >>= 1) my_cand += __shfl_down(my_cand, off);
@@ -1448,7 +1458,12 @@ BulgePlan* bulge_plan_create(const Query& q) {
             if (is_n_char(p.ascii[i])) pf.push_back(kCodec.enc_mixed[(uint8_t)p.eff_filter[i]]);
         by_key[std::make_tuple(p.L, p.D, p.R, ps, pf)].push_back(pi);
     }
+    // 32-bit window groups first, then 64-bit ones, so the patterns of
+    // consecutive groups of one class are consecutive in the table (the
+    // filter scans runs of such groups together)
+    for (int wide = 0; wide < 2; wide++)
     for (auto& kv : by_key) {
+        if ((std::get<0>(kv.first) + std::get<1>(kv.first) > 32) != (wide == 1)) continue;
         BGroup G{};
         G.L = std::get<0>(kv.first);
         G.D = std::get<1>(kv.first);
@@ -1474,6 +1489,12 @@ BulgePlan* bulge_plan_create(const Query& q) {
                 P.ft |= (uint64_t)(f & 1) << i; P.fc |= (uint64_t)((f >> 1) & 1) << i;
                 P.fa |= (uint64_t)((f >> 2) & 1) << i; P.fg |= (uint64_t)((f >> 3) & 1) << i;
                 if (is_n_char(p.ascii[i])) P.is_n |= 1ull << i;
+                const uint64_t bi = 1ull << i;
+                if (cls == 0 || cls == 4) P.peq[0] |= bi;
+                if (cls == 1 || cls == 4) P.peq[1] |= bi;
+                if (cls == 2 || cls == 4) P.peq[2] |= bi;
+                if (cls == 3 || cls == 4) P.peq[3] |= bi;
+                if (cls == 4) P.peq[4] |= bi;
             }
             P.L = p.L; P.M = p.M; P.D = p.D; P.R = p.R; P.E = p.M + p.D + p.R; P.pid = pi;
             P.hash_id = pattern_id_of(p);
