@@ -264,6 +264,54 @@ __device__ __forceinline__ uint32_t popc_w(WORD x) {
     else return (uint32_t)__popc((uint32_t)x);
 }
 
+// Quick PAM check, bit-parallel.  A cell-pass bitmap marks, over the
+// tile's LDS window, the positions whose nibble passes one PAM cell's filter
+// (pam_cell_ok, search.rs:737-743), 32 positions per word from four genome
+// words of SWAR tests; a group's quick bits for 32 ends are then the OR over
+// its bulge shifts of the AND over its cells of the cell bitmaps at the
+// cells' offsets, masked to the ends where the shift's window starts after
+// the batch floor and its cells lie before the PAM limit.
+constexpr int kPassWords = (kWinWords * 8 + 31) / 32;
+constexpr uint32_t kMaxPamCells = 8;      // more cells: the per-end loop
+
+__device__ __forceinline__ uint32_t cell_pass_word(const uint32_t* gw, uint32_t w, uint32_t f) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t gi = 4 * w + (uint32_t)k;
+        const uint32_t x = gi < (uint32_t)kWinWords ? gw[gi] : 0u;
+        const uint32_t nz_x = ((((x & 0x77777777u) + 0x77777777u) | x) >> 3) & 0x11111111u;
+        uint32_t pass;
+        if (f == 0xFu) {
+            pass = nz_x;                                   // any nibble but padding
+        } else {
+            const uint32_t y = x & (f * 0x11111111u);
+            const uint32_t nz_y = ((((y & 0x77777777u) + 0x77777777u) | y) >> 3) & 0x11111111u;
+            const uint32_t isF = x & (x >> 1) & (x >> 2) & (x >> 3) & 0x11111111u;
+            pass = nz_y & ~isF;                            // a shared base, not N
+        }
+        r |= gather_bit0(pass) << (8 * k);
+    }
+    return r;
+}
+
+// 32 bits of bitmap P starting at bit o (bits outside [0, 32 kPassWords) are 0)
+__device__ __forceinline__ uint32_t bits_at(const uint32_t* P, int64_t o) {
+    const int64_t i = o >> 5;
+    const uint32_t r = (uint32_t)o & 31u;
+    const uint32_t lo = (i >= 0 && i < kPassWords) ? P[i] : 0u;
+    const uint32_t hi = (i + 1 >= 0 && i + 1 < kPassWords) ? P[i + 1] : 0u;
+    return __builtin_amdgcn_alignbit(hi, lo, r);
+}
+
+// the ends e in [e0w, e0w + 32) with lo <= e < hi, as bits
+__device__ __forceinline__ uint32_t range_bits(int64_t e0w, int64_t lo, int64_t hi) {
+    uint32_t m = ~0u;
+    if (lo > e0w) m = lo >= e0w + 32 ? 0u : m & (~0u << (uint32_t)(lo - e0w));
+    if (hi < e0w + 32) m = hi <= e0w ? 0u : m & ((1u << (uint32_t)(hi - e0w)) - 1u);
+    return m;
+}
+
 // One step of Myers' bit-parallel edit distance with the pattern in the
 // bit-vector and the text streaming (semi-global: any text start); score is
 // the distance of the whole pattern to the best text suffix so far.  Bits
@@ -300,6 +348,7 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
     uint32_t q_cap, const uint32_t* __restrict__ tile_ids, uint32_t* __restrict__ tile_cand) {
     __shared__ uint32_t gw[kWinWords];
     __shared__ uint32_t qbits[kRunGroups][kTileEnds / 32];   // quick-check bitmap per group of a run
+    __shared__ uint32_t pbits[kMaxPamCells][kPassWords];      // cell-pass bitmaps of the current group
     __shared__ WORD eqt[kFilterThreads / 64][16][64];
     // per-wave staging of pre-selected ends: (end offset << 20 | pattern -
     // group base); they are re-scored and queued a wave's batch at a time
@@ -399,6 +448,33 @@ __global__ __launch_bounds__(kFilterThreads) void bulge_filter_kernel(
         for (uint32_t gr = 0; gr < g1 - g0; gr++) {
             const BGroup& G = groups[g0 + gr];
             const uint32_t dbr = G.pam_off == 0 ? G.D + 1 : 1, rbr = G.pam_off == 0 ? G.R + 1 : 1;
+            if (G.pam_len <= kMaxPamCells) {
+                for (uint32_t k = 0; k < G.pam_len; k++)
+                    for (uint32_t w = tid; w < (uint32_t)kPassWords; w += kFilterThreads)
+                        pbits[k][w] = cell_pass_word(gw, w, G.pam_filter[k]);
+                __syncthreads();
+                const int64_t jb = (int64_t)(T.j0 - base);    // LDS position of end offset 0
+                for (uint32_t w = tid; w < (uint32_t)kTileEnds / 32u; w += kFilterThreads) {
+                    const int64_t e0w = 32 * (int64_t)w;
+                    uint32_t Q = 0;
+                    for (uint32_t bd = 0; bd < dbr; bd++)
+                        for (uint32_t br = 0; br < rbr; br++) {
+                            const int64_t span = (int64_t)G.L + bd - br;
+                            if (span <= 0) continue;
+                            // window start after the batch floor; every cell before the PAM limit
+                            const int64_t hi = G.pam_len ? (int64_t)T.limit + span - (int64_t)G.pam_off -
+                                                               (int64_t)G.pam_len - (int64_t)T.j0
+                                                         : (int64_t)1 << 40;
+                            uint32_t A = range_bits(e0w, (int64_t)T.floor + span - 1 - (int64_t)T.j0, hi);
+                            for (uint32_t k = 0; k < G.pam_len && A; k++)
+                                A &= bits_at(pbits[k], jb + 1 - span + (int64_t)G.pam_off + k + e0w);
+                            Q |= A;
+                        }
+                    qbits[gr][w] = Q & range_bits(e0w, (int64_t)T.floor + G.L - 1 - (int64_t)T.j0, (int64_t)T.n);
+                }
+                __syncthreads();      // the next group's cell bitmaps overwrite these
+                continue;
+            }
             for (uint32_t b0 = 0; b0 < (uint32_t)kTileEnds; b0 += kFilterThreads) {
                 const uint32_t e = b0 + tid;
                 bool ok = false;
