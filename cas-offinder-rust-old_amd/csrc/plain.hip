@@ -125,12 +125,17 @@ __device__ __forceinline__ uint32_t multi_nibbles(uint32_t w) {
     uint32_t o = (w | (w >> 1)) & 0x55555555u;
     return ((x | (x >> 2) | (o & (o >> 2))) & 0x11111111u) << 3;
 }
-// a0, a1 &= (the base with planes (h, l) is in the filter set `code`), 32
-// positions per word (base (h, l) = T C A G -> code bit 0 1 2 3).
-__device__ __forceinline__ void pam_cell2(uint32_t& a0, uint32_t& a1, uint32_t h0, uint32_t l0, uint32_t h1,
-                                          uint32_t l1, uint32_t code) {
-    // branch-free: the set's bits as uniform masks, selected by l then by h
-    // (bitop3 table 0xCA: first operand ? second : third)
+// a0, a1 &= [the base at offset sh of plane words q0 / q1 is in the filter
+// set `code`], 32 starts per word (base (h, l) = T C A G -> code bit 0 1 2 3).
+// Branch-free: the set's bits as uniform masks, selected by l then by h
+// (bitop3 table 0xCA: first operand ? second : third).
+__device__ __forceinline__ void pam_cell(uint32_t& a0, uint32_t& a1, const uint2* gp, uint32_t q0, uint32_t q1,
+                                         uint32_t sh, uint32_t code) {
+    const uint2 x0 = gp[q0], x1 = gp[q0 + 1], y0 = gp[q1], y1 = gp[q1 + 1];
+    const uint32_t h0 = __builtin_amdgcn_alignbit(x1.x, x0.x, sh);
+    const uint32_t l0 = __builtin_amdgcn_alignbit(x1.y, x0.y, sh);
+    const uint32_t h1 = __builtin_amdgcn_alignbit(y1.x, y0.x, sh);
+    const uint32_t l1 = __builtin_amdgcn_alignbit(y1.y, y0.y, sh);
     const uint32_t cT = (code & 1u) ? ~0u : 0u, cC = (code & 2u) ? ~0u : 0u;
     const uint32_t cA = (code & 4u) ? ~0u : 0u, cG = (code & 8u) ? ~0u : 0u;
     a0 &= __builtin_amdgcn_bitop3_b32(h0, __builtin_amdgcn_bitop3_b32(l0, cG, cA, 0xCA),
@@ -161,10 +166,12 @@ __device__ __forceinline__ void tile_sync() {
 // a source adds 0), then row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1
 // and 3, and row_bcast:31 adds lane 31 to rows 2 and 3.
 __device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t v) {
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);   // row_shr:1
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);   // row_shr:2
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    // (bound_ctrl: a lane without a source reads 0, so each shift step is
+    // one v_add_u32 with a DPP operand)
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x118, 0xF, 0xF, true);   // row_shr:8
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
     v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
     return v;
@@ -374,29 +381,17 @@ __device__ __forceinline__ void scan_group(
     uint32_t m8[8];
     if constexpr (kPlainTile) {
         // planes: per PAM cell and 32 starts, two bitop3 selects by the l
-        // plane, one by the h plane and an AND (pam_cell2)
+        // plane, one by the h plane and an AND (pam_cell).  The tile holding
+        // scan_end is a flagged tile (the bit4 branch masks its starts), so
+        // no start here lies past it.
         uint32_t ok[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
         for (uint32_t k = 0; k < G.n_pam; k++) {
             const uint32_t pk = G.pam_pos[k], code = G.pam_mask[k] & 0xFu;   // uniform
-            const uint32_t qo = pk >> 5, sh = pk & 31u;
-            const uint32_t q0 = tid + qo, q1 = kThreads + tid + qo;
-            const uint2 a0 = gp[q0], a1 = gp[q0 + 1], b0 = gp[q1], b1 = gp[q1 + 1];
-            const uint32_t h0 = __builtin_amdgcn_alignbit(a1.x, a0.x, sh);
-            const uint32_t l0 = __builtin_amdgcn_alignbit(a1.y, a0.y, sh);
-            const uint32_t h1 = __builtin_amdgcn_alignbit(b1.x, b0.x, sh);
-            const uint32_t l1 = __builtin_amdgcn_alignbit(b1.y, b0.y, sh);
-            pam_cell2(ok[0], ok[1], h0, l0, h1, l1, code);
+            const uint32_t qo = pk >> 5;
+            pam_cell(ok[0], ok[1], gp, tid + qo, kThreads + tid + qo, pk & 31u, code);
         }
         m8[0] = ok[0];
         m8[1] = ok[1];
-        if (tail) {
-#pragma unroll
-            for (int w = 0; w < 2; w++) {
-                const uint64_t first = s0 + ((uint64_t)w * kThreads + tid) * 32u;
-                if (scan_end <= first) m8[w] = 0;
-                else if (scan_end < first + 32) m8[w] &= (1u << (uint32_t)(scan_end - first)) - 1u;
-            }
-        }
 #pragma unroll
         for (int w = 2; w < 8; w++) m8[w] = 0;
     } else {
@@ -723,7 +718,10 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
         pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
     }
-    int flags = 0;
+    // only the tile holding scan_end masks starts past it; it runs as a
+    // flagged tile (bit 1: the bit4 branch), whose PAM test does the masking
+    const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
+    int flags = tail ? 2 : 0;
 #pragma unroll
     for (int u = 0; u < kThreads / 64; u++) flags |= img_flags[(s0 >> 12) + u];   // uniform
     if (bit4_all || flags) {
@@ -773,8 +771,6 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     // starts each.  Consecutive lanes read consecutive LDS words (no bank
     // conflicts).
 
-    // only the tile holding scan_end masks starts past it
-    const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
     // The group loop, compiled once for tiles of plain bases (plane PAM test,
     // matrix-core or plane compare) and once for flagged tiles (bit4 words),
     // so neither copy carries the other's set-up.
@@ -891,7 +887,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             ph[it] = in ? img_hi[(w0 >> 2) + j] : 0u;
             pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
         }
-        const int flags = img_flags[s0 >> 12];   // uniform
+        // the tile holding scan_end runs as a flagged tile, as in plain_scan_kernel
+        const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
+        const int flags = img_flags[s0 >> 12] | (tail ? 2 : 0);   // uniform
         if (bit4_all || flags) {
             constexpr int kGroups4 = (kTileWords + 3) / 4;
             constexpr int kLoadIt = (kGroups4 + kThreads - 1) / kThreads;
@@ -931,7 +929,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         }
         __syncthreads();
         const int iupac = flags & 1;
-        const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
         if (flags == 0) {
             if (resident) {
                 scan_group<W, 64, true, true>(groups[wave], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0,
