@@ -146,36 +146,53 @@ private:
 };
 
 // Runs fn(begin, end, out, aux) -> count over slices of [0, n) in parallel,
-// twice: first with out == nullptr to count each slice's hits, then with
-// out/aux pointing at the slice's place in the final arrays (hits stay in
-// input order).  The converters' work per raw hit is a bucket lookup and a
-// few compares, so evaluating it twice costs less than staging 64-byte
-// records through a scratch copy.
+// once: slice [b0, b1) writes the hits it keeps to out/aux positions b0,
+// b0 + 1, ... of the final arrays (sized for every raw hit), leaving holes
+// after them.  The converters keep almost every raw hit, so the few holes
+// below the kept total are then filled with kept hits from above it (hit
+// order is unspecified by the interface; out[i] and aux[i] move together).
 template <typename Fn>
 void parallel_convert(size_t n, HitVec& out, AuxVec& aux, Fn fn) {
     if (!n) return;
     const size_t base = out.size();
     Pool& pool = Pool::get();
     const unsigned nt = n < 4096 ? 1u : pool.size() * 4;
-    std::vector<size_t> cnt(nt), off(nt + 1, 0);
-    auto slice = [&](unsigned t) { return std::make_pair(n * t / nt, n * (t + 1) / nt); };
-    auto each = [&](const std::function<void(unsigned)>& f) {
-        if (nt == 1) f(0);
-        else pool.run(nt, f);
-    };
-    each([&](unsigned t) {
-        auto [b0, b1] = slice(t);
-        cnt[t] = fn(b0, b1, nullptr, nullptr);
-    });
-    for (unsigned t = 0; t < nt; t++) off[t + 1] = off[t] + cnt[t];
-    out.resize(base + off[nt]);
-    aux.resize(base + off[nt]);
+    std::vector<size_t> cnt(nt);
+    auto lo_of = [&](unsigned t) { return n * t / nt; };
+    out.resize(base + n);
+    aux.resize(base + n);
     cof_hit* const o = out.data() + base;
     HitAux* const a = aux.data() + base;
-    each([&](unsigned t) {
-        auto [b0, b1] = slice(t);
-        if (cnt[t]) fn(b0, b1, o + off[t], a + off[t]);
-    });
+    auto body = [&](unsigned t) {
+        const size_t b0 = lo_of(t), b1 = lo_of(t + 1);
+        cnt[t] = fn(b0, b1, o + b0, a + b0);
+    };
+    if (nt == 1) body(0);
+    else pool.run(nt, body);
+    size_t kept = 0;
+    for (unsigned t = 0; t < nt; t++) kept += cnt[t];
+    if (kept < n) {
+        // the holes in [0, kept) are as many as the kept hits in [kept, n):
+        // pair them, holes upwards from slice 0, kept hits downwards from the
+        // last slice
+        unsigned st = nt - 1;
+        size_t s_end = lo_of(st) + cnt[st];   // one past the next kept hit to move
+        for (unsigned t = 0; t < nt && lo_of(t) < kept; t++) {
+            const size_t h_end = std::min(lo_of(t + 1), kept);
+            for (size_t h = lo_of(t) + cnt[t]; h < h_end; h++) {
+                while (s_end <= std::max(lo_of(st), kept)) {
+                    if (st == 0) throw Error(COF_E_INVALID, "internal: hit conversion lost count of its holes");
+                    st--;
+                    s_end = lo_of(st) + cnt[st];
+                }
+                s_end--;
+                o[h] = o[s_end];
+                a[h] = a[s_end];
+            }
+        }
+    }
+    out.resize(base + kept);
+    aux.resize(base + kept);
 }
 }  // namespace
 
