@@ -66,8 +66,9 @@ namespace {
 
 // Persistent worker pool for the converters (creating 16 threads per search
 // costs about a millisecond).  After a job the workers spin for a short while
-// before sleeping, so the second pass of a conversion and the next search's
-// conversion start without a condition-variable wake-up.
+// before sleeping, so a conversion that follows soon after (the plain then the
+// bulge hits of one search, or the next search's) starts without a
+// condition-variable wake-up.
 class Pool {
 public:
     static Pool& get() {

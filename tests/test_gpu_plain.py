@@ -74,6 +74,23 @@ def test_synthetic_pam_grid(tmp_path, glen, pam, three_prime, mm):
     assert got == want
 
 
+def test_conversion_drops_across_slices(tmp_path):
+    # 300 chromosomes of 3 kbp tight-packed: the device also scans windows
+    # that run from one chromosome into the next, and the conversion drops
+    # them (search.rs:1456-1475).  With ~25 k raw hits the converter works in
+    # many slices, so the hits dropped near chromosome ends leave holes in
+    # many of them, which are filled from the end of the hit array.
+    import random
+    rng = random.Random(31)
+    guides = [g + "NNN" for g in _guides(rng, 4, 20)]
+    sg = cofpy.SynthGenome(77, [3000] * 300)
+    fa = tmp_path / "g.fa"
+    sg.write_fasta(fa)
+    got, want, st = run_both(tmp_path, fa, "N" * 20 + "NGG", guides, 11)
+    assert st.n_raw_hits >= 4096 and st.n_raw_hits > len(want)
+    assert got == want
+
+
 @pytest.mark.parametrize("batch_nucl", [1 << 17, 3 << 16])
 def test_batch_seams(tmp_path, batch_nucl):
     # chunks of 65520 nt, chromosomes with tiny tails, batches of 2-3 chunks
