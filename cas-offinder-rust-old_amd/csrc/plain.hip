@@ -246,10 +246,10 @@ __device__ __forceinline__ mf_v4f mfma_fp4(const mf_v8i& a, const mf_v8i& b) {
 // One chunk of NB pattern blocks (held in registers) against every window of
 // the list: per 16 windows, NB independent MFMAs and an AND of their score
 // signs; the hit path runs only when some score is positive.
-template <int NB, int THREADS, bool kPre>
+template <int NB, int THREADS>
 __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand,
                                            uint32_t nc, const GroupDev& G, uint32_t bc,
-                                           const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8],
+                                           const uint4* __restrict__ mfrag,
                                            const PlaneRec* __restrict__ prec,
                                            uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
                                            uint32_t lane, uint32_t wave) {
@@ -258,12 +258,8 @@ __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand
     mf_v4i bv[NB];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        if constexpr (kPre) {
-            bv[b] = pre[b];   // the group's blocks, held by the caller (bc == 0)
-        } else {
-            const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
-            bv[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
-        }
+        const uint4 v = mfrag[G.frag_begin + (bc + b) * 64u + lane];
+        bv[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
     }
     // The fragments arrive here (an empty asm that reads them): otherwise the
     // loop waits for them on every iteration, and so also for the hit records
@@ -328,12 +324,11 @@ __device__ __forceinline__ void mfma_chunk(const uint2* gp, const uint16_t* cand
     }
 }
 
-// The group's pattern blocks [pb, pe) in chunks of up to 8 blocks (kPre: the
-// group has at most 8 blocks, all in pre).
-template <int THREADS, bool kPre>
+// The group's pattern blocks [pb, pe) in chunks of up to 8 blocks.
+template <int THREADS>
 __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* cand,
                                              uint32_t nc, const GroupDev& G, uint32_t pb, uint32_t pe,
-                                             const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8],
+                                             const uint4* __restrict__ mfrag,
                                              const PlaneRec* __restrict__ prec,
                                              uint64_t s0, PlainRaw* out, uint32_t* out_count, uint32_t out_cap,
                                              uint32_t lane, uint32_t wave) {
@@ -341,10 +336,9 @@ __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* ca
     const uint32_t blk_hi = (pe - G.pat_begin + 15u) >> 4;
     for (uint32_t bc = blk_lo; bc < blk_hi; bc += 8) {
         const uint32_t nb = min(8u, blk_hi - bc);   // uniform
-#define COF_MFMA_CHUNK(N)                                                                                         \
-    case N:                                                                                                       \
-        mfma_chunk<N, THREADS, kPre>(gp, cand, nc, G, bc, mfrag, pre, prec, s0, out, out_count, out_cap, lane, \
-                                     wave);                                                                        \
+#define COF_MFMA_CHUNK(N)                                                                                    \
+    case N:                                                                                                  \
+        mfma_chunk<N, THREADS>(gp, cand, nc, G, bc, mfrag, prec, s0, out, out_count, out_cap, lane, wave); \
         break;
         switch (nb) {
             COF_MFMA_CHUNK(1) COF_MFMA_CHUNK(2) COF_MFMA_CHUNK(3) COF_MFMA_CHUNK(4)
@@ -359,14 +353,13 @@ __device__ __forceinline__ void mfma_compare(const uint2* gp, const uint16_t* ca
 // for flagged tiles), with a window list of half a tile in cand: the PAM
 // test, the list, and the compare, by a workgroup of THREADS threads
 // (tid < THREADS; wave = tid / 64).  kPlainTile: the tile holds plain bases
-// only (its planes serve); kPre: the group's matrix-core fragments (at most 8
-// blocks) are already in registers (pre).
-template <int W, int THREADS, bool kPlainTile, bool kPre>
+// only (its planes serve).
+template <int W, int THREADS, bool kPlainTile>
 __device__ __forceinline__ void scan_group(
     const GroupDev& G, uint32_t pat_chunk, uint32_t chunk_row, const uint2* gp, const uint32_t* gw, uint16_t* cand,
     uint32_t* wave_sum, uint32_t tid, uint32_t lane, uint32_t wave, uint64_t s0, uint64_t scan_end, bool tail,
     int iupac, const PatRec* __restrict__ pats, const PlaneRec* __restrict__ prec,
-    const uint32_t* __restrict__ full_words, const uint4* __restrict__ mfrag, const mf_v4i (&pre)[8], PlainRaw* out,
+    const uint32_t* __restrict__ full_words, const uint4* __restrict__ mfrag, PlainRaw* out,
     uint32_t* out_count, uint32_t out_cap) {
     constexpr int kThreads = THREADS;
     constexpr int kTile = THREADS * 64;
@@ -516,8 +509,7 @@ __device__ __forceinline__ void scan_group(
         tile_sync<kThreads>();
         // 3. compare windows against the group's patterns
         if (mfma) {
-            mfma_compare<kThreads, kPre>(gp, cand, nc, G, pb, pe, mfrag, pre, prec, s0, out, out_count, out_cap,
-                                         lane, wave);
+            mfma_compare<kThreads>(gp, cand, nc, G, pb, pe, mfrag, prec, s0, out, out_count, out_cap, lane, wave);
             tile_sync<kThreads>();
             continue;
         }
@@ -774,17 +766,16 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     // The group loop, compiled once for tiles of plain bases (plane PAM test,
     // matrix-core or plane compare) and once for flagged tiles (bit4 words),
     // so neither copy carries the other's set-up.
-    const mf_v4i no_pre[8] = {};
     if (flags == 0) {
         for (uint32_t gi = 0; gi < n_groups; gi++)
-            scan_group<W, THREADS, true, false>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid, lane,
-                                                wave, s0, scan_end, tail, iupac, pats, prec, full_words, mfrag,
-                                                no_pre, out, out_count, out_cap);
+            scan_group<W, THREADS, true>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid, lane, wave,
+                                         s0, scan_end, tail, iupac, pats, prec, full_words, mfrag, out, out_count,
+                                         out_cap);
     } else {
         for (uint32_t gi = 0; gi < n_groups; gi++)
-            scan_group<W, THREADS, false, false>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid,
-                                                 lane, wave, s0, scan_end, tail, iupac, pats, prec, full_words, mfrag,
-                                                 no_pre, out, out_count, out_cap);
+            scan_group<W, THREADS, false>(groups[gi], pat_chunk, blockIdx.y, gp, gw, cand, wave_sum, tid, lane, wave,
+                                          s0, scan_end, tail, iupac, pats, prec, full_words, mfrag, out, out_count,
+                                          out_cap);
     }
 }
 
@@ -794,8 +785,7 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
 // first, so consecutive tiles and their halos share that XCD's L2, then
 // the others').  The two waves load a tile's planes (and bit4 words when
 // needed) together and then scan the groups alternately, each with its own
-// window list; when the launch has one matrix-core group per wave, each wave
-// holds its group's pattern fragments in registers for the whole kernel.
+// window list.
 constexpr uint32_t kXcds = 8;
 
 __device__ __forceinline__ uint32_t xcc_id() {
@@ -845,21 +835,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     uint32_t* const out_count = counts + (blockIdx.x & (kHitSegs - 1)) * kSegStride;
     PlainRaw* const out = out_all + (size_t)(blockIdx.x & (kHitSegs - 1)) * seg_cap;
     uint16_t* const cand = lds.cand[wave];
-    // one matrix-core group per wave, in one pass of at most 8 blocks: its
-    // fragments stay in registers
-    constexpr bool resident = false;
-    mf_v4i pre[8] = {};
-    if (resident) {
-        const GroupDev& G = groups[wave];
-        const uint32_t nblk = (G.pat_count + 15u) >> 4;
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            if ((uint32_t)b < nblk) {
-                const uint4 v = mfrag[G.frag_begin + (uint32_t)b * 64u + lane];
-                pre[b] = mf_v4i{(int32_t)v.x, (int32_t)v.y, (int32_t)v.z, (int32_t)v.w};
-            }
-        }
-    }
     uint32_t t = 0, t_end = 0;   // uniform: the tiles claimed and not yet scanned
     for (;;) {
         if (t == t_end) {
@@ -930,21 +905,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         __syncthreads();
         const int iupac = flags & 1;
         if (flags == 0) {
-            if (resident) {
-                scan_group<W, 64, true, true>(groups[wave], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0,
-                                              scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
-                                              out_count, seg_cap);
-            } else {
-                for (uint32_t gi = wave; gi < n_groups; gi += 2)
-                    scan_group<W, 64, true, false>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0,
-                                                   s0, scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
-                                                   out_count, seg_cap);
-            }
+            for (uint32_t gi = wave; gi < n_groups; gi += 2)
+                scan_group<W, 64, true>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0, scan_end,
+                                        tail, iupac, pats, prec, full_words, mfrag, out, out_count, seg_cap);
         } else {
             for (uint32_t gi = wave; gi < n_groups; gi += 2)
-                scan_group<W, 64, false, false>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0,
-                                                scan_end, tail, iupac, pats, prec, full_words, mfrag, pre, out,
-                                                out_count, seg_cap);
+                scan_group<W, 64, false>(groups[gi], 0, 0, lds.gp, lds.gw, cand, nullptr, lane, lane, 0, s0, scan_end,
+                                         tail, iupac, pats, prec, full_words, mfrag, out, out_count, seg_cap);
         }
         __syncthreads();   // both waves are done with the tile before the next one loads
         t++;
