@@ -651,7 +651,7 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 template <int W, int THREADS>
-__global__ __launch_bounds__(THREADS) void plain_scan_kernel(
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(6, 8))) void plain_scan_kernel(
     const uint32_t* __restrict__ genome, uint64_t scan_end, uint64_t first_tile_nib,
     const GroupDev* __restrict__ groups, uint32_t n_groups,
     const PatRec* __restrict__ pats,          // [pattern] compare words + threshold
@@ -662,22 +662,28 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     const uint8_t* __restrict__ img_flags,    // scan image flags per 4096-start unit
     PlainRaw* __restrict__ out_all, uint32_t* __restrict__ counts, uint32_t seg_cap,
     uint32_t pat_chunk,                       // patterns per grid row (0: whole group)
-    uint32_t bit4_all) {                      // a group compares bit4 words: every tile loads them
+    uint32_t bit4_all) {                      // a group compares bit4 words: every tile is a flagged tile
     constexpr int kTile = THREADS * 64;
     constexpr int kTileWords = kTile / 8 + kHaloWords;
     constexpr int kPlaneWords = (kTileWords + 3) / 4 + 1;
     constexpr int kThreads = THREADS;
     // one LDS block, the plane words at offset 0: a window's or PAM cell's
-    // four plane words are one ds_read2_b64 with no base to add
+    // four plane words are one ds_read2_b64 with no base to add.  A tile of
+    // plain bases reads only its planes and a flagged tile only its bit4
+    // words, so the two share their LDS: 6.2 KB per one-wave workgroup, which
+    // with at most 80 VGPRs (waves_per_eu 6; a 16-byte spill) keeps six waves
+    // per SIMD resident instead of five (+1.4% on the 3.1 Gbp headline).
     struct alignas(16) TileLds {
-        uint2 gp[kPlaneWords];      // (hi, lo) plane words
-        uint32_t gw[kTileWords];    // bit4 words (flagged tiles, bit4 groups)
+        union {
+            uint2 gp[kPlaneWords];     // (hi, lo) plane words (tiles of plain bases)
+            uint32_t gw[kTileWords];   // bit4 words (flagged tiles)
+        } w;
         uint16_t cand[kTile / 2];   // one half tile's windows
         uint32_t wave_sum[kThreads / 64];
     };
     __shared__ TileLds lds;
-    uint32_t* const gw = lds.gw;
-    uint2* const gp = lds.gp;
+    uint32_t* const gw = lds.w.gw;
+    uint2* const gp = lds.w.gp;
     uint16_t* const cand = lds.cand;
     uint32_t* const wave_sum = lds.wave_sum;
 
@@ -695,9 +701,9 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
     //    32-position bit planes (base bit 1 = A/G, base bit 0 = C/G), which
     //    serve tiles of plain bases only, and its flags (IUPAC letters; zero
     //    nibbles: masked N, padding) as the OR of its 4096-start units.  The
-    //    plane loads are issued before the flags arrive.  Flagged tiles, and
-    //    every tile when a group of the launch compares bit4 words
-    //    (bit4_all), also load their bit4 words, N-masked (0xF -> 0,
+    //    plane loads are issued before the flags arrive.  Flagged tiles
+    //    (every tile when a group of the launch compares bit4 words:
+    //    bit4_all) load their bit4 words instead, N-masked (0xF -> 0,
     //    search.rs:512-531) where the tile has zero nibbles; a tile without
     //    them holds no N.  Every load of a thread is issued before any is
     //    used.
@@ -711,9 +717,11 @@ __global__ __launch_bounds__(THREADS) void plain_scan_kernel(
         pl[it] = in ? img_lo[(w0 >> 2) + j] : 0u;
     }
     // only the tile holding scan_end masks starts past it; it runs as a
-    // flagged tile (bit 1: the bit4 branch), whose PAM test does the masking
+    // flagged tile (bit 1: the bit4 branch), whose PAM test does the masking.
+    // With bit4_all every tile is flagged: plane groups then compare on the
+    // bit4 words too (same results), which frees the planes' LDS for them.
     const bool tail = s0 + (uint64_t)kTile > scan_end;   // uniform
-    int flags = tail ? 2 : 0;
+    int flags = tail || bit4_all ? 2 : 0;
 #pragma unroll
     for (int u = 0; u < kThreads / 64; u++) flags |= img_flags[(s0 >> 12) + u];   // uniform
     if (bit4_all || flags) {

@@ -195,13 +195,17 @@ def test_plain_queue_kernel(tmp_path):
         in_text = f"{fa}\n{filt}\n" + "".join(f"{p} {m}\n" for p in gs)
         oracle.run(in_text, tmp_path / "ref.tsv", batch_nucl=1 << 22)
         want |= set(cofpy.read_rows(tmp_path / "ref.tsv"))
-    s = cofpy.Search(g1 + g2, [FILT] * len(g1) + [gfilt] * len(g2), [3] * len(g1) + [4] * len(g2),
-                     batch_nucl=1 << 22)
-    s.set_tuning(plain_queue=1)
-    s.load_file(fa)
-    got = _rows(s, tmp_path)
-    assert sorted(got) == sorted(want) and len(got) == len(want)
-    s.close()
+    # the same mixed launch on the persistent scan and on the default grid
+    # scan (where a launch with a bit4 group runs every tile as a flagged
+    # tile, so its plane groups compare bit4 words too)
+    for queue in (1, 0):
+        s = cofpy.Search(g1 + g2, [FILT] * len(g1) + [gfilt] * len(g2), [3] * len(g1) + [4] * len(g2),
+                         batch_nucl=1 << 22)
+        s.set_tuning(plain_queue=queue)
+        s.load_file(fa)
+        got = _rows(s, tmp_path)
+        assert sorted(got) == sorted(want) and len(got) == len(want), queue
+        s.close()
 
 
 def test_autotune_keeps_results(tmp_path):
